@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark: v-Poisson mode-solves/s to a 1e-9 residual on one MI355X.
+
+Workload (the batched configuration the metric is quoted on): nx=1025 (dx=0.02)
+x ny=193 (dy0=2e-4, stretch 1.015), 64 spanwise modes kz=0.1k, 4 multigrid
+levels, synthetic wavepacket right-hand sides (6 packets per mode and part).
+One step = one solve of all 64 modes from the initial guess to
+max|S - L v| < 1e-9 (source assembly, cycles and copy-out included), with
+inputs resident in HBM.
+
+Paths: plain FP64 FAS multigrid (the reference's cycle with the GPU's colour
+line ordering), and the flagship path -- FP32 line smoothing under FP64
+defect correction, started from the learned initial guess when a trained
+network is present (trans_bl_amd/guess_*.bin), else from zero.  `value` is
+the flagship path's whole-job mode-solves/s; the plain path is reported
+beside it.
+
+Prints ONE JSON line.  Also reports the smoother's HBM roofline (live hipEvent
+timing of the level-1 smoother kernel) and a CPU baseline (the CPU oracle's
+reference-ordered FP64 solve on a bounded sample of modes, on the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "v-Poisson mode-solves/s to 1e-9 residual; learned+MG vs plain MG on 1 GPU"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--tol", type=float, default=1e-9)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-modes", type=int, default=0, help="modes in the CPU sample")
+    ap.add_argument("--no-guess", action="store_true", help="ignore a trained network")
+    return ap.parse_args()
+
+
+def cpu_worker(args):
+    import oracle
+    from trans_bl_amd import workload as wl
+    g, f = args
+    p = oracle.make_params(g.nx, g.ny, g.nmodes, g.levels, g.dx, g.dy0, g.stf, g.alpha, g.beta)
+    z = np.zeros_like(f)
+    t = time.perf_counter()
+    _, cyc, res = oracle.poi_uy(p, z, -f, z, 1e-9, 11, 0)
+    return time.perf_counter() - t, cyc.tolist(), res.tolist()
+
+
+def cpu_baseline(grid, f, nmodes, workers):
+    """CPU oracle (reference ordering, FP64) on `nmodes` modes, one process
+    per mode, `workers` at a time."""
+    import dataclasses
+    import multiprocessing as mp
+    jobs = []
+    for k in range(nmodes):
+        g1 = dataclasses.replace(grid, nmodes=1)
+        # mode k of the batch has kz = k*beta: keep it by shifting beta
+        g1 = dataclasses.replace(g1, beta=grid.beta * k if k else 0.0, nmodes=2) if k else g1
+        fk = f[k:k + 1] if not k else np.stack([np.zeros_like(f[k]), f[k]])
+        jobs.append((g1, fk))
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(workers) as pool:
+        out = pool.map(cpu_worker, jobs)
+    wall = time.perf_counter() - t0
+    return wall, out
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the path shards "
+              "by mode with no communication, see the design notes", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import trans_bl_amd as tb
+    from trans_bl_amd import workload as wl
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    grid = wl.BATCH_1025x193x64
+    f = wl.wavepacket_rhs(grid, tag="bench")
+    K = grid.nmodes
+    ctx = tb.Context(grid)
+    fd = torch.from_numpy(f).to("cuda")
+    ud = torch.zeros_like(fd)
+    torch.cuda.synchronize()
+
+    guess = None
+    if not a.no_guess:
+        try:
+            from trans_bl_amd import guess as gmod
+            guess = gmod.load_default(ctx)
+        except Exception as e:  # noqa: BLE001 - report and run without it
+            print(f"learned guess unavailable: {e}", file=sys.stderr)
+            guess = None
+
+    def run(prec, learned, steps, warm):
+        stats = []
+        for it in range(warm + steps):
+            if it == warm:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+            ud.zero_()
+            kw = dict(precision=prec, max_cycles=40, use_graph=1)
+            if learned:
+                cyc, res, st = guess.solve(ctx, fd, ud, a.tol, **kw)
+            else:
+                cyc, res, st = ctx.solve_rhs_dev(fd.data_ptr(), ud.data_ptr(), a.tol, guess=1, **kw)
+            if it >= warm:
+                stats.append((cyc.copy(), res.copy(), st))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return wall, stats
+
+    wall64, st64 = run(0, False, a.steps, a.warmup)
+    v64 = ud.clone()
+    flag_learned = guess is not None
+    wallm, stm = run(1, flag_learned, a.steps, a.warmup)
+    vm = ud.clone()
+
+    def summar(stats):
+        cyc = np.array([s[0] for s in stats])
+        res = np.array([s[1] for s in stats])
+        return {"cycles_mean": float(cyc.mean()), "cycles_max": int(cyc.max()),
+                "residual_max": float(res.max()),
+                "converged": bool(np.all(res < a.tol)),
+                "device_ms_mean": float(np.mean([s[2]["device_ms"] for s in stats]))}
+
+    s64, sm = summar(st64), summar(stm)
+    rate64 = K * a.steps / wall64
+    ratem = K * a.steps / wallm
+    diff = float(torch.linalg.vector_norm(vm - v64) / torch.linalg.vector_norm(v64))
+
+    # ---- roofline of the dominant kernel: level-1 line smoother ----------
+    # algorithmic bytes per sweep: read v, read S, write v over every point of
+    # the level (3 passes), complex FP64 = 16 B, FP32 = 8 B
+    n1 = grid.nx * grid.ny * K
+    roof = {}
+    for prec, esize, name in ((1, 8, "smooth_phase_kernel<float> level 1"),
+                              (0, 16, "smooth_phase_kernel<double> level 1")):
+        sweeps = 20
+        ctx.time_smoother(prec, 1, 2)            # warm
+        ms, launches = ctx.time_smoother(prec, 1, sweeps)
+        per_launch_s = ms / 1e3 / launches
+        bytes_per_launch = 3 * esize * n1 / 4.0
+        gbs = bytes_per_launch / per_launch_s / 1e9
+        roof[prec] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                      "kernel": name, "avg_launch_us": round(per_launch_s * 1e6, 2),
+                      "bytes_per_launch": int(bytes_per_launch),
+                      "lanes_per_line": ctx.line_shape(prec, 1)}
+
+    cpu = None
+    if not a.no_cpu:
+        ncpu = min(len(os.sched_getaffinity(0)), 16)
+        nm = a.cpu_modes or ncpu
+        wall_c, out = cpu_baseline(grid, f, nm, ncpu)
+        cpu = {"value": round(nm / wall_c, 4), "unit": "mode-solves/s", "cores": ncpu,
+               "kind": "port",
+               "sample": f"{nm} of the 64 modes (k=0..{nm - 1}) of the same workload, CPU "
+                         f"oracle FP64 with the reference's lexicographic line order, one "
+                         f"process per mode, {ncpu} at a time; {wall_c:.1f} s",
+               "cycles": [c for o in out for c in o[1]][:nm]}
+
+    line = {
+        "metric": METRIC,
+        "value": round(ratem, 2),
+        "unit": "mode-solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wallm / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 smoothing + f64 defect correction",
+        "data": "synthetic wavepacket RHS (6 packets/mode/part, seeded)",
+        "config": {"workload": "v-Poisson 1025x193x64 modes to 1e-9",
+                   "nx": grid.nx, "ny": grid.ny, "modes": K, "levels": grid.levels,
+                   "dx": grid.dx, "dy0": grid.dy0, "stf": grid.stf, "beta": grid.beta,
+                   "initial_guess": "learned" if flag_learned else "zero",
+                   "parallelism": "1 GPU, all modes batched"},
+        "flagship": sm,
+        "plain_fp64": dict(s64, value=round(rate64, 2),
+                           ms_per_step=round(wall64 / a.steps * 1e3, 3)),
+        "speedup_vs_plain_fp64": round(ratem / rate64, 3),
+        "flagship_vs_fp64_rel_l2": diff,
+        "roofline": roof[1],
+        "roofline_fp64_smoother": roof[0],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

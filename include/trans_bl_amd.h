@@ -1,0 +1,181 @@
+/*
+ * trans_bl_amd -- MI355X (gfx950) implementation of trans_bl's per-spanwise-
+ * mode v-Poisson multigrid and compact finite-difference derivatives.
+ *
+ * C ABI: plain pointers, sizes and small structs; no C++ or torch types.
+ * Complex fields are interleaved (re, im) doubles in the reference's Fortran
+ * order (ptsx, jmax, kfour): x index fastest, then y, then mode
+ * (comm.var:2-5).  Every call returns a tbl_status; the library keeps no
+ * global state besides per-context allocations.
+ *
+ * Reference interfaces replaced (file:line into the reference tree):
+ *   tbl_coefs_generate   pre_processing/coef_generation.f90:18-64 (coefs.bin)
+ *   tbl_create           create_ctes main_par.f:1828-1892 + derivs_k
+ *                        derivs_par.f:6-29 + the /coefs/ read main_par.f:852-873
+ *   tbl_poi_uy(_dev)     poi_uy(dwzdx, erromax)       poisson_par.f:6-64
+ *   tbl_derparx(_dev)    derparx(ddx, fc)             derivs_par.f:80-95
+ *   tbl_derparxx(_dev)   derparxx(d2dx2, fc)          derivs_par.f:98-113
+ *   tbl_dery(_dev)       dery(ddy, fc)                derivs_par.f:116-129
+ *   tbl_deryy(_dev)      deryy(d2dy2, fc)             derivs_par.f:164-177
+ *   tbl_deryfv(_dev)     deryfv(ddy, fc)              derivs_par.f:148-161
+ */
+#ifndef TRANS_BL_AMD_H
+#define TRANS_BL_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TBL_MAX_LEVELS 8
+
+typedef enum tbl_status {
+    TBL_OK = 0,
+    TBL_ERR_ARG = 1,      /* null pointer or out-of-range argument */
+    TBL_ERR_GRID = 2,     /* grid not usable by the multigrid (verifica, main_par.f:60-134) */
+    TBL_ERR_HIP = 3,      /* a HIP runtime call failed */
+    TBL_ERR_ALLOC = 4,    /* device or host allocation failed */
+    TBL_ERR_STATE = 5,    /* call not valid in the context's current state */
+    TBL_ERR_NOCONV = 6    /* reserved: returned only when the caller asks for it */
+} tbl_status;
+
+/* Runtime form of the par.for PARAMETERs the path reads (par.for:41-108). */
+typedef struct tbl_grid {
+    int nx;          /* ptsx  (one rank)           */
+    int ny;          /* jmax                       */
+    int nmodes;      /* kfour                      */
+    int levels;      /* msh                        */
+    double dx;       /* dx                         */
+    double dy0;      /* dy  (first wall spacing)   */
+    double stf;      /* stf (geometric y stretch)  */
+    double alpha;    /* alpha (Robin top BC)       */
+    double beta;     /* beta  (kz = (k-1)*beta)    */
+    double fac_y;    /* fac_y                      */
+} tbl_grid;
+
+/* COMMON /coefs/ (comm.coef:1-29), coefficient index order as in coefs.bin.
+   Poisson tables are [level][coef] (Fortran sp_poi_coef(7, msh) etc.). */
+typedef struct tbl_coefs {
+    double fp_fd_e[6], sp_fd[9], cp_fd[8], pp_fd[9], lp_fd[7];
+    double fp_sd[8], sp_sd[10], cp_sd[8], pp_sd[10], lp_sd[8];
+    double sp_poi[TBL_MAX_LEVELS][7];
+    double cp_poi[TBL_MAX_LEVELS][8];
+    double pp_poi[TBL_MAX_LEVELS][4];
+    double lp_poi[TBL_MAX_LEVELS][5];
+} tbl_coefs;
+
+/* Solver options.  Zero-initialised fields take the reference's values. */
+typedef struct tbl_solve_opts {
+    int precision;    /* 0: FP64 FAS cycle (poisson_par.f:33-50);
+                         1: FP32 line smoothing under FP64 defect correction */
+    int max_cycles;   /* 0 -> 11, the reference cap (poisson_par.f:33) */
+    int nu1;          /* pre-smoothing sweeps, 0 -> 3 (itera, poisson_par.f:24) */
+    int nu2;          /* post-smoothing sweeps, 0 -> 1 (poisson_par.f:49) */
+    int ncoarse;      /* coarsest-level sweeps, 0 -> 60 (20*itera, :45) */
+    int use_graph;    /* 1: replay the V-cycle as a captured hipGraph */
+    int guess;        /* 0: start from uy (the reference's warm start,
+                            poisson_par.f:157-161); 1: start from zero;
+                         2: learned initial guess (tbl_guess_*), falling back to
+                            uy when its residual is not smaller */
+    int reserved[5];
+} tbl_solve_opts;
+
+/* Per-call statistics (the reference prints k, vcycles-1, sum_err per mode,
+   poisson_par.f:58-60). */
+typedef struct tbl_solve_stats {
+    int cycles_max;       /* largest completed V-cycle count over modes */
+    int cycles_total;     /* sum over modes */
+    int nconverged;       /* modes whose last checked residual < erromax */
+    int fallback;         /* modes where the learned guess was rejected */
+    double residual_max;  /* largest last-checked max |S - L v| */
+    double guess_residual_max; /* max residual of the initial guess (guess=2) */
+    double device_ms;     /* GPU time of the solve (hipEvents) */
+} tbl_solve_stats;
+
+typedef struct tbl_ctx tbl_ctx;
+
+/* Version string of the library build. */
+const char *tbl_version(void);
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char *tbl_last_error(void);
+
+/* Coefficient tables for grid stretch stf and `levels` multigrid levels:
+   the offline generator (pre_processing/derivative.f90:1762-1882) as a host
+   routine (Poisson schemes in 80-bit long double).  Replaces running `coef`
+   and reading coefs.bin. */
+int tbl_coefs_generate(double stf, int levels, tbl_coefs *out);
+
+/* Create a context on the current HIP device: checks the grid like
+   verifica (main_par.f:60-134), builds the multigrid metrics (create_ctes),
+   the derivative LHS (derivs_k) and the line-LU factors, and allocates the
+   device work arrays.  coefs == NULL uses tbl_coefs_generate(grid->stf). */
+int tbl_create(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx **out);
+int tbl_destroy(tbl_ctx *ctx);
+
+/* Stream for all following work of the context (hipStream_t as void*;
+   NULL = the null stream). */
+int tbl_set_stream(tbl_ctx *ctx, void *hip_stream);
+
+/* Wait until all work the context queued on its stream has finished. */
+int tbl_synchronize(tbl_ctx *ctx);
+
+/* poi_uy on host arrays (poisson_par.f:6-64): wx, dwzdx, uy are complex
+   (nx, ny, nmodes); uy is the initial guess on entry and holds the solution
+   on i=2..nx-1, j=2..ny on exit (other entries untouched).  cycles and
+   residual (length nmodes, may be NULL) receive each mode's completed
+   V-cycle count and last checked max residual.  Synchronous. */
+int tbl_poi_uy(tbl_ctx *ctx, const double *wx, const double *dwzdx, double *uy,
+               double erromax, const tbl_solve_opts *opts, int *cycles,
+               double *residual, tbl_solve_stats *stats);
+
+/* Same on device pointers (inputs resident in HBM).  wx may be NULL (treated
+   as zero).  Returns after the solve has finished on the context's stream. */
+int tbl_poi_uy_dev(tbl_ctx *ctx, const double *d_wx, const double *d_dwzdx,
+                   double *d_uy, double erromax, const tbl_solve_opts *opts,
+                   int *cycles, double *residual, tbl_solve_stats *stats);
+
+/* Solve L v = B f for every mode (the v-Poisson with wx = 0, dwzdx = -f):
+   same as tbl_poi_uy_dev with d_f = -dwzdx.  Device pointers. */
+int tbl_solve_rhs_dev(tbl_ctx *ctx, const double *d_f, double *d_uy, double erromax,
+                      const tbl_solve_opts *opts, int *cycles, double *residual,
+                      tbl_solve_stats *stats);
+
+/* Compact derivatives on host arrays, complex (nx, ny, nmodes). */
+int tbl_derparx(tbl_ctx *ctx, const double *fc, double *ddx);
+int tbl_derparxx(tbl_ctx *ctx, const double *fc, double *d2dx2);
+int tbl_dery(tbl_ctx *ctx, const double *fc, double *ddy);
+int tbl_deryy(tbl_ctx *ctx, const double *fc, double *d2dy2);
+int tbl_deryfv(tbl_ctx *ctx, const double *fc, double *ddy);
+
+/* Device-pointer forms, asynchronous on the context's stream.
+   kind: 0 derparx, 1 derparxx, 2 dery, 3 deryy, 4 deryfv. */
+int tbl_deriv_dev(tbl_ctx *ctx, int kind, const double *d_fc, double *d_out);
+
+/* Apply the fine-grid operator: out = S - L v on i=2..nx-1, j=2..ny
+   (defect with si=-1, poisson_par.f:543-793) for every mode, with
+   S = B(kb*wx - dwzdx) (function_source, poisson_par.f:111-164).  Device
+   pointers; out entries outside that range are set to zero.  Also returns
+   each mode's max |out| in residual (host, may be NULL). */
+int tbl_residual_dev(tbl_ctx *ctx, const double *d_wx, const double *d_dwzdx,
+                     const double *d_uy, double *d_out, double *residual);
+
+/* Line-solve shape of one multigrid level's smoother: `lanes` (16, 32 or
+   64) lanes of a wavefront share one y-line.  prec 0 = FP64 smoother,
+   1 = FP32 smoother.  Chosen by the autotuner; no reference counterpart
+   (the reference solves each line sequentially, penta_par.f:367-404). */
+int tbl_set_line_shape(tbl_ctx *ctx, int prec, int level, int lanes);
+int tbl_get_line_shape(tbl_ctx *ctx, int prec, int level);
+
+/* Measurement hook (no reference counterpart): run `sweeps` line-relaxation
+   sweeps of one level on the context's current multigrid state, bracketed
+   by hipEvents on the context's stream; *ms receives the elapsed time and
+   *launches the number of smoother kernel launches it made.  prec 0 times
+   the FP64 smoother, 1 the FP32 one. */
+int tbl_time_smoother(tbl_ctx *ctx, int prec, int level, int sweeps, double *ms,
+                      int *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRANS_BL_AMD_H */

@@ -1,0 +1,90 @@
+/*
+ * CPU oracle for the trans_bl v-Poisson / compact-derivative hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: a plain-C restatement of the reference's FP64
+ * algorithm, used by tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline as the checker.  The product never links or calls it.
+ *
+ * Arrays are Fortran order with the first index fastest, complex values
+ * interleaved (re, im) -- the reference's complex*16 (ptsx, jmax, kfour)
+ * layout (comm.var:2-5).  Indices below are 0-based; comments quote the
+ * reference's 1-based indices.
+ */
+#ifndef TBL_ORACLE_H
+#define TBL_ORACLE_H
+
+#include <complex.h>
+
+#define OR_MAXLVL 8
+
+typedef double _Complex or_cplx;
+
+typedef struct or_params {
+    int nx, ny, nk, msh;          /* ptsx, jmax, kfour, msh (par.for:71-108) */
+    double dx, dy0, stf;          /* dx, dy, stf (par.for:71-83) */
+    double alpha, beta, fac_y;    /* par.for:41, 54, 63 */
+    /* COMMON /coefs/ (comm.coef:1-29) */
+    double fp_fd_e[6], sp_fd[9], cp_fd[8], pp_fd[9], lp_fd[7];
+    double fp_sd[8], sp_sd[10], cp_sd[8], pp_sd[10], lp_sd[8];
+    double sp_poi[OR_MAXLVL][7], cp_poi[OR_MAXLVL][8];
+    double pp_poi[OR_MAXLVL][4], lp_poi[OR_MAXLVL][5];
+} or_params;
+
+/* create_ctes (main_par.f:1828-1892).  qdy: ny; qdy2: msh*ny ([lvl][j]). */
+typedef struct or_metrics {
+    int ptsx[OR_MAXLVL], ptsy[OR_MAXLVL];
+    double dy0[OR_MAXLVL], stf[OR_MAXLVL], dx2[OR_MAXLVL];
+    double qdy[1024];
+    double qdy2[OR_MAXLVL][1024];
+} or_metrics;
+
+int or_create_ctes(const or_params *p, or_metrics *m);
+
+/* ---- compact derivatives (derivs_par.f:80-177) --------------------------- */
+int or_derparx (const or_params *p, const or_cplx *f, or_cplx *out);
+int or_derparxx(const or_params *p, const or_cplx *f, or_cplx *out);
+int or_dery    (const or_params *p, const or_cplx *f, or_cplx *out);
+int or_deryy   (const or_params *p, const or_cplx *f, or_cplx *out);
+int or_deryfv  (const or_params *p, const or_cplx *f, or_cplx *out);
+
+/* ---- banded LU (penta_par.f:308-404) -------------------------------------- */
+/* a: n x 5 row-major bands (offsets -2..+2); al: n x 2 multipliers */
+void or_band5_factor(double *a, int n, double *al);
+void or_band5_solve(const double *a, int n, const double *al, or_cplx *rhs);
+
+/* ---- Poisson pieces on one level (compact nx_l x ny_l planes) ------------- */
+/* out(i,j) = S(i,j) + si * (L v)(i,j) for i=2..nx_l-1, j=2..ny_l
+   (defect, poisson_par.f:543-793).  Other entries of out are untouched. */
+int or_defect(const or_params *p, const or_metrics *m, int lvl, int k,
+              const or_cplx *v, const or_cplx *S, or_cplx *out, double si);
+/* one-mode source assembly (function_source, poisson_par.f:111-164) */
+int or_source(const or_params *p, int k, const or_cplx *wx, const or_cplx *dwzdx,
+              or_cplx *S);
+/* itera line-relaxation sweeps.  order: 0 = reference lexicographic
+   (poisson_par.f:191-221); 1 = colour order used by the GPU smoother. */
+int or_sor(const or_params *p, const or_metrics *m, int lvl, int k,
+           or_cplx *v, const or_cplx *S, int itera, int order);
+
+/* full v-Poisson solve for every mode (poi_uy, poisson_par.f:6-64).
+   uy in/out (nx,ny,nk); cycles[k] = completed V-cycles (printed value),
+   res[k] = last max residual checked. */
+int or_poi_uy(const or_params *p, const or_cplx *wx, const or_cplx *dwzdx,
+              or_cplx *uy, double erromax, int max_cycles, int order,
+              int *cycles, double *res);
+
+/* finest-level operator for one mode, all points (i=2..nx-1, j=2..ny get
+   (L v); others 0) -- used to assemble the dense system in tests. */
+int or_apply_L(const or_params *p, int k, const or_cplx *v, or_cplx *out);
+
+/* same for an arbitrary level (compact nx_l x ny_l) */
+int or_apply_L_level(const or_params *p, int lvl, int k, const or_cplx *v,
+                     or_cplx *out);
+
+/* restriction, prolongation+correction on one mode (poisson_par.f:796-942) */
+int or_restrict(const or_params *p, const or_metrics *m, int lvl,
+                const or_cplx *vf, const or_cplx *rf, or_cplx *vc, or_cplx *wc,
+                or_cplx *Sc);
+int or_correct_prolong(const or_params *p, const or_metrics *m, int lvl,
+                       or_cplx *vc, const or_cplx *wc, or_cplx *vf);
+
+#endif

@@ -1,0 +1,290 @@
+"""trans_bl_amd: MI355X-native v-Poisson multigrid and compact derivatives.
+
+Python mirror of the reference's hot-path interface (poi_uy, derparx,
+derparxx, dery, deryy, deryfv; poisson_par.f:6, derivs_par.f:80-177) over
+the C ABI of ``libtrans_bl_amd.so`` (include/trans_bl_amd.h).  Host entry
+points take numpy complex128 arrays of shape (nmodes, ny, nx) -- the C view
+of the reference's Fortran (ptsx, jmax, kfour) arrays; ``*_dev`` entry points
+take device pointers (for instance ``torch.Tensor.data_ptr()`` of a complex128
+CUDA tensor of that shape).
+
+The HIP library is required: there is no CPU fallback.  Importing works on a
+machine without a GPU (for the host-only routines); device calls fail loudly
+if the library is missing or no GPU is present.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .workload import Grid  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtrans_bl_amd.so")
+MAX_LEVELS = 8
+
+STATUS = {0: "OK", 1: "bad argument", 2: "grid not usable by the multigrid", 3: "HIP error",
+          4: "allocation failed", 5: "invalid state", 6: "not converged"}
+
+
+class TblError(RuntimeError):
+    pass
+
+
+class CGrid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nmodes", C.c_int), ("levels", C.c_int),
+                ("dx", C.c_double), ("dy0", C.c_double), ("stf", C.c_double),
+                ("alpha", C.c_double), ("beta", C.c_double), ("fac_y", C.c_double)]
+
+
+class CCoefs(C.Structure):
+    _fields_ = [
+        ("fp_fd_e", C.c_double * 6), ("sp_fd", C.c_double * 9), ("cp_fd", C.c_double * 8),
+        ("pp_fd", C.c_double * 9), ("lp_fd", C.c_double * 7),
+        ("fp_sd", C.c_double * 8), ("sp_sd", C.c_double * 10), ("cp_sd", C.c_double * 8),
+        ("pp_sd", C.c_double * 10), ("lp_sd", C.c_double * 8),
+        ("sp_poi", (C.c_double * 7) * MAX_LEVELS), ("cp_poi", (C.c_double * 8) * MAX_LEVELS),
+        ("pp_poi", (C.c_double * 4) * MAX_LEVELS), ("lp_poi", (C.c_double * 5) * MAX_LEVELS),
+    ]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("max_cycles", C.c_int), ("nu1", C.c_int),
+                ("nu2", C.c_int), ("ncoarse", C.c_int), ("use_graph", C.c_int),
+                ("guess", C.c_int), ("reserved", C.c_int * 5)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("cycles_max", C.c_int), ("cycles_total", C.c_int), ("nconverged", C.c_int),
+                ("fallback", C.c_int), ("residual_max", C.c_double),
+                ("guess_residual_max", C.c_double), ("device_ms", C.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+# every extern "C" symbol include/trans_bl_amd.h declares
+EXPORTS = [
+    "tbl_version", "tbl_last_error", "tbl_coefs_generate", "tbl_create", "tbl_destroy",
+    "tbl_set_stream", "tbl_synchronize", "tbl_poi_uy", "tbl_poi_uy_dev", "tbl_solve_rhs_dev", "tbl_derparx",
+    "tbl_derparxx", "tbl_dery", "tbl_deryy", "tbl_deryfv", "tbl_deriv_dev", "tbl_residual_dev",
+    "tbl_set_line_shape", "tbl_get_line_shape", "tbl_time_smoother",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libtrans_bl_amd.so (fails loudly when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TblError(f"{LIB_PATH} is missing: run `python -m trans_bl_amd.build` first")
+    L = C.CDLL(LIB_PATH)
+    vp, ip, dp = C.c_void_p, C.c_int, C.c_double
+    L.tbl_version.restype = C.c_char_p
+    L.tbl_last_error.restype = C.c_char_p
+    L.tbl_coefs_generate.argtypes = [dp, ip, C.POINTER(CCoefs)]
+    L.tbl_create.argtypes = [C.POINTER(CGrid), C.POINTER(CCoefs), C.POINTER(vp)]
+    L.tbl_destroy.argtypes = [vp]
+    L.tbl_set_stream.argtypes = [vp, vp]
+    L.tbl_synchronize.argtypes = [vp]
+    sig = [vp, vp, vp, vp, dp, C.POINTER(SolveOpts), vp, vp, C.POINTER(SolveStats)]
+    L.tbl_poi_uy.argtypes = sig
+    L.tbl_poi_uy_dev.argtypes = sig
+    L.tbl_solve_rhs_dev.argtypes = [vp, vp, vp, dp, C.POINTER(SolveOpts), vp, vp,
+                                    C.POINTER(SolveStats)]
+    for n in ("tbl_derparx", "tbl_derparxx", "tbl_dery", "tbl_deryy", "tbl_deryfv"):
+        getattr(L, n).argtypes = [vp, vp, vp]
+    L.tbl_deriv_dev.argtypes = [vp, ip, vp, vp]
+    L.tbl_residual_dev.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.tbl_set_line_shape.argtypes = [vp, ip, ip, ip]
+    L.tbl_get_line_shape.argtypes = [vp, ip, ip]
+    L.tbl_time_smoother.argtypes = [vp, ip, ip, ip, C.POINTER(dp), C.POINTER(ip)]
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().tbl_last_error().decode(errors="replace")
+        raise TblError(f"{what} failed: {STATUS.get(rc, rc)} {msg}".strip())
+
+
+def coefs_generate(stf: float, levels: int) -> dict:
+    """Coefficient tables (the reference's coefs.bin content) from the
+    library's host generator."""
+    c = CCoefs()
+    _check(lib().tbl_coefs_generate(stf, levels, C.byref(c)), "tbl_coefs_generate")
+    return coefs_to_dict(c, levels)
+
+
+def coefs_to_dict(c: CCoefs, levels: int) -> dict:
+    out = {}
+    for name, _ in CCoefs._fields_:
+        arr = np.ctypeslib.as_array(getattr(c, name)).copy()
+        if name.endswith("_poi"):
+            arr = arr[:levels]
+        out[name] = arr
+    return out
+
+
+def coefs_from_dict(d: dict) -> CCoefs:
+    c = CCoefs()
+    for name, _ in CCoefs._fields_:
+        if name not in d:
+            continue
+        src = np.asarray(d[name], dtype=np.float64)
+        dst = np.ctypeslib.as_array(getattr(c, name))
+        if name.endswith("_poi"):
+            dst[:src.shape[0], :src.shape[1]] = src
+        else:
+            dst[:src.shape[0]] = src
+    return c
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One grid's solver state on the current GPU (runtime par.for + the
+    COMMON blocks the hot path reads)."""
+
+    def __init__(self, grid: Grid, coefs: dict | None = None):
+        self.grid = grid
+        g = CGrid(grid.nx, grid.ny, grid.nmodes, grid.levels, grid.dx, grid.dy0, grid.stf,
+                  grid.alpha, grid.beta, grid.fac_y)
+        self._cc = coefs_from_dict(coefs) if coefs is not None else None
+        h = C.c_void_p()
+        _check(lib().tbl_create(C.byref(g), C.byref(self._cc) if self._cc else None,
+                                C.byref(h)), "tbl_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tbl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def shape(self):
+        return (self.grid.nmodes, self.grid.ny, self.grid.nx)
+
+    def _arr(self, a, name):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        if a.shape != self.shape:
+            raise ValueError(f"{name} must have shape {self.shape}, got {a.shape}")
+        return a
+
+    @staticmethod
+    def opts(precision=0, max_cycles=11, nu1=0, nu2=0, ncoarse=0, use_graph=0, guess=0):
+        return SolveOpts(precision, max_cycles, nu1, nu2, ncoarse, use_graph, guess)
+
+    # ---- the reference's subroutines on host arrays ----------------------
+    def poi_uy(self, wx, dwzdx, uy, erromax, **kw):
+        """poi_uy(dwzdx, erromax) with the COMMON inputs wx, uy explicit.
+        Returns (uy_new, cycles, residuals, stats)."""
+        wx = None if wx is None else self._arr(wx, "wx")
+        dz = self._arr(dwzdx, "dwzdx")
+        out = np.array(self._arr(uy, "uy"), copy=True)
+        cyc = np.zeros(self.grid.nmodes, np.int32)
+        res = np.zeros(self.grid.nmodes, np.float64)
+        st = SolveStats()
+        o = self.opts(**kw)
+        _check(lib().tbl_poi_uy(self.h, None if wx is None else _ptr(wx), _ptr(dz), _ptr(out),
+                                erromax, C.byref(o), _ptr(cyc), _ptr(res), C.byref(st)),
+               "tbl_poi_uy")
+        return out, cyc, res, st.as_dict()
+
+    def _deriv_host(self, fn, f):
+        f = self._arr(f, "fc")
+        out = np.zeros_like(f)
+        _check(getattr(lib(), fn)(self.h, _ptr(f), _ptr(out)), fn)
+        return out
+
+    def derparx(self, f):
+        return self._deriv_host("tbl_derparx", f)
+
+    def derparxx(self, f):
+        return self._deriv_host("tbl_derparxx", f)
+
+    def dery(self, f):
+        return self._deriv_host("tbl_dery", f)
+
+    def deryy(self, f):
+        return self._deriv_host("tbl_deryy", f)
+
+    def deryfv(self, f):
+        return self._deriv_host("tbl_deryfv", f)
+
+    # ---- device-pointer entry points --------------------------------------
+    def set_stream(self, handle):
+        _check(lib().tbl_set_stream(self.h, C.c_void_p(handle) if handle else None),
+               "tbl_set_stream")
+
+    def synchronize(self):
+        _check(lib().tbl_synchronize(self.h), "tbl_synchronize")
+
+    def solve_rhs_dev(self, f_ptr, uy_ptr, erromax, **kw):
+        cyc = np.zeros(self.grid.nmodes, np.int32)
+        res = np.zeros(self.grid.nmodes, np.float64)
+        st = SolveStats()
+        o = self.opts(**kw)
+        _check(lib().tbl_solve_rhs_dev(self.h, C.c_void_p(f_ptr), C.c_void_p(uy_ptr), erromax,
+                                       C.byref(o), _ptr(cyc), _ptr(res), C.byref(st)),
+               "tbl_solve_rhs_dev")
+        return cyc, res, st.as_dict()
+
+    def poi_uy_dev(self, wx_ptr, dwzdx_ptr, uy_ptr, erromax, **kw):
+        cyc = np.zeros(self.grid.nmodes, np.int32)
+        res = np.zeros(self.grid.nmodes, np.float64)
+        st = SolveStats()
+        o = self.opts(**kw)
+        _check(lib().tbl_poi_uy_dev(self.h, C.c_void_p(wx_ptr) if wx_ptr else None,
+                                    C.c_void_p(dwzdx_ptr), C.c_void_p(uy_ptr), erromax,
+                                    C.byref(o), _ptr(cyc), _ptr(res), C.byref(st)),
+               "tbl_poi_uy_dev")
+        return cyc, res, st.as_dict()
+
+    def deriv_dev(self, kind, in_ptr, out_ptr):
+        _check(lib().tbl_deriv_dev(self.h, kind, C.c_void_p(in_ptr), C.c_void_p(out_ptr)),
+               "tbl_deriv_dev")
+
+    def residual_dev(self, wx_ptr, dwzdx_ptr, uy_ptr, out_ptr):
+        res = np.zeros(self.grid.nmodes, np.float64)
+        _check(lib().tbl_residual_dev(self.h, C.c_void_p(wx_ptr) if wx_ptr else None,
+                                      C.c_void_p(dwzdx_ptr), C.c_void_p(uy_ptr),
+                                      C.c_void_p(out_ptr), _ptr(res)), "tbl_residual_dev")
+        return res
+
+    def set_line_shape(self, prec, level, lanes):
+        _check(lib().tbl_set_line_shape(self.h, prec, level, lanes), "tbl_set_line_shape")
+
+    def time_smoother(self, prec, level, sweeps):
+        """(elapsed ms, kernel launches) of `sweeps` sweeps on one level."""
+        ms = C.c_double()
+        n = C.c_int()
+        _check(lib().tbl_time_smoother(self.h, prec, level, sweeps, C.byref(ms), C.byref(n)),
+               "tbl_time_smoother")
+        return ms.value, n.value
+
+    def line_shape(self, prec, level):
+        return lib().tbl_get_line_shape(self.h, prec, level)
+
+
+DERIV_KINDS = {"x": 0, "xx": 1, "y": 2, "yy": 3, "yfv": 4}

@@ -1,0 +1,57 @@
+// Context object behind the tbl_ctx handle of the C ABI.
+#pragma once
+
+#include <vector>
+
+#include "tbl_host.h"
+#include "tbl_kernels.h"
+
+struct tbl_ctx {
+    tbl_grid g{};
+    tbl_coefs c{};
+    tbl::Metrics m;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    int K = 0, L = 0;
+
+    std::vector<tbl::Level> lv;
+    // FP64 multigrid state per level: v, source, FAS injected copy, residual
+    std::vector<tbl::cx<double> *> V, S, W, R;
+    // FP32 correction-scheme state per level (mixed precision)
+    std::vector<tbl::cx<float> *> E32, S32, R32;
+    std::vector<double *> qdy2_d;
+    std::vector<float *> qdy2_f;
+    std::vector<double *> lu_d;   // [K][2][5][ld] per level
+    std::vector<float *> lu_f;
+    std::vector<tbl::LevelArgs<double>> la_d;
+    std::vector<tbl::LevelArgs<float>> la_f;
+    std::vector<int> G64, G32;    // lanes per line for each level
+
+    int *d_active = nullptr, *d_cycles = nullptr, *d_nactive = nullptr, *d_cycle = nullptr;
+    double *d_resmax = nullptr, *d_lastres = nullptr;
+    int *h_nactive = nullptr;     // pinned
+
+    // derivatives
+    double *d_qdy = nullptr, *d_thomas = nullptr;
+    tbl::DerivArgs dargs{};
+
+    // staging for the host-pointer entry points (user layout)
+    double *d_u0 = nullptr, *d_u1 = nullptr, *d_u2 = nullptr;
+
+    // captured V-cycle graphs (precision 0 / 1) and their parameters
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_key[2][5] = {{0}};
+    double graph_tol[2] = {0, 0};
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    // learned initial guess (optional)
+    struct Guess *guess = nullptr;
+};
+
+namespace tbl {
+int solve_dev(tbl_ctx *ctx, const double *wx, const double *dz, double *uy, int direct_rhs,
+              double tol, const tbl_solve_opts *opts, int *cycles, double *residual,
+              tbl_solve_stats *stats);
+}

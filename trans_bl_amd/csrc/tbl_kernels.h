@@ -1,0 +1,79 @@
+// Kernel argument blocks and launcher declarations of trans_bl_amd.
+#pragma once
+
+#include "tbl_common.h"
+
+namespace tbl {
+
+// Per-level constants of the Poisson operator (poisson_par.f:229-793)
+template <typename T>
+struct LevelArgs {
+    int nx, ny, ld, K, jblocks;
+    size_t plane;                 // nx * ld
+    T sp[7], cp[8], pp[4], lp[5]; // sp_poi_coef(:,l) ... lp_poi_coef(:,l)
+    T qx;                         // 1 / (12 dx_l^2 fac_y)
+    T dx2, fac_y, alpha, beta;
+    T sqq_top;                    // sqrt(v_qdy2(ny_l-2, l))
+    const T *qdy2;                // v_qdy2(:, l) on the device
+};
+
+inline Level make_level(int nx, int ny, double dx2)
+{
+    Level L;
+    L.nx = nx;
+    L.ny = ny;
+    L.ld = round_up(ny, 8);
+    L.dx2 = dx2;
+    L.plane = static_cast<size_t>(nx) * L.ld;
+    return L;
+}
+
+int line_rows_for(int G, int ny);
+
+template <typename T>
+int smooth_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
+                  const int *active, int G, hipStream_t st);
+template <typename T>
+int smooth_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps,
+                    T rf, const int *active, int G, hipStream_t st);
+template <typename T, typename TO>
+void defect(const LevelArgs<T> &L, const cx<T> *V, const cx<T> *S, cx<TO> *out, T si,
+            const int *active, double *resmax, hipStream_t st);
+template <typename T>
+void restrict_level(const Level &F, const Level &Cl, int K, const cx<T> *Vf, const cx<T> *Rf,
+                    cx<T> *Vc, cx<T> *Wc, cx<T> *Sc, int fas, const int *active, hipStream_t st);
+template <typename T>
+void prolong_level(const Level &F, const Level &Cl, int K, cx<T> *Vf, const cx<T> *Vc,
+                   const cx<T> *Wc, int fas, const int *active, hipStream_t st);
+void gather(int nx, int ny, int K, const Level &F, const double *wx, const double *dz,
+            const double *uy, double beta, cx<double> *G, cx<double> *V, int direct_rhs,
+            hipStream_t st);
+void source(const LevelArgs<double> &L, const cx<double> *G, cx<double> *S, hipStream_t st);
+void scatter(int nx, int ny, int K, const Level &F, const cx<double> *V, double *uy,
+             hipStream_t st);
+void decide(int K, const double *resmax, double tol, int cycle, int *active, int *cycles,
+            double *lastres, int *nactive, hipStream_t st);
+void fill(double *p, size_t n, double v, hipStream_t st);
+void fill_int(int *p, int n, int v, hipStream_t st);
+void update(const Level &F, int K, cx<double> *V, const cx<float> *E, const int *active,
+            hipStream_t st);
+template <typename TA, typename TB>
+void convert(size_t n, const cx<TA> *a, cx<TB> *b, int zero, hipStream_t st);
+
+// ---- derivatives (user layout) ------------------------------------------------
+struct DerivArgs {
+    int nx, ny, K;
+    double dx, dxx;
+    // y-scheme coefficients (COMMON /coefs/)
+    double fp_e[6], sp_fd[9], cp_fd[8], pp_fd[9], lp_fd[7];
+    double fp_sd[8], sp_sd[10], cp_sd[8], pp_sd[10], lp_sd[8];
+    const double *qdy;     // v_qdy
+    const double *qdy2;    // v_qdy2(:,1)
+    // Thomas factors per scheme: a, bet, gam (kind-major, stride nmax)
+    const double *thomas;  // [kind][3][nmax]
+    int nmax;
+};
+
+int deriv(const DerivArgs &A, int kind, const double *f, double *out, hipStream_t st);
+
+}  // namespace tbl

@@ -1,0 +1,766 @@
+// Multigrid kernels of the v-Poisson solve (gfx950).
+//
+// Internal layout of every level: complex values, mode-major, then x column,
+// then y contiguous with padded column stride ld (tbl::Level).  A y-line
+// (one x column of one mode) is therefore one contiguous run of ny values.
+//
+// Line relaxation (sor, poisson_par.f:167-226): every line of a colour phase
+// is solved by a group of G lanes of one wavefront; lane t owns rows
+// [t*R, t*R+R).  The pentadiagonal system (LU factors from band5_poi,
+// penta_par.f:308-364) is solved with a partitioned scan: each lane composes
+// the affine map of its rows, the maps are combined with a log2(G)-step
+// shuffle scan, and each lane replays its rows from the incoming state
+// (forward substitution, then back substitution; banbk5_poi,
+// penta_par.f:367-404).
+#include "tbl_kernels.h"
+
+namespace tbl {
+
+// ---------------------------------------------------------------------------
+// x-direction stencils (poisson_par.f:343-540 and 543-793)
+// ---------------------------------------------------------------------------
+
+// Off-diagonal part used on the smoother's right-hand side.  side -1/+1 is
+// the left/right border column (rhs_border1/2), 0 an interior column
+// (rhs_center, grouped form on rows ny-1 and ny).
+template <typename T>
+__device__ __forceinline__ cx<T> xoff(const cx<T> *__restrict__ col, int ld, int j,
+                                      int side, bool grouped)
+{
+    // col points at column i of the mode's plane; neighbours at +/- ld
+    if (side < 0) {
+        return T(10) * col[j - ld] - T(4) * col[j + ld] + T(14) * col[j + 2 * ld]
+               - T(6) * col[j + 3 * ld] + col[j + 4 * ld];
+    }
+    if (side > 0) {
+        return T(10) * col[j + ld] - T(4) * col[j - ld] + T(14) * col[j - 2 * ld]
+               - T(6) * col[j - 3 * ld] + col[j - 4 * ld];
+    }
+    if (grouped)
+        return -(col[j + 2 * ld] + col[j - 2 * ld]) + T(16) * (col[j + ld] + col[j - ld]);
+    return T(-1) * col[j - 2 * ld] + T(16) * col[j - ld] + T(16) * col[j + ld]
+           - T(1) * col[j + 2 * ld];
+}
+
+// Full x stencil of the operator (defect).
+template <typename T>
+__device__ __forceinline__ cx<T> xfull(const cx<T> *__restrict__ col, int ld, int j,
+                                       int side, bool grouped)
+{
+    if (side < 0)
+        return T(10) * col[j - ld] - T(15) * col[j] - T(4) * col[j + ld]
+               + T(14) * col[j + 2 * ld] - T(6) * col[j + 3 * ld] + col[j + 4 * ld];
+    if (side > 0)
+        return T(10) * col[j + ld] - T(15) * col[j] - T(4) * col[j - ld]
+               + T(14) * col[j - 2 * ld] - T(6) * col[j - 3 * ld] + col[j - 4 * ld];
+    if (grouped)
+        return T(-1) * (col[j - 2 * ld] + col[j + 2 * ld])
+               + T(16) * (col[j - ld] + col[j + ld]) - T(30) * col[j];
+    return T(-1) * col[j - 2 * ld] + T(16) * col[j - ld] - T(30) * col[j]
+           + T(16) * col[j + ld] - T(1) * col[j + 2 * ld];
+}
+
+// ---------------------------------------------------------------------------
+// one y-line relaxation by a group of G lanes
+// ---------------------------------------------------------------------------
+template <typename T, int G, int R>
+__device__ __forceinline__ void relax_line(const LevelArgs<T> &L, cx<T> *col,
+                                           const cx<T> *__restrict__ Scol, const T *__restrict__ lu,
+                                           int side, T rf, int lane)
+{
+    const int ny = L.ny, ld = L.ld;
+    const int j0 = lane * R;
+    const T qx = L.qx;
+    const T *l1 = lu, *l2 = lu + ld, *u1 = lu + 2 * ld, *u2 = lu + 3 * ld, *pv = lu + 4 * ld;
+
+    cx<T> vo[R], y[R];
+    // x off-diagonal terms for rows j0-1 .. j0+R
+    cx<T> X[R + 2];
+#pragma unroll
+    for (int r = -1; r <= R; ++r) {
+        const int j = j0 + r;
+        cx<T> x = {T(0), T(0)};
+        if (j >= 0 && j < ny) x = xoff<T>(col, ld, j, side, side == 0 && j >= ny - 2);
+        X[r + 1] = x;
+    }
+    // right-hand sides (rhs_border1/2, rhs_center)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        cx<T> v = {T(0), T(0)}, rhs = {T(0), T(0)};
+        if (j < ny) {
+            v = col[j];
+            if (j == 0) {
+                rhs = v;
+            } else if (j == 1) {
+                rhs = Scol[j] - qx * (L.sp[2] * X[r + 2] + L.sp[1] * X[r + 1]);
+            } else if (j <= ny - 3) {
+                rhs = Scol[j] - qx * (L.cp[0] * X[r] + L.cp[1] * X[r + 1] + L.cp[2] * X[r + 2]);
+            } else {
+                rhs = Scol[j] - qx * X[r + 1];
+            }
+        }
+        vo[r] = v;
+        y[r] = rhs;
+    }
+
+    // ---- forward substitution: y_j = (r_j - l2_j y_{j-2}) - l1_j y_{j-1}
+    // state (a, b) = (y_{j-2}, y_{j-1}); map M_j = [[0,1],[-l2,-l1]]
+    T p00 = 1, p01 = 0, p10 = 0, p11 = 1;
+    cx<T> c0 = {0, 0}, c1 = {0, 0};
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        const T a1 = (j < ny) ? l1[j] : T(0), a2 = (j < ny) ? l2[j] : T(0);
+        const T n10 = -a2 * p00 - a1 * p10, n11 = -a2 * p01 - a1 * p11;
+        p00 = p10; p01 = p11; p10 = n10; p11 = n11;
+        const cx<T> nc1 = (y[r] - a2 * c0) - a1 * c1;
+        c0 = c1; c1 = nc1;
+    }
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+        const T q00 = __shfl_up(p00, d, G), q01 = __shfl_up(p01, d, G);
+        const T q10 = __shfl_up(p10, d, G), q11 = __shfl_up(p11, d, G);
+        const cx<T> e0 = shfl_up(c0, d, G), e1 = shfl_up(c1, d, G);
+        if (lane >= d) {
+            const cx<T> nc0 = p00 * e0 + p01 * e1 + c0;
+            const cx<T> nc1 = p10 * e0 + p11 * e1 + c1;
+            c0 = nc0; c1 = nc1;
+            const T m00 = p00 * q00 + p01 * q10, m01 = p00 * q01 + p01 * q11;
+            const T m10 = p10 * q00 + p11 * q10, m11 = p10 * q01 + p11 * q11;
+            p00 = m00; p01 = m01; p10 = m10; p11 = m11;
+        }
+    }
+    {
+        cx<T> a = shfl_up(c0, 1, G), b = shfl_up(c1, 1, G);
+        if (lane == 0) { a = {0, 0}; b = {0, 0}; }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int j = j0 + r;
+            const T a1 = (j < ny) ? l1[j] : T(0), a2 = (j < ny) ? l2[j] : T(0);
+            const cx<T> yy = (y[r] - a2 * a) - a1 * b;
+            a = b; b = yy;
+            y[r] = yy;
+        }
+    }
+
+    // ---- back substitution: x_j = ((y_j - u1_j x_{j+1}) - u2_j x_{j+2}) / piv_j
+    // state (a, b) = (x_{j+1}, x_{j+2}); map [[-u1/p, -u2/p],[1, 0]]
+    p00 = 1; p01 = 0; p10 = 0; p11 = 1;
+    c0 = {0, 0}; c1 = {0, 0};
+#pragma unroll
+    for (int r = R - 1; r >= 0; --r) {
+        const int j = j0 + r;
+        const bool ok = j < ny;
+        const T pj = ok ? pv[j] : T(1), w1 = ok ? u1[j] : T(0), w2 = ok ? u2[j] : T(0);
+        const T m0 = -w1 / pj, m1 = -w2 / pj;
+        const T n00 = m0 * p00 + m1 * p10, n01 = m0 * p01 + m1 * p11;
+        p10 = p00; p11 = p01; p00 = n00; p01 = n01;
+        const cx<T> nc0 = y[r] / pj + m0 * c0 + m1 * c1;
+        c1 = c0; c0 = nc0;
+    }
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+        const T q00 = __shfl_down(p00, d, G), q01 = __shfl_down(p01, d, G);
+        const T q10 = __shfl_down(p10, d, G), q11 = __shfl_down(p11, d, G);
+        const cx<T> e0 = shfl_down(c0, d, G), e1 = shfl_down(c1, d, G);
+        if (lane + d < G) {
+            const cx<T> nc0 = p00 * e0 + p01 * e1 + c0;
+            const cx<T> nc1 = p10 * e0 + p11 * e1 + c1;
+            c0 = nc0; c1 = nc1;
+            const T m00 = p00 * q00 + p01 * q10, m01 = p00 * q01 + p01 * q11;
+            const T m10 = p10 * q00 + p11 * q10, m11 = p10 * q01 + p11 * q11;
+            p00 = m00; p01 = m01; p10 = m10; p11 = m11;
+        }
+    }
+    {
+        cx<T> a = shfl_down(c0, 1, G), b = shfl_down(c1, 1, G);
+        if (lane == G - 1) { a = {0, 0}; b = {0, 0}; }
+#pragma unroll
+        for (int r = R - 1; r >= 0; --r) {
+            const int j = j0 + r;
+            const bool ok = j < ny;
+            const T pj = ok ? pv[j] : T(1), w1 = ok ? u1[j] : T(0), w2 = ok ? u2[j] : T(0);
+            const cx<T> xx = ((y[r] - w1 * a) - w2 * b) / pj;
+            b = a; a = xx;
+            y[r] = xx;
+        }
+    }
+    // relaxation v <- v + rf (v* - v) on rows 2..ny (poisson_par.f:215-218)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        if (j >= 1 && j < ny) col[j] = vo[r] + rf * (y[r] - vo[r]);
+    }
+}
+
+// column of line c in a phase: phase 0 = the two border columns (Fortran 2
+// and nx-1), phases 1..3 = interior colours i = 3+p-1, 3+p-1+3, ... (Fortran)
+__device__ __forceinline__ int phase_column(int phase, int c, int nx)
+{
+    if (phase == 0) return c == 0 ? 1 : nx - 2;
+    return 2 + (phase - 1) + 3 * c;
+}
+
+__host__ __device__ inline int phase_lines(int phase, int nx)
+{
+    if (phase == 0) return 2;
+    const int first = 2 + (phase - 1), last = nx - 3;
+    return first > last ? 0 : (last - first) / 3 + 1;
+}
+
+template <typename T, int G, int R>
+__global__ void __launch_bounds__(256) smooth_phase_kernel(LevelArgs<T> L, cx<T> *V,
+                                                           const cx<T> *S, const T *lu,
+                                                           int phase, T rf, const int *active)
+{
+    const int nlines = phase_lines(phase, L.nx);
+    const int gid = (blockIdx.x * 256 + threadIdx.x) / G;
+    const int lane = threadIdx.x % G;
+    const int k = gid / nlines;
+    if (k >= L.K) return;
+    if (active && !active[k]) return;
+    const int c = gid - k * nlines;
+    const int i = phase_column(phase, c, L.nx);
+    const int side = (phase == 0) ? (c == 0 ? -1 : 1) : 0;
+    const size_t base = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * L.ld;
+    const T *lut = lu + (static_cast<size_t>(k) * 2 + (side != 0)) * 5 * L.ld;
+    relax_line<T, G, R>(L, V + base, S + base, lut, side, rf, lane);
+}
+
+// Coarsest level: one workgroup per mode keeps the whole level in LDS and
+// runs every sweep (20*itera, poisson_par.f:45) without leaving the CU.
+template <typename T, int G, int R>
+__global__ void __launch_bounds__(1024) smooth_coarsest_kernel(LevelArgs<T> L, cx<T> *V,
+                                                               const cx<T> *S, const T *lu,
+                                                               int sweeps, T rf,
+                                                               const int *active)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *sv = reinterpret_cast<cx<T> *>(smem_raw);
+    const int k = blockIdx.x;
+    if (active && !active[k]) return;
+    const size_t plane = L.plane;
+    cx<T> *gv = V + static_cast<size_t>(k) * plane;
+    const cx<T> *gs = S + static_cast<size_t>(k) * plane;
+    for (size_t e = threadIdx.x; e < plane; e += blockDim.x) sv[e] = gv[e];
+    __syncthreads();
+    const int groups = blockDim.x / G;
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+    const T *luc = lu + (static_cast<size_t>(k) * 2) * 5 * L.ld;
+    const T *lub = luc + 5 * L.ld;
+    for (int it = 0; it < sweeps; ++it) {
+        for (int phase = 0; phase < 4; ++phase) {
+            const int nl = phase_lines(phase, L.nx);
+            for (int c = grp; c < nl; c += groups) {
+                const int i = phase_column(phase, c, L.nx);
+                const int side = (phase == 0) ? (c == 0 ? -1 : 1) : 0;
+                relax_line<T, G, R>(L, sv + static_cast<size_t>(i) * L.ld,
+                                    gs + static_cast<size_t>(i) * L.ld, side ? lub : luc, side,
+                                    rf, lane);
+            }
+            __syncthreads();
+        }
+    }
+    for (size_t e = threadIdx.x; e < plane; e += blockDim.x) gv[e] = sv[e];
+}
+
+// ---------------------------------------------------------------------------
+// defect (poisson_par.f:543-793): out = S + si * L v on i=2..nx-1, j=2..ny.
+// With resmax != NULL the max |out| of each mode is accumulated (the
+// verifica_poi / get_max_value test, poisson_par.f:67-108, 945-963).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ cx<T> apply_L(const LevelArgs<T> &L, const cx<T> *col, int j,
+                                         int side, T k2b2, T alpha2)
+{
+    const int ld = L.ld, ny = L.ny;
+    const T scx = T(12) * L.dx2 * L.fac_y;
+    cx<T> b, c, d;
+    T q;
+    if (j == 1) {
+        b = L.sp[2] * xfull<T>(col, ld, j + 1, side, false) + L.sp[1] * xfull<T>(col, ld, j, side, false);
+        c = L.sp[3] * col[j - 1] + L.sp[4] * col[j] + L.sp[5] * col[j + 1] + L.sp[6] * col[j + 2];
+        d = k2b2 * (L.sp[1] * col[j] + L.sp[2] * col[j + 1]);
+        q = L.qdy2[0];
+    } else if (j <= ny - 3) {
+        b = L.cp[0] * xfull<T>(col, ld, j - 1, side, false) + L.cp[1] * xfull<T>(col, ld, j, side, false)
+            + L.cp[2] * xfull<T>(col, ld, j + 1, side, false);
+        c = L.cp[3] * col[j - 2] + L.cp[4] * col[j - 1] + L.cp[5] * col[j] + L.cp[6] * col[j + 1]
+            + L.cp[7] * col[j + 2];
+        d = k2b2 * (L.cp[0] * col[j - 1] + L.cp[1] * col[j] + L.cp[2] * col[j + 1]);
+        q = L.qdy2[j - 2];
+    } else if (j == ny - 2) {
+        b = xfull<T>(col, ld, j, side, side == 0);
+        c = L.pp[3] * col[j + 1] + L.pp[2] * col[j] + L.pp[1] * col[j - 1];
+        d = k2b2 * col[j];
+        q = L.qdy2[j - 1];
+    } else {
+        b = xfull<T>(col, ld, j, side, side == 0);
+        c = L.lp[2] * col[j] + L.lp[3] * col[j - 1] + L.lp[4] * col[j - 2];
+        d = k2b2 * col[j];
+        q = L.qdy2[j - 2];
+        return b / scx + c * q + d / L.fac_y - L.lp[1] * alpha2 * col[j] * L.sqq_top;
+    }
+    return b / scx + c * q + d / L.fac_y;
+}
+
+template <typename T>
+__device__ __forceinline__ void atomic_max_nonneg(double *addr, double v)
+{
+    // non-negative doubles order like their bit patterns
+    atomicMax(reinterpret_cast<unsigned long long *>(addr), __double_as_longlong(v));
+}
+
+template <typename T, typename TO>
+__global__ void __launch_bounds__(256) defect_kernel(LevelArgs<T> L, const cx<T> *V,
+                                                     const cx<T> *S, cx<TO> *out, T si,
+                                                     const int *active, double *resmax)
+{
+    // block: 64 rows (j) x 4 columns (i); grid.y = mode
+    const int j = blockIdx.x % L.jblocks * 64 + threadIdx.x % 64;
+    const int i = (blockIdx.x / L.jblocks) * 4 + threadIdx.x / 64;
+    const int k = blockIdx.y;
+    double amax = 0.0;
+    const bool live = !(active && !active[k]);
+    if (live && i >= 1 && i <= L.nx - 2 && j >= 1 && j < L.ny) {
+        const size_t base = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * L.ld;
+        const int side = (i == 1) ? -1 : (i == L.nx - 2) ? 1 : 0;
+        const T kk = static_cast<T>(k);
+        const T k2b2 = -kk * kk * L.beta * L.beta;
+        const T alpha2 = sqrt((L.alpha * L.alpha - k2b2) / L.fac_y);
+        const cx<T> lv = apply_L<T>(L, V + base, j, side, k2b2, alpha2);
+        const cx<T> r = S[base + j] + si * lv;
+        out[base + j] = cx<TO>{static_cast<TO>(r.re), static_cast<TO>(r.im)};
+        if (resmax) amax = hypot(static_cast<double>(r.re), static_cast<double>(r.im));
+    }
+    if (resmax) {
+        __shared__ double red[4];
+        for (int o = 32; o > 0; o >>= 1) amax = fmax(amax, __shfl_down(amax, o, 64));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = amax;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+            if (live) atomic_max_nonneg<T>(resmax + k, m);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// restrict (poisson_par.f:796-874): inject v into the coarse level (and the
+// FAS copy w), full-weight the fine residual into the coarse source.
+// fas == false: correction scheme (coarse v = 0, no copy).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) restrict_kernel(Level F, Level Cl, int K,
+                                                       const cx<T> *Vf, const cx<T> *Rf,
+                                                       cx<T> *Vc, cx<T> *Wc, cx<T> *Sc,
+                                                       int fas, const int *active)
+{
+    const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    const size_t per = Cl.plane;
+    const int k = static_cast<int>(idx / per);
+    if (k >= K) return;
+    if (active && !active[k]) return;
+    const size_t rem = idx - static_cast<size_t>(k) * per;
+    const int i = static_cast<int>(rem / Cl.ld), j = static_cast<int>(rem % Cl.ld);
+    if (j >= Cl.ny) return;
+    const int fi = 2 * i, fj = 2 * j;
+    const cx<T> *vf = Vf + static_cast<size_t>(k) * F.plane;
+    const cx<T> *rf = Rf + static_cast<size_t>(k) * F.plane;
+    const size_t co = static_cast<size_t>(k) * Cl.plane + static_cast<size_t>(i) * Cl.ld + j;
+    if (fas) {
+        const cx<T> v = vf[static_cast<size_t>(fi) * F.ld + fj];
+        Vc[co] = v;
+        Wc[co] = v;
+    } else {
+        Vc[co] = cx<T>{T(0), T(0)};
+    }
+    if (i >= 1 && i <= Cl.nx - 2) {
+        auto R = [&](int ii, int jj) { return rf[static_cast<size_t>(ii) * F.ld + jj]; };
+        if (j >= 1 && j <= Cl.ny - 2) {
+            Sc[co] = T(6.25e-2) * (T(2) * (R(fi - 1, fj) + R(fi + 1, fj) + R(fi, fj - 1) + R(fi, fj + 1))
+                                   + T(4) * R(fi, fj) + R(fi + 1, fj + 1) + R(fi + 1, fj - 1)
+                                   + R(fi - 1, fj + 1) + R(fi - 1, fj - 1));
+        } else if (j == Cl.ny - 1) {
+            Sc[co] = T(0.25) * (R(fi - 1, fj) + T(2) * R(fi, fj) + R(fi + 1, fj));
+        }
+    }
+}
+
+// correction + intpol (poisson_par.f:877-942).  The coarse correction is
+// v_c - w_c on interior columns and v_c itself on the two boundary columns
+// (the reference interpolates those unchanged values as well).  For the
+// correction scheme (fas == 0) it is v_c.
+template <typename T>
+__global__ void __launch_bounds__(256) prolong_kernel(Level F, Level Cl, int K, cx<T> *Vf,
+                                                      const cx<T> *Vc, const cx<T> *Wc,
+                                                      int fas, const int *active)
+{
+    const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    const int k = static_cast<int>(idx / F.plane);
+    if (k >= K) return;
+    if (active && !active[k]) return;
+    const size_t rem = idx - static_cast<size_t>(k) * F.plane;
+    const int i = static_cast<int>(rem / F.ld), j = static_cast<int>(rem % F.ld);
+    if (j >= F.ny || i < 1 || i > F.nx - 2) return;
+    const cx<T> *vc = Vc + static_cast<size_t>(k) * Cl.plane;
+    const cx<T> *wc = Wc + static_cast<size_t>(k) * Cl.plane;
+    auto E = [&](int ci, int cj) {
+        const size_t o = static_cast<size_t>(ci) * Cl.ld + cj;
+        if (fas && ci >= 1 && ci <= Cl.nx - 2) return vc[o] - wc[o];
+        return vc[o];
+    };
+    auto row = [&](int cj) {   // value on an even (0-based) fine row
+        if ((i & 1) == 0) return E(i >> 1, cj);
+        return T(0.5) * (E(i >> 1, cj) + E((i >> 1) + 1, cj));
+    };
+    cx<T> t;
+    if ((j & 1) == 0) t = row(j >> 1);
+    else t = T(0.5) * (row((j >> 1) + 1) + row(j >> 1));
+    const size_t o = static_cast<size_t>(k) * F.plane + static_cast<size_t>(i) * F.ld + j;
+    Vf[o] = Vf[o] + t;
+}
+
+// ---------------------------------------------------------------------------
+// user layout (x fastest) <-> internal layout (y fastest) through LDS tiles
+// ---------------------------------------------------------------------------
+// g = kb*wx - dwzdx (wx may be NULL) into G (internal), uy into V (internal)
+__global__ void __launch_bounds__(256) gather_kernel(int nx, int ny, size_t ld, size_t plane,
+                                                     const cx<double> *wx, const cx<double> *dz,
+                                                     const cx<double> *uy, double beta,
+                                                     cx<double> *Gout, cx<double> *Vout,
+                                                     int direct_rhs)
+{
+    __shared__ cx<double> tg[32][33], tv[32][33];
+    const int k = blockIdx.z;
+    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;   // 32 x 8
+    const double kb = -static_cast<double>(k) * beta;           // kb = (0, kb)
+    const size_t ubase = static_cast<size_t>(k) * nx * ny;
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + tx, j = j0 + r;
+        cx<double> g = {0, 0}, v = {0, 0};
+        if (i < nx && j < ny) {
+            const size_t u = ubase + static_cast<size_t>(j) * nx + i;
+            cx<double> d = dz ? dz[u] : cx<double>{0, 0};
+            if (direct_rhs) {
+                g = d;                      // d holds f directly (solve with S = B f)
+            } else {
+                cx<double> kw = {0, 0};
+                if (wx) { const cx<double> w = wx[u]; kw = cmul(cx<double>{0.0, kb}, w); }
+                g = kw - d;
+            }
+            v = uy ? uy[u] : cx<double>{0, 0};
+        }
+        tg[r][tx] = g;
+        tv[r][tx] = v;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r, j = j0 + tx;
+        if (i < nx && j < ny) {
+            const size_t o = static_cast<size_t>(k) * plane + static_cast<size_t>(i) * ld + j;
+            Gout[o] = tg[tx][r];
+            Vout[o] = tv[tx][r];
+        }
+    }
+}
+
+// function_source weights (poisson_par.f:126-144), internal layout
+__global__ void __launch_bounds__(256) source_kernel(LevelArgs<double> L, const cx<double> *Gin,
+                                                     cx<double> *S)
+{
+    const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    const int k = static_cast<int>(idx / L.plane);
+    if (k >= L.K) return;
+    const size_t rem = idx - static_cast<size_t>(k) * L.plane;
+    const int j = static_cast<int>(rem % L.ld);
+    if (j >= L.ny) return;
+    const cx<double> *g = Gin + (idx - j);
+    cx<double> s = {0, 0};
+    if (j == 1) s = L.sp[1] * g[1] + L.sp[2] * g[2];
+    else if (j >= 2 && j <= L.ny - 3) s = L.cp[0] * g[j - 1] + L.cp[1] * g[j] + L.cp[2] * g[j + 1];
+    else if (j == L.ny - 2) s = g[j];
+    S[idx] = s;
+}
+
+// internal V -> user uy on i=2..nx-1, j=2..ny (poisson_par.f:52-57)
+__global__ void __launch_bounds__(256) scatter_kernel(int nx, int ny, size_t ld, size_t plane,
+                                                      const cx<double> *V, cx<double> *uy)
+{
+    __shared__ cx<double> t[32][33];
+    const int k = blockIdx.z;
+    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r, j = j0 + tx;
+        if (i < nx && j < ny)
+            t[r][tx] = V[static_cast<size_t>(k) * plane + static_cast<size_t>(i) * ld + j];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + tx, j = j0 + r;
+        if (i >= 1 && i <= nx - 2 && j >= 1 && j < ny)
+            uy[static_cast<size_t>(k) * nx * ny + static_cast<size_t>(j) * nx + i] = t[tx][r];
+    }
+}
+
+// convergence bookkeeping after the level-1 check of cycle `cycle`
+__global__ void decide_kernel(int K, const double *resmax, double tol, int cycle, int *active,
+                              int *cycles, double *lastres, int *nactive)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    if (!active[k]) return;
+    lastres[k] = resmax[k];
+    if (resmax[k] < tol) {
+        active[k] = 0;
+        cycles[k] = cycle - 1;
+    } else {
+        cycles[k] = cycle;      // will complete this cycle
+        atomicAdd(nactive, 1);
+    }
+}
+
+__global__ void fill_kernel(double *p, size_t n, double v)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void fill_int_kernel(int *p, int n, int v)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// mixed precision helpers ----------------------------------------------------
+// v64 += e32 on the interior (i=2..nx-1, j=2..ny)
+__global__ void __launch_bounds__(256) update_kernel(Level F, int K, cx<double> *V,
+                                                     const cx<float> *E, const int *active)
+{
+    const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    const int k = static_cast<int>(idx / F.plane);
+    if (k >= K) return;
+    if (active && !active[k]) return;
+    const size_t rem = idx - static_cast<size_t>(k) * F.plane;
+    const int i = static_cast<int>(rem / F.ld), j = static_cast<int>(rem % F.ld);
+    if (i < 1 || i > F.nx - 2 || j < 1 || j >= F.ny) return;
+    const cx<float> e = E[idx];
+    V[idx] = cx<double>{V[idx].re + static_cast<double>(e.re), V[idx].im + static_cast<double>(e.im)};
+}
+
+template <typename TA, typename TB>
+__global__ void __launch_bounds__(256) convert_kernel(size_t n, const cx<TA> *a, cx<TB> *b,
+                                                      int zero_a_into_b)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    if (zero_a_into_b) { b[i] = cx<TB>{TB(0), TB(0)}; return; }
+    b[i] = cx<TB>{static_cast<TB>(a[i].re), static_cast<TB>(a[i].im)};
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned nblk(size_t n, unsigned b = 256) { return static_cast<unsigned>((n + b - 1) / b); }
+
+template <typename T, int G, int R>
+static void launch_phase(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int phase,
+                         T rf, const int *active, hipStream_t st)
+{
+    const size_t lines = static_cast<size_t>(phase_lines(phase, L.nx)) * L.K;
+    if (!lines) return;
+    const size_t threads = lines * G;
+    hipLaunchKernelGGL((smooth_phase_kernel<T, G, R>), dim3(nblk(threads)), dim3(256), 0, st, L, V, S,
+                       lu, phase, rf, active);
+}
+
+template <typename T, int G, int R>
+static void launch_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu,
+                            int sweeps, T rf, const int *active, hipStream_t st, int threads)
+{
+    const size_t shm = L.plane * sizeof(cx<T>);
+    static const bool attr_set = [] {
+        return hipFuncSetAttribute(reinterpret_cast<const void *>(&smooth_coarsest_kernel<T, G, R>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) ==
+               hipSuccess;
+    }();
+    (void)attr_set;
+    hipLaunchKernelGGL((smooth_coarsest_kernel<T, G, R>), dim3(L.K), dim3(threads), shm, st, L, V,
+                       S, lu, sweeps, rf, active);
+}
+
+// (G, R) table shared by both smoother forms
+#define TBL_LINE_SHAPES(X) \
+    X(16, 1) X(16, 2) X(16, 3) X(16, 4) X(16, 5) X(16, 7) X(16, 9) X(16, 13) X(16, 17) \
+    X(32, 1) X(32, 2) X(32, 3) X(32, 4) X(32, 5) X(32, 7) X(32, 9) \
+    X(64, 1) X(64, 2) X(64, 3) X(64, 4) X(64, 5) X(64, 7) X(64, 9) X(64, 13) X(64, 17)
+
+static const int kRs[] = {1, 2, 3, 4, 5, 7, 9, 13, 17};
+
+int line_rows_for(int G, int ny)
+{
+    const int need = (ny + G - 1) / G;
+    for (int r : kRs) {
+        if (r < need) continue;
+        if (G == 32 && r > 9) return -1;
+        return r;
+    }
+    return -1;
+}
+
+template <typename T>
+int smooth_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
+                  const int *active, int G, hipStream_t st)
+{
+    const int R = line_rows_for(G, L.ny);
+    if (R < 0) return TBL_ERR_GRID;
+    for (int it = 0; it < sweeps; ++it)
+        for (int ph = 0; ph < 4; ++ph) {
+            bool done = false;
+#define X(GG, RR)                                                             \
+    if (!done && G == GG && R == RR) {                                        \
+        launch_phase<T, GG, RR>(L, V, S, lu, ph, rf, active, st);             \
+        done = true;                                                          \
+    }
+            TBL_LINE_SHAPES(X)
+#undef X
+            if (!done) return TBL_ERR_GRID;
+        }
+    return TBL_OK;
+}
+
+template <typename T>
+int smooth_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps,
+                    T rf, const int *active, int G, hipStream_t st)
+{
+    const size_t shm = L.plane * sizeof(cx<T>);
+    if (shm > 160 * 1024) return smooth_sweeps<T>(L, V, S, lu, sweeps, rf, active, G, st);
+    const int R = line_rows_for(G, L.ny);
+    if (R < 0) return TBL_ERR_GRID;
+    bool done = false;
+#define X(GG, RR)                                                                 \
+    if (!done && G == GG && R == RR) {                                            \
+        launch_coarsest<T, GG, RR>(L, V, S, lu, sweeps, rf, active, st, 1024);    \
+        done = true;                                                              \
+    }
+    TBL_LINE_SHAPES(X)
+#undef X
+    return done ? TBL_OK : TBL_ERR_GRID;
+}
+
+template <typename T, typename TO>
+void defect(const LevelArgs<T> &L, const cx<T> *V, const cx<T> *S, cx<TO> *out, T si,
+            const int *active, double *resmax, hipStream_t st)
+{
+    const int ib = (L.nx + 3) / 4;
+    hipLaunchKernelGGL((defect_kernel<T, TO>), dim3(ib * L.jblocks, L.K), dim3(256), 0, st, L, V, S,
+                       out, si, active, resmax);
+}
+
+template <typename T>
+void restrict_level(const Level &F, const Level &Cl, int K, const cx<T> *Vf, const cx<T> *Rf,
+                    cx<T> *Vc, cx<T> *Wc, cx<T> *Sc, int fas, const int *active, hipStream_t st)
+{
+    const size_t n = Cl.plane * K;
+    hipLaunchKernelGGL((restrict_kernel<T>), dim3(nblk(n)), dim3(256), 0, st, F, Cl, K, Vf, Rf, Vc,
+                       Wc, Sc, fas, active);
+}
+
+template <typename T>
+void prolong_level(const Level &F, const Level &Cl, int K, cx<T> *Vf, const cx<T> *Vc,
+                   const cx<T> *Wc, int fas, const int *active, hipStream_t st)
+{
+    const size_t n = F.plane * K;
+    hipLaunchKernelGGL((prolong_kernel<T>), dim3(nblk(n)), dim3(256), 0, st, F, Cl, K, Vf, Vc, Wc,
+                       fas, active);
+}
+
+void gather(int nx, int ny, int K, const Level &F, const double *wx, const double *dz,
+            const double *uy, double beta, cx<double> *G, cx<double> *V, int direct_rhs,
+            hipStream_t st)
+{
+    dim3 grid((nx + 31) / 32, (ny + 31) / 32, K);
+    hipLaunchKernelGGL(gather_kernel, grid, dim3(256), 0, st, nx, ny, static_cast<size_t>(F.ld),
+                       F.plane, reinterpret_cast<const cx<double> *>(wx),
+                       reinterpret_cast<const cx<double> *>(dz),
+                       reinterpret_cast<const cx<double> *>(uy), beta, G, V, direct_rhs);
+}
+
+void source(const LevelArgs<double> &L, const cx<double> *G, cx<double> *S, hipStream_t st)
+{
+    hipLaunchKernelGGL(source_kernel, dim3(nblk(L.plane * L.K)), dim3(256), 0, st, L, G, S);
+}
+
+void scatter(int nx, int ny, int K, const Level &F, const cx<double> *V, double *uy,
+             hipStream_t st)
+{
+    dim3 grid((nx + 31) / 32, (ny + 31) / 32, K);
+    hipLaunchKernelGGL(scatter_kernel, grid, dim3(256), 0, st, nx, ny, static_cast<size_t>(F.ld),
+                       F.plane, V, reinterpret_cast<cx<double> *>(uy));
+}
+
+void decide(int K, const double *resmax, double tol, int cycle, int *active, int *cycles,
+            double *lastres, int *nactive, hipStream_t st)
+{
+    hipLaunchKernelGGL(decide_kernel, dim3((K + 255) / 256), dim3(256), 0, st, K, resmax, tol,
+                       cycle, active, cycles, lastres, nactive);
+}
+
+void fill(double *p, size_t n, double v, hipStream_t st)
+{
+    if (n) hipLaunchKernelGGL(fill_kernel, dim3(nblk(n)), dim3(256), 0, st, p, n, v);
+}
+
+void fill_int(int *p, int n, int v, hipStream_t st)
+{
+    if (n) hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+}
+
+void update(const Level &F, int K, cx<double> *V, const cx<float> *E, const int *active,
+            hipStream_t st)
+{
+    hipLaunchKernelGGL(update_kernel, dim3(nblk(F.plane * K)), dim3(256), 0, st, F, K, V, E, active);
+}
+
+template <typename TA, typename TB>
+void convert(size_t n, const cx<TA> *a, cx<TB> *b, int zero, hipStream_t st)
+{
+    hipLaunchKernelGGL((convert_kernel<TA, TB>), dim3(nblk(n)), dim3(256), 0, st, n, a, b, zero);
+}
+
+// explicit instantiations
+template int smooth_sweeps<double>(const LevelArgs<double> &, cx<double> *, const cx<double> *,
+                                   const double *, int, double, const int *, int, hipStream_t);
+template int smooth_sweeps<float>(const LevelArgs<float> &, cx<float> *, const cx<float> *,
+                                  const float *, int, float, const int *, int, hipStream_t);
+template int smooth_coarsest<double>(const LevelArgs<double> &, cx<double> *, const cx<double> *,
+                                     const double *, int, double, const int *, int, hipStream_t);
+template void defect<double, double>(const LevelArgs<double> &, const cx<double> *,
+                                     const cx<double> *, cx<double> *, double, const int *,
+                                     double *, hipStream_t);
+template void defect<double, float>(const LevelArgs<double> &, const cx<double> *,
+                                    const cx<double> *, cx<float> *, double, const int *, double *,
+                                    hipStream_t);
+template void defect<float, float>(const LevelArgs<float> &, const cx<float> *, const cx<float> *,
+                                   cx<float> *, float, const int *, double *, hipStream_t);
+template void restrict_level<double>(const Level &, const Level &, int, const cx<double> *,
+                                     const cx<double> *, cx<double> *, cx<double> *, cx<double> *,
+                                     int, const int *, hipStream_t);
+template void restrict_level<float>(const Level &, const Level &, int, const cx<float> *,
+                                    const cx<float> *, cx<float> *, cx<float> *, cx<float> *, int,
+                                    const int *, hipStream_t);
+template void prolong_level<double>(const Level &, const Level &, int, cx<double> *,
+                                    const cx<double> *, const cx<double> *, int, const int *,
+                                    hipStream_t);
+template void prolong_level<float>(const Level &, const Level &, int, cx<float> *,
+                                   const cx<float> *, const cx<float> *, int, const int *,
+                                   hipStream_t);
+template void convert<double, float>(size_t, const cx<double> *, cx<float> *, int, hipStream_t);
+template void convert<float, float>(size_t, const cx<float> *, cx<float> *, int, hipStream_t);
+template void convert<double, double>(size_t, const cx<double> *, cx<double> *, int, hipStream_t);
+template void convert<float, double>(size_t, const cx<float> *, cx<double> *, int, hipStream_t);
+
+}  // namespace tbl

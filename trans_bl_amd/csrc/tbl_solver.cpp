@@ -1,0 +1,597 @@
+// Context creation, the batched FAS / mixed-precision V-cycle drivers and the
+// C ABI of trans_bl_amd.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "tbl_ctx.h"
+
+using namespace tbl;
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void tbl_set_last_error(const char *msg, const char *file, int line)
+{
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "%s (%s:%d)", msg, file, line);
+    g_last_error = buf;
+}
+
+extern "C" const char *tbl_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" const char *tbl_version(void) { return "trans_bl_amd 0.1 gfx950"; }
+
+namespace {
+
+template <typename T>
+int dalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) return TBL_OK;
+    TBL_HIP(hipMalloc(reinterpret_cast<void **>(p), n * sizeof(T)));
+    TBL_HIP(hipMemset(*p, 0, n * sizeof(T)));
+    return TBL_OK;
+}
+
+#define TRY(x)                     \
+    do {                           \
+        int rc_ = (x);             \
+        if (rc_ != TBL_OK) return rc_; \
+    } while (0)
+
+int default_G(int ny)
+{
+    if (ny >= 160) return 64;
+    if (ny >= 48) return 32;
+    return 16;
+}
+
+template <typename T>
+LevelArgs<T> level_args(const tbl_ctx &c, int l, const T *qdy2)
+{
+    LevelArgs<T> A{};
+    const Level &lv = c.lv[l];
+    A.nx = lv.nx;
+    A.ny = lv.ny;
+    A.ld = lv.ld;
+    A.K = c.K;
+    A.jblocks = (lv.ld + 63) / 64;
+    A.plane = lv.plane;
+    for (int i = 0; i < 7; ++i) A.sp[i] = static_cast<T>(c.c.sp_poi[l][i]);
+    for (int i = 0; i < 8; ++i) A.cp[i] = static_cast<T>(c.c.cp_poi[l][i]);
+    for (int i = 0; i < 4; ++i) A.pp[i] = static_cast<T>(c.c.pp_poi[l][i]);
+    for (int i = 0; i < 5; ++i) A.lp[i] = static_cast<T>(c.c.lp_poi[l][i]);
+    A.qx = static_cast<T>(1.0 / (12.0 * c.m.dx2[l] * c.g.fac_y));
+    A.dx2 = static_cast<T>(c.m.dx2[l]);
+    A.fac_y = static_cast<T>(c.g.fac_y);
+    A.alpha = static_cast<T>(c.g.alpha);
+    A.beta = static_cast<T>(c.g.beta);
+    A.sqq_top = static_cast<T>(std::sqrt(c.m.qdy2[static_cast<size_t>(l) * c.g.ny + (lv.ny - 3)]));
+    A.qdy2 = qdy2;
+    return A;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+static int create_impl(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx *c)
+{
+    c->g = *grid;
+    TRY(check_grid(c->g));
+    if (coefs) c->c = *coefs;
+    else if (!generate_coefs(c->g.stf, c->g.levels, &c->c)) return TBL_ERR_ARG;
+    build_metrics(c->g, c->m);
+    TBL_HIP(hipGetDevice(&c->device));
+    c->K = c->g.nmodes;
+    c->L = c->g.levels;
+    const int K = c->K, L = c->L;
+    c->lv.resize(L);
+    for (int l = 0; l < L; ++l) c->lv[l] = make_level(c->m.nx[l], c->m.ny[l], c->m.dx2[l]);
+    c->V.assign(L, nullptr); c->S.assign(L, nullptr); c->W.assign(L, nullptr); c->R.assign(L, nullptr);
+    c->E32.assign(L, nullptr); c->S32.assign(L, nullptr); c->R32.assign(L, nullptr);
+    c->qdy2_d.assign(L, nullptr); c->qdy2_f.assign(L, nullptr);
+    c->lu_d.assign(L, nullptr); c->lu_f.assign(L, nullptr);
+    c->G64.assign(L, 0); c->G32.assign(L, 0);
+    for (int l = 0; l < L; ++l) {
+        const Level &lv = c->lv[l];
+        const size_t n = lv.plane * K;
+        TRY(dalloc(&c->V[l], n));
+        TRY(dalloc(&c->S[l], n));
+        TRY(dalloc(&c->W[l], n));
+        TRY(dalloc(&c->R[l], n));
+        TRY(dalloc(&c->E32[l], n));
+        TRY(dalloc(&c->S32[l], n));
+        TRY(dalloc(&c->R32[l], n));
+        // metric v_qdy2(:, l)
+        std::vector<double> q(lv.ld, 0.0);
+        std::vector<float> qf(lv.ld, 0.0f);
+        for (int j = 0; j < lv.ny - 1; ++j) {
+            q[j] = c->m.qdy2[static_cast<size_t>(l) * c->g.ny + j];
+            qf[j] = static_cast<float>(q[j]);
+        }
+        TRY(dalloc(&c->qdy2_d[l], lv.ld));
+        TRY(dalloc(&c->qdy2_f[l], lv.ld));
+        TBL_HIP(hipMemcpy(c->qdy2_d[l], q.data(), sizeof(double) * lv.ld, hipMemcpyHostToDevice));
+        TBL_HIP(hipMemcpy(c->qdy2_f[l], qf.data(), sizeof(float) * lv.ld, hipMemcpyHostToDevice));
+        // line LU factors per (mode, centre/border)
+        const size_t per = static_cast<size_t>(5) * lv.ld;
+        std::vector<double> lu(per * 2 * K, 0.0);
+        std::vector<float> luf(lu.size(), 0.0f);
+        for (int k = 0; k < K; ++k)
+            for (int t = 0; t < 2; ++t) {
+                double *b = &lu[(static_cast<size_t>(k) * 2 + t) * per];
+                line_lu(c->g, c->c, c->m, l + 1, k, t == 1, b, b + lv.ld, b + 2 * lv.ld,
+                        b + 3 * lv.ld, b + 4 * lv.ld);
+            }
+        for (size_t e = 0; e < lu.size(); ++e) luf[e] = static_cast<float>(lu[e]);
+        TRY(dalloc(&c->lu_d[l], lu.size()));
+        TRY(dalloc(&c->lu_f[l], luf.size()));
+        TBL_HIP(hipMemcpy(c->lu_d[l], lu.data(), sizeof(double) * lu.size(), hipMemcpyHostToDevice));
+        TBL_HIP(hipMemcpy(c->lu_f[l], luf.data(), sizeof(float) * luf.size(), hipMemcpyHostToDevice));
+        c->G64[l] = default_G(lv.ny);
+        c->G32[l] = default_G(lv.ny);
+    }
+    c->la_d.resize(L);
+    c->la_f.resize(L);
+    for (int l = 0; l < L; ++l) {
+        c->la_d[l] = level_args<double>(*c, l, c->qdy2_d[l]);
+        c->la_f[l] = level_args<float>(*c, l, c->qdy2_f[l]);
+    }
+    TRY(dalloc(&c->d_active, K));
+    TRY(dalloc(&c->d_cycles, K));
+    TRY(dalloc(&c->d_nactive, 1));
+    TRY(dalloc(&c->d_cycle, 1));
+    TRY(dalloc(&c->d_resmax, K));
+    TRY(dalloc(&c->d_lastres, K));
+    TBL_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->h_nactive), sizeof(int), hipHostMallocDefault));
+
+    // derivatives: metrics and Thomas factors of the five schemes
+    const int nmax = std::max(c->g.nx, c->g.ny);
+    std::vector<double> th(static_cast<size_t>(5) * 3 * nmax, 0.0);
+    for (int kind = 0; kind < 5; ++kind) {
+        std::vector<double> a, b, cc, gam, bet;
+        deriv_lhs(c->g, c->c, kind, a, b, cc);
+        thomas_factors(a, b, cc, gam, bet);
+        double *t = &th[static_cast<size_t>(kind) * 3 * nmax];
+        for (size_t j = 0; j < a.size(); ++j) {
+            t[j] = a[j];
+            t[nmax + j] = bet[j];
+            t[2 * nmax + j] = gam[j];
+        }
+    }
+    TRY(dalloc(&c->d_thomas, th.size()));
+    TBL_HIP(hipMemcpy(c->d_thomas, th.data(), sizeof(double) * th.size(), hipMemcpyHostToDevice));
+    TRY(dalloc(&c->d_qdy, c->g.ny));
+    TBL_HIP(hipMemcpy(c->d_qdy, c->m.qdy.data(), sizeof(double) * c->g.ny, hipMemcpyHostToDevice));
+    DerivArgs &A = c->dargs;
+    A.nx = c->g.nx;
+    A.ny = c->g.ny;
+    A.K = K;
+    A.dx = c->g.dx;
+    A.dxx = c->g.dx * c->g.dx;
+    std::memcpy(A.fp_e, c->c.fp_fd_e, sizeof(A.fp_e));
+    std::memcpy(A.sp_fd, c->c.sp_fd, sizeof(A.sp_fd));
+    std::memcpy(A.cp_fd, c->c.cp_fd, sizeof(A.cp_fd));
+    std::memcpy(A.pp_fd, c->c.pp_fd, sizeof(A.pp_fd));
+    std::memcpy(A.lp_fd, c->c.lp_fd, sizeof(A.lp_fd));
+    std::memcpy(A.fp_sd, c->c.fp_sd, sizeof(A.fp_sd));
+    std::memcpy(A.sp_sd, c->c.sp_sd, sizeof(A.sp_sd));
+    std::memcpy(A.cp_sd, c->c.cp_sd, sizeof(A.cp_sd));
+    std::memcpy(A.pp_sd, c->c.pp_sd, sizeof(A.pp_sd));
+    std::memcpy(A.lp_sd, c->c.lp_sd, sizeof(A.lp_sd));
+    A.qdy = c->d_qdy;
+    A.qdy2 = c->qdy2_d[0];
+    A.thomas = c->d_thomas;
+    A.nmax = nmax;
+    TBL_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+    c->stream = c->own_stream;
+    TBL_HIP(hipEventCreate(&c->ev0));
+    TBL_HIP(hipEventCreate(&c->ev1));
+    TBL_HIP(hipDeviceSynchronize());
+    return TBL_OK;
+}
+
+extern "C" int tbl_coefs_generate(double stf, int levels, tbl_coefs *out)
+{
+    if (!out || levels < 1 || levels > TBL_MAX_LEVELS || !(stf > 0)) return TBL_ERR_ARG;
+    return generate_coefs(stf, levels, out) ? TBL_OK : TBL_ERR_ARG;
+}
+
+extern "C" int tbl_destroy(tbl_ctx *c)
+{
+    if (!c) return TBL_ERR_ARG;
+    auto fr = [](void *p) { if (p) (void)hipFree(p); };
+    for (auto *v : {&c->V, &c->S, &c->W, &c->R})
+        for (auto *p : *v) fr(p);
+    for (auto *v : {&c->E32, &c->S32, &c->R32})
+        for (auto *p : *v) fr(p);
+    for (auto *p : c->qdy2_d) fr(p);
+    for (auto *p : c->qdy2_f) fr(p);
+    for (auto *p : c->lu_d) fr(p);
+    for (auto *p : c->lu_f) fr(p);
+    fr(c->d_active); fr(c->d_cycles); fr(c->d_nactive); fr(c->d_cycle);
+    fr(c->d_resmax); fr(c->d_lastres); fr(c->d_qdy); fr(c->d_thomas);
+    fr(c->d_u0); fr(c->d_u1); fr(c->d_u2);
+    if (c->h_nactive) (void)hipHostFree(c->h_nactive);
+    for (auto &g : c->graph)
+        if (g) (void)hipGraphExecDestroy(g);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    delete c;
+    return TBL_OK;
+}
+
+extern "C" int tbl_create(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx **out)
+{
+    if (!grid || !out) return TBL_ERR_ARG;
+    *out = nullptr;
+    tbl_ctx *c = new (std::nothrow) tbl_ctx();
+    if (!c) return TBL_ERR_ALLOC;
+    int rc = create_impl(grid, coefs, c);
+    if (rc != TBL_OK) {
+        tbl_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return TBL_OK;
+}
+
+extern "C" int tbl_synchronize(tbl_ctx *c)
+{
+    if (!c) return TBL_ERR_ARG;
+    TBL_HIP(hipStreamSynchronize(c->stream));
+    return TBL_OK;
+}
+
+extern "C" int tbl_set_stream(tbl_ctx *c, void *s)
+{
+    if (!c) return TBL_ERR_ARG;
+    c->stream = s ? static_cast<hipStream_t>(s) : c->own_stream;
+    return TBL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle drivers
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Sched {
+    int maxc, nu1, nu2, nc;
+};
+
+Sched sched_of(const tbl_solve_opts *o)
+{
+    Sched s{11, 3, 1, 60};
+    if (o) {
+        if (o->max_cycles > 0) s.maxc = o->max_cycles;
+        if (o->nu1 > 0) s.nu1 = o->nu1;
+        if (o->nu2 > 0) s.nu2 = o->nu2;
+        if (o->ncoarse > 0) s.nc = o->ncoarse;
+    }
+    return s;
+}
+
+// rf = 1.2, or 1.0 when a single sweep is done (poisson_par.f:181-182)
+inline double rf_of(int sweeps) { return sweeps == 1 ? 1.0 : 1.2; }
+
+__global__ void tick_kernel(int *cycle, double *resmax, int K, int *nactive)
+{
+    const int t = threadIdx.x;
+    if (t == 0) {
+        *cycle += 1;
+        *nactive = 0;
+    }
+    for (int k = t; k < K; k += blockDim.x) resmax[k] = 0.0;
+}
+
+__global__ void decide2_kernel(int K, const double *resmax, double tol, const int *cycle,
+                               int *active, int *cycles, double *lastres, int *nactive)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K || !active[k]) return;
+    const int c = *cycle;
+    lastres[k] = resmax[k];
+    if (resmax[k] < tol) {
+        active[k] = 0;
+        cycles[k] = c - 1;
+    } else {
+        cycles[k] = c;
+        atomicAdd(nactive, 1);
+    }
+}
+
+// One FAS V-cycle of every active mode (poisson_par.f:33-50)
+int fas_cycle(tbl_ctx *c, const Sched &s, double tol)
+{
+    hipStream_t st = c->stream;
+    const int L = c->L, K = c->K;
+    hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(256), 0, st, c->d_cycle, c->d_resmax, K, c->d_nactive);
+    for (int l = 0; l < L - 1; ++l) {
+        TRY(smooth_sweeps<double>(c->la_d[l], c->V[l], c->S[l], c->lu_d[l], s.nu1, rf_of(s.nu1),
+                                  c->d_active, c->G64[l], st));
+        defect<double, double>(c->la_d[l], c->V[l], c->S[l], c->R[l], -1.0, c->d_active,
+                               l == 0 ? c->d_resmax : nullptr, st);
+        if (l == 0)
+            hipLaunchKernelGGL(decide2_kernel, dim3((K + 255) / 256), dim3(256), 0, st, K,
+                               c->d_resmax, tol, c->d_cycle, c->d_active, c->d_cycles,
+                               c->d_lastres, c->d_nactive);
+        restrict_level<double>(c->lv[l], c->lv[l + 1], K, c->V[l], c->R[l], c->V[l + 1],
+                               c->W[l + 1], c->S[l + 1], 1, c->d_active, st);
+        defect<double, double>(c->la_d[l + 1], c->V[l + 1], c->S[l + 1], c->S[l + 1], 1.0,
+                               c->d_active, nullptr, st);
+    }
+    TRY(smooth_coarsest<double>(c->la_d[L - 1], c->V[L - 1], c->S[L - 1], c->lu_d[L - 1], s.nc,
+                                rf_of(s.nc), c->d_active, c->G64[L - 1], st));
+    for (int l = L - 1; l >= 1; --l) {
+        prolong_level<double>(c->lv[l - 1], c->lv[l], K, c->V[l - 1], c->V[l], c->W[l], 1,
+                              c->d_active, st);
+        TRY(smooth_sweeps<double>(c->la_d[l - 1], c->V[l - 1], c->S[l - 1], c->lu_d[l - 1], s.nu2,
+                                  rf_of(s.nu2), c->d_active, c->G64[l - 1], st));
+    }
+    return TBL_OK;
+}
+
+// One FP64 defect-correction step with an FP32 correction-scheme V-cycle
+int mixed_cycle(tbl_ctx *c, const Sched &s, double tol)
+{
+    hipStream_t st = c->stream;
+    const int L = c->L, K = c->K;
+    hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(256), 0, st, c->d_cycle, c->d_resmax, K, c->d_nactive);
+    // FP64 residual of the current v, rounded into the FP32 source
+    defect<double, float>(c->la_d[0], c->V[0], c->S[0], c->S32[0], -1.0, c->d_active, c->d_resmax, st);
+    hipLaunchKernelGGL(decide2_kernel, dim3((K + 255) / 256), dim3(256), 0, st, K, c->d_resmax,
+                       tol, c->d_cycle, c->d_active, c->d_cycles, c->d_lastres, c->d_nactive);
+    convert<float, float>(c->lv[0].plane * K, c->E32[0], c->E32[0], 1, st);
+    for (int l = 0; l < L - 1; ++l) {
+        TRY(smooth_sweeps<float>(c->la_f[l], c->E32[l], c->S32[l], c->lu_f[l], s.nu1,
+                                 static_cast<float>(rf_of(s.nu1)), c->d_active, c->G32[l], st));
+        defect<float, float>(c->la_f[l], c->E32[l], c->S32[l], c->R32[l], -1.0f, c->d_active,
+                             nullptr, st);
+        restrict_level<float>(c->lv[l], c->lv[l + 1], K, c->E32[l], c->R32[l], c->E32[l + 1],
+                              nullptr, c->S32[l + 1], 0, c->d_active, st);
+    }
+    // coarsest correction solved in FP64
+    const size_t nc = c->lv[L - 1].plane * K;
+    convert<float, double>(nc, c->S32[L - 1], c->R[L - 1], 0, st);
+    convert<double, double>(nc, c->W[L - 1], c->W[L - 1], 1, st);
+    TRY(smooth_coarsest<double>(c->la_d[L - 1], c->W[L - 1], c->R[L - 1], c->lu_d[L - 1], s.nc,
+                                rf_of(s.nc), c->d_active, c->G64[L - 1], st));
+    convert<double, float>(nc, c->W[L - 1], c->E32[L - 1], 0, st);
+    for (int l = L - 1; l >= 1; --l) {
+        prolong_level<float>(c->lv[l - 1], c->lv[l], K, c->E32[l - 1], c->E32[l], nullptr, 0,
+                             c->d_active, st);
+        TRY(smooth_sweeps<float>(c->la_f[l - 1], c->E32[l - 1], c->S32[l - 1], c->lu_f[l - 1],
+                                 s.nu2, static_cast<float>(rf_of(s.nu2)), c->d_active,
+                                 c->G32[l - 1], st));
+    }
+    update(c->lv[0], K, c->V[0], c->E32[0], c->d_active, st);
+    return TBL_OK;
+}
+
+int run_cycle(tbl_ctx *c, int prec, const Sched &s, double tol, bool use_graph)
+{
+    if (!use_graph) return prec ? mixed_cycle(c, s, tol) : fas_cycle(c, s, tol);
+    const int key[5] = {s.nu1, s.nu2, s.nc, c->G64[0], c->G32[0]};
+    if (!c->graph[prec] || std::memcmp(key, c->graph_key[prec], sizeof(key)) != 0 ||
+        c->graph_tol[prec] != tol) {
+        if (c->graph[prec]) {
+            TBL_HIP(hipGraphExecDestroy(c->graph[prec]));
+            c->graph[prec] = nullptr;
+        }
+        if (!c->stream) return TBL_ERR_STATE;   // capture needs a non-null stream
+        hipGraph_t g;
+        TBL_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = prec ? mixed_cycle(c, s, tol) : fas_cycle(c, s, tol);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc != TBL_OK) return rc;
+        TBL_HIP(e);
+        TBL_HIP(hipGraphInstantiate(&c->graph[prec], g, nullptr, nullptr, 0));
+        TBL_HIP(hipGraphDestroy(g));
+        std::memcpy(c->graph_key[prec], key, sizeof(key));
+        c->graph_tol[prec] = tol;
+    }
+    TBL_HIP(hipGraphLaunch(c->graph[prec], c->stream));
+    return TBL_OK;
+}
+
+}  // namespace
+
+namespace tbl {
+
+int solve_dev(tbl_ctx *c, const double *wx, const double *dz, double *uy, int direct_rhs,
+              double tol, const tbl_solve_opts *opts, int *cycles, double *residual,
+              tbl_solve_stats *stats)
+{
+    if (!c || !dz || !uy) return TBL_ERR_ARG;
+    const Sched s = sched_of(opts);
+    const int prec = opts ? opts->precision : 0;
+    if (prec < 0 || prec > 1) return TBL_ERR_ARG;
+    const bool use_graph = opts && opts->use_graph;
+    hipStream_t st = c->stream;
+    const int K = c->K;
+    const Level &F = c->lv[0];
+    TBL_HIP(hipEventRecord(c->ev0, st));
+    // source assembly: S = B g, v = uy (function_source, poisson_par.f:111-164)
+    // guess 1: start from zero (boundary data taken as zero as well)
+    const bool zero_start = opts && opts->guess == 1;
+    gather(c->g.nx, c->g.ny, K, F, wx, dz, zero_start ? nullptr : uy, c->g.beta, c->R[0], c->V[0],
+           direct_rhs, st);
+    source(c->la_d[0], c->R[0], c->S[0], st);
+    fill_int(c->d_active, K, 1, st);
+    fill_int(c->d_cycles, K, 0, st);
+    fill_int(c->d_cycle, 1, 0, st);
+    fill(c->d_lastres, K, 0.0, st);
+    int done_cycles = 0;
+    for (int cyc = 1; cyc <= s.maxc; ++cyc) {
+        TRY(run_cycle(c, prec, s, tol, use_graph));
+        TBL_HIP(hipGetLastError());
+        TBL_HIP(hipMemcpyAsync(c->h_nactive, c->d_nactive, sizeof(int), hipMemcpyDeviceToHost, st));
+        TBL_HIP(hipStreamSynchronize(st));
+        done_cycles = cyc;
+        if (*c->h_nactive == 0) break;
+    }
+    (void)done_cycles;
+    scatter(c->g.nx, c->g.ny, K, F, c->V[0], uy, st);
+    TBL_HIP(hipGetLastError());
+    TBL_HIP(hipEventRecord(c->ev1, st));
+    std::vector<int> cyc(K);
+    std::vector<double> res(K);
+    TBL_HIP(hipMemcpyAsync(cyc.data(), c->d_cycles, sizeof(int) * K, hipMemcpyDeviceToHost, st));
+    TBL_HIP(hipMemcpyAsync(res.data(), c->d_lastres, sizeof(double) * K, hipMemcpyDeviceToHost, st));
+    TBL_HIP(hipStreamSynchronize(st));
+    if (cycles) std::memcpy(cycles, cyc.data(), sizeof(int) * K);
+    if (residual) std::memcpy(residual, res.data(), sizeof(double) * K);
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        for (int k = 0; k < K; ++k) {
+            stats->cycles_max = std::max(stats->cycles_max, cyc[k]);
+            stats->cycles_total += cyc[k];
+            stats->nconverged += res[k] < tol ? 1 : 0;
+            stats->residual_max = std::max(stats->residual_max, res[k]);
+        }
+        float ms = 0.f;
+        TBL_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        stats->device_ms = ms;
+    }
+    return TBL_OK;
+}
+
+}  // namespace tbl
+
+extern "C" int tbl_poi_uy_dev(tbl_ctx *c, const double *d_wx, const double *d_dwzdx, double *d_uy,
+                              double erromax, const tbl_solve_opts *opts, int *cycles,
+                              double *residual, tbl_solve_stats *stats)
+{
+    return solve_dev(c, d_wx, d_dwzdx, d_uy, 0, erromax, opts, cycles, residual, stats);
+}
+
+// Solve L v = B f directly (the synthetic form: wx = 0, dwzdx = -f)
+extern "C" int tbl_solve_rhs_dev(tbl_ctx *c, const double *d_f, double *d_uy, double erromax,
+                                 const tbl_solve_opts *opts, int *cycles, double *residual,
+                                 tbl_solve_stats *stats)
+{
+    return solve_dev(c, nullptr, d_f, d_uy, 1, erromax, opts, cycles, residual, stats);
+}
+
+static int staging(tbl_ctx *c)
+{
+    if (c->d_u0) return TBL_OK;
+    const size_t n = static_cast<size_t>(c->g.nx) * c->g.ny * c->K * 2;
+    TRY(dalloc(&c->d_u0, n));
+    TRY(dalloc(&c->d_u1, n));
+    TRY(dalloc(&c->d_u2, n));
+    return TBL_OK;
+}
+
+extern "C" int tbl_poi_uy(tbl_ctx *c, const double *wx, const double *dwzdx, double *uy,
+                          double erromax, const tbl_solve_opts *opts, int *cycles,
+                          double *residual, tbl_solve_stats *stats)
+{
+    if (!c || !dwzdx || !uy) return TBL_ERR_ARG;
+    TRY(staging(c));
+    const size_t bytes = sizeof(double) * 2 * c->g.nx * c->g.ny * c->K;
+    if (wx) TBL_HIP(hipMemcpy(c->d_u0, wx, bytes, hipMemcpyHostToDevice));
+    TBL_HIP(hipMemcpy(c->d_u1, dwzdx, bytes, hipMemcpyHostToDevice));
+    TBL_HIP(hipMemcpy(c->d_u2, uy, bytes, hipMemcpyHostToDevice));
+    TRY(solve_dev(c, wx ? c->d_u0 : nullptr, c->d_u1, c->d_u2, 0, erromax, opts, cycles, residual,
+                  stats));
+    TBL_HIP(hipMemcpy(uy, c->d_u2, bytes, hipMemcpyDeviceToHost));
+    return TBL_OK;
+}
+
+extern "C" int tbl_deriv_dev(tbl_ctx *c, int kind, const double *d_fc, double *d_out)
+{
+    if (!c || !d_fc || !d_out || kind < 0 || kind > 4) return TBL_ERR_ARG;
+    if (d_fc == d_out) return TBL_ERR_ARG;
+    TRY(deriv(c->dargs, kind, d_fc, d_out, c->stream));
+    TBL_HIP(hipGetLastError());
+    return TBL_OK;
+}
+
+static int deriv_host(tbl_ctx *c, int kind, const double *fc, double *out)
+{
+    if (!c || !fc || !out) return TBL_ERR_ARG;
+    TRY(staging(c));
+    const size_t bytes = sizeof(double) * 2 * c->g.nx * c->g.ny * c->K;
+    TBL_HIP(hipMemcpy(c->d_u0, fc, bytes, hipMemcpyHostToDevice));
+    TRY(deriv(c->dargs, kind, c->d_u0, c->d_u1, c->stream));
+    TBL_HIP(hipStreamSynchronize(c->stream));
+    TBL_HIP(hipMemcpy(out, c->d_u1, bytes, hipMemcpyDeviceToHost));
+    return TBL_OK;
+}
+
+extern "C" int tbl_derparx(tbl_ctx *c, const double *fc, double *o) { return deriv_host(c, 0, fc, o); }
+extern "C" int tbl_derparxx(tbl_ctx *c, const double *fc, double *o) { return deriv_host(c, 1, fc, o); }
+extern "C" int tbl_dery(tbl_ctx *c, const double *fc, double *o) { return deriv_host(c, 2, fc, o); }
+extern "C" int tbl_deryy(tbl_ctx *c, const double *fc, double *o) { return deriv_host(c, 3, fc, o); }
+extern "C" int tbl_deryfv(tbl_ctx *c, const double *fc, double *o) { return deriv_host(c, 4, fc, o); }
+
+extern "C" int tbl_residual_dev(tbl_ctx *c, const double *d_wx, const double *d_dwzdx,
+                                const double *d_uy, double *d_out, double *residual)
+{
+    if (!c || !d_dwzdx || !d_uy || !d_out) return TBL_ERR_ARG;
+    hipStream_t st = c->stream;
+    const int K = c->K;
+    const Level &F = c->lv[0];
+    gather(c->g.nx, c->g.ny, K, F, d_wx, d_dwzdx, d_uy, c->g.beta, c->R[0], c->V[0], 0, st);
+    source(c->la_d[0], c->R[0], c->S[0], st);
+    fill(c->d_resmax, K, 0.0, st);
+    fill(reinterpret_cast<double *>(c->R[0]), F.plane * K * 2, 0.0, st);
+    defect<double, double>(c->la_d[0], c->V[0], c->S[0], c->R[0], -1.0, nullptr, c->d_resmax, st);
+    fill(d_out, static_cast<size_t>(c->g.nx) * c->g.ny * K * 2, 0.0, st);
+    scatter(c->g.nx, c->g.ny, K, F, c->R[0], d_out, st);
+    if (residual)
+        TBL_HIP(hipMemcpyAsync(residual, c->d_resmax, sizeof(double) * K, hipMemcpyDeviceToHost, st));
+    TBL_HIP(hipStreamSynchronize(st));
+    return TBL_OK;
+}
+
+// Lanes per line of a level's smoother (the autotuner's knob).  prec 0: FP64
+// smoother, 1: FP32 smoother.
+extern "C" int tbl_set_line_shape(tbl_ctx *c, int prec, int level, int lanes)
+{
+    if (!c || level < 1 || level > c->L || (lanes != 16 && lanes != 32 && lanes != 64))
+        return TBL_ERR_ARG;
+    if (line_rows_for(lanes, c->lv[level - 1].ny) < 0) return TBL_ERR_GRID;
+    (prec ? c->G32 : c->G64)[level - 1] = lanes;
+    return TBL_OK;
+}
+
+extern "C" int tbl_get_line_shape(tbl_ctx *c, int prec, int level)
+{
+    if (!c || level < 1 || level > c->L) return -1;
+    return (prec ? c->G32 : c->G64)[level - 1];
+}
+
+extern "C" int tbl_time_smoother(tbl_ctx *c, int prec, int level, int sweeps, double *ms,
+                                 int *launches)
+{
+    if (!c || !ms || level < 1 || level > c->L || sweeps < 1) return TBL_ERR_ARG;
+    const int l = level - 1;
+    hipStream_t st = c->stream;
+    TBL_HIP(hipEventRecord(c->ev0, st));
+    if (prec)
+        TRY(smooth_sweeps<float>(c->la_f[l], c->E32[l], c->S32[l], c->lu_f[l], sweeps, 1.2f,
+                                 nullptr, c->G32[l], st));
+    else
+        TRY(smooth_sweeps<double>(c->la_d[l], c->V[l], c->S[l], c->lu_d[l], sweeps, 1.2,
+                                  nullptr, c->G64[l], st));
+    TBL_HIP(hipGetLastError());
+    TBL_HIP(hipEventRecord(c->ev1, st));
+    TBL_HIP(hipEventSynchronize(c->ev1));
+    float t = 0.f;
+    TBL_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
+    *ms = t;
+    if (launches) *launches = 4 * sweeps;
+    return TBL_OK;
+}

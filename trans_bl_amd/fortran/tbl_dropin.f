@@ -1,0 +1,175 @@
+cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
+c     Drop-in replacements of trans_bl's v-Poisson and compact
+c     derivative subroutines, running on an MI355X through libtrans_bl_amd.
+c
+c     Same names and argument lists as the reference (poi_uy,
+c     poisson_par.f:6; derparx, derparxx, dery, deryy, deryfv,
+c     derivs_par.f:80-177), same hidden inputs: the PARAMETERs of par.for,
+c     COMMON /var/ (uy, wx) and COMMON /coefs/.  Compile this file instead
+c     of poisson_par.f's poi_uy and derivs_par.f's derivative routines,
+c     with the run's own par.for and comm.* on the include path, and link
+c     tbl_iface.o and libtrans_bl_amd.so.  One rank (np = 1).
+c
+c     The library context is created on the first call, after init_gv has
+c     read coefs.bin into COMMON /coefs/ (main_par.f:852-873).
+cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
+      subroutine tbl_setup
+
+      use tbl_iface
+      implicit none
+      include 'par.for'
+      include 'comm.coef'
+      type(tbl_grid_t) grid
+      type(tbl_coefs_t) coefs
+      integer(c_int) rc
+      integer l
+
+      if (c_associated(tbl_ctx_handle)) return
+      grid%nx = ptsx
+      grid%ny = jmax
+      grid%nmodes = kfour
+      grid%levels = msh
+      grid%dx = dx
+      grid%dy0 = dy
+      grid%stf = stf
+      grid%alpha = alpha
+      grid%beta = beta
+      grid%fac_y = fac_y
+      coefs%fp_fd_e = fp_fd_coef_e
+      coefs%sp_fd = sp_fd_coef
+      coefs%cp_fd = cp_fd_coef
+      coefs%pp_fd = pp_fd_coef
+      coefs%lp_fd = lp_fd_coef
+      coefs%fp_sd = fp_sd_coef
+      coefs%sp_sd = sp_sd_coef
+      coefs%cp_sd = cp_sd_coef
+      coefs%pp_sd = pp_sd_coef
+      coefs%lp_sd = lp_sd_coef
+      coefs%sp_poi = 0.d0
+      coefs%cp_poi = 0.d0
+      coefs%pp_poi = 0.d0
+      coefs%lp_poi = 0.d0
+      do l = 1, msh
+        coefs%sp_poi(:,l) = sp_poi_coef(:,l)
+        coefs%cp_poi(:,l) = cp_poi_coef(:,l)
+        coefs%pp_poi(:,l) = pp_poi_coef(:,l)
+        coefs%lp_poi(:,l) = lp_poi_coef(:,l)
+      end do
+      rc = tbl_create(grid, coefs, tbl_ctx_handle)
+      if (rc .ne. 0) call tbl_fail('tbl_create', rc)
+
+      return
+      end
+
+cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
+      subroutine poi_uy(dwzdx, erromax)
+
+      ! v-Poisson FAS multigrid for every mode (poisson_par.f:6-64):
+      ! reads wx and the initial guess uy from COMMON /var/, writes uy on
+      ! i = 2..ptsx-1, j = 2..jmax and prints (k, V-cycles, residual)
+      use tbl_iface
+      implicit none
+      include 'par.for'
+      include 'comm.var'
+      integer k
+      real*8 erromax
+      complex*16 dwzdx(ptsx,jmax,kfour)
+      integer(c_int) rc, cycles(kfour)
+      real(c_double) residual(kfour)
+      type(tbl_solve_stats_t) stats
+
+      call tbl_setup
+      rc = tbl_poi_uy(tbl_ctx_handle, wx, dwzdx, uy, erromax, tbl_opts,
+     &                cycles, residual, stats)
+      if (rc .ne. 0) call tbl_fail('poi_uy', rc)
+      do k = 1, kfour
+        write(*,*) k, cycles(k), residual(k)
+      end do
+
+      return
+      end
+
+cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
+      subroutine derparx(ddx, fc)
+
+      ! first derivative in x (derivs_par.f:80-95)
+      use tbl_iface
+      implicit none
+      include 'par.for'
+      complex*16 fc(ptsx,jmax,kfour), ddx(ptsx,jmax,kfour)
+      integer(c_int) rc
+
+      call tbl_setup
+      rc = tbl_derparx(tbl_ctx_handle, fc, ddx)
+      if (rc .ne. 0) call tbl_fail('derparx', rc)
+
+      return
+      end
+
+cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
+      subroutine derparxx(d2dx2, fc)
+
+      ! second derivative in x (derivs_par.f:98-113)
+      use tbl_iface
+      implicit none
+      include 'par.for'
+      complex*16 fc(ptsx,jmax,kfour), d2dx2(ptsx,jmax,kfour)
+      integer(c_int) rc
+
+      call tbl_setup
+      rc = tbl_derparxx(tbl_ctx_handle, fc, d2dx2)
+      if (rc .ne. 0) call tbl_fail('derparxx', rc)
+
+      return
+      end
+
+cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
+      subroutine dery(ddy, fc)
+
+      ! first derivative in y (derivs_par.f:116-129)
+      use tbl_iface
+      implicit none
+      include 'par.for'
+      complex*16 fc(ptsx,jmax,kfour), ddy(ptsx,jmax,kfour)
+      integer(c_int) rc
+
+      call tbl_setup
+      rc = tbl_dery(tbl_ctx_handle, fc, ddy)
+      if (rc .ne. 0) call tbl_fail('dery', rc)
+
+      return
+      end
+
+cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
+      subroutine deryy(d2dy2, fc)
+
+      ! second derivative in y (derivs_par.f:164-177)
+      use tbl_iface
+      implicit none
+      include 'par.for'
+      complex*16 fc(ptsx,jmax,kfour), d2dy2(ptsx,jmax,kfour)
+      integer(c_int) rc
+
+      call tbl_setup
+      rc = tbl_deryy(tbl_ctx_handle, fc, d2dy2)
+      if (rc .ne. 0) call tbl_fail('deryy', rc)
+
+      return
+      end
+
+cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
+      subroutine deryfv(ddy, fc)
+
+      ! first derivative in y with dv/dy = 0 at the wall (derivs_par.f:148-161)
+      use tbl_iface
+      implicit none
+      include 'par.for'
+      complex*16 fc(ptsx,jmax,kfour), ddy(ptsx,jmax,kfour)
+      integer(c_int) rc
+
+      call tbl_setup
+      rc = tbl_deryfv(tbl_ctx_handle, fc, ddy)
+      if (rc .ne. 0) call tbl_fail('deryfv', rc)
+
+      return
+      end
