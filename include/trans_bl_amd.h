@@ -160,7 +160,8 @@ int tbl_residual_dev(tbl_ctx *ctx, const double *d_wx, const double *d_dwzdx,
                      const double *d_uy, double *d_out, double *residual);
 
 /* Line-solve shape of one multigrid level's smoother: `lanes` (16, 32 or
-   64) lanes of a wavefront share one y-line.  prec 0 = FP64 smoother,
+   64) lanes of a wavefront share one y-line; a negative value selects the
+   same shape with LDS-staged, coalesced loads and stores.  prec 0 = FP64 smoother,
    1 = FP32 smoother.  Chosen by the autotuner; no reference counterpart
    (the reference solves each line sequentially, penta_par.f:367-404). */
 int tbl_set_line_shape(tbl_ctx *ctx, int prec, int level, int lanes);
@@ -173,6 +174,42 @@ int tbl_get_line_shape(tbl_ctx *ctx, int prec, int level);
    the FP64 smoother, 1 the FP32 one. */
 int tbl_time_smoother(tbl_ctx *ctx, int prec, int level, int sweeps, double *ms,
                       int *launches);
+
+/* ---- learned initial guess (no reference counterpart: the reference
+   warm-starts from the previous stage, poisson_par.f:157-161) ------------ */
+typedef struct tbl_net tbl_net;
+
+/* BF16 U-Net (4 levels, base 32 channels, 3x3) for H x W images (multiples
+   of 8), batches up to max_batch; He-initialised from seed. */
+int tbl_net_create(int H, int W, int max_batch, unsigned seed, tbl_net **out);
+int tbl_net_destroy(tbl_net *net);
+long tbl_net_num_params(const tbl_net *net);
+int tbl_net_get_params(tbl_net *net, float *host);
+int tbl_net_set_params(tbl_net *net, const float *host);
+/* x: device fp32 (N, H, W, 4); y: device fp32 (N, H, W, 2) */
+int tbl_net_forward(tbl_net *net, const float *d_x, float *d_y, int N);
+/* forward + mean-squared-error against t + backward: loss, parameter
+   gradients (host, may be NULL), input gradient (device, may be NULL) */
+int tbl_net_gradients(tbl_net *net, const float *d_x, const float *d_t, int N, float *loss,
+                      float *host_grad, float *d_xgrad);
+/* one Adam step (fp32 master weights, bf16 compute copy) */
+int tbl_net_train_step(tbl_net *net, const float *d_x, const float *d_t, int N, float lr,
+                       float *loss);
+int tbl_net_synchronize(tbl_net *net);
+/* C = A x B on the matrix cores (bf16 in, fp32 out), row-major; with
+   transpose_a A is stored [K, M], with transpose_b B is stored [N, K]. */
+int tbl_mfma_gemm_bf16(int M, int N, int K, const void *d_a, const void *d_b, float *d_c,
+                       int transpose_a, int transpose_b, int splitk);
+
+/* Attach a network to a context; it predicts v on multigrid level `level`
+   (opts.guess = 2 then starts each mode from its prolongated prediction,
+   falling back to zero when the prediction's residual is larger).
+   net == NULL detaches. */
+int tbl_guess_attach(tbl_ctx *ctx, tbl_net *net, int level);
+int tbl_guess_shape(tbl_ctx *ctx, int *H, int *W);
+/* Training pairs from solved modes: f and v device arrays (user layout);
+   writes the network's input (nmodes, H, W, 4) and target (nmodes, H, W, 2). */
+int tbl_guess_pairs(tbl_ctx *ctx, const double *d_f, const double *d_v, float *d_x, float *d_t);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,10 @@ EXPORTS = [
     "tbl_set_stream", "tbl_synchronize", "tbl_poi_uy", "tbl_poi_uy_dev", "tbl_solve_rhs_dev", "tbl_derparx",
     "tbl_derparxx", "tbl_dery", "tbl_deryy", "tbl_deryfv", "tbl_deriv_dev", "tbl_residual_dev",
     "tbl_set_line_shape", "tbl_get_line_shape", "tbl_time_smoother",
+    "tbl_net_create", "tbl_net_destroy", "tbl_net_num_params", "tbl_net_get_params",
+    "tbl_net_set_params", "tbl_net_forward", "tbl_net_gradients", "tbl_net_train_step",
+    "tbl_net_synchronize", "tbl_mfma_gemm_bf16", "tbl_guess_attach", "tbl_guess_shape",
+    "tbl_guess_pairs",
 ]
 
 _lib = None

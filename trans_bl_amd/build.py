@@ -34,6 +34,8 @@ SOURCES = [
     "tbl_mg_kernels.hip",
     "tbl_deriv_kernels.hip",
     "tbl_unet.hip",
+    "tbl_net.cpp",
+    "tbl_guess.hip",
     "tbl_batch.cpp",
 ]
 

@@ -6,6 +6,10 @@
 #include "tbl_host.h"
 #include "tbl_kernels.h"
 
+namespace tbl {
+struct Guess;
+}
+
 struct tbl_ctx {
     tbl_grid g{};
     tbl_coefs c{};
@@ -47,10 +51,12 @@ struct tbl_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     // learned initial guess (optional)
-    struct Guess *guess = nullptr;
+    tbl::Guess *guess = nullptr;
 };
 
 namespace tbl {
+int guess_apply(tbl_ctx *c, hipStream_t st, int *fallback, double *guess_res_max);
+int guess_free(tbl_ctx *c);
 int solve_dev(tbl_ctx *ctx, const double *wx, const double *dz, double *uy, int direct_rhs,
               double tol, const tbl_solve_opts *opts, int *cycles, double *residual,
               tbl_solve_stats *stats);

@@ -1,0 +1,263 @@
+// Learned initial guess for the v-Poisson solve.
+//
+// The U-Net (tbl_net.cpp) works on one coarse multigrid level lg: its input
+// is the source g = kb*wx - dwzdx sampled at that level's points (scaled by
+// the mode's max |g|), kz and the normalised height y/ytop; its output is v
+// at those points scaled by sv = max|g| / (kz^2 + kappa^2).  The prediction
+// is prolongated to the fine grid with the multigrid's own bilinear
+// interpolation (poisson_par.f:877-920).  A mode whose guess has a larger
+// max residual than the zero start falls back to zero; multigrid then runs
+// to the same tolerance either way.
+#include <algorithm>
+#include <cmath>
+
+#include "tbl_ctx.h"
+
+struct tbl_net;
+extern "C" int tbl_net_forward(tbl_net *n, const float *d_x, float *d_y, int N);
+
+namespace tbl {
+
+struct Guess {
+    tbl_net *net = nullptr;
+    int lvl = 0, H = 0, W = 0;
+    double kappa2 = 1.0, ytop = 1.0;
+    float *feat = nullptr, *pred = nullptr;
+    double *sf = nullptr, *res0 = nullptr, *resg = nullptr;
+    int *fallback = nullptr;
+    double *ycoarse = nullptr;    // y of the coarse level's rows
+};
+
+namespace {
+
+// max |g| per mode over the fine grid (internal layout)
+__global__ void mode_absmax_kernel(Level F, int K, const cx<double> *G, double *out)
+{
+    const int k = blockIdx.x;
+    double m = 0.0;
+    const cx<double> *g = G + static_cast<size_t>(k) * F.plane;
+    for (size_t e = threadIdx.x; e < F.plane; e += blockDim.x) {
+        const int j = static_cast<int>(e % F.ld);
+        if (j < F.ny) m = fmax(m, hypot(g[e].re, g[e].im));
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = red[0] > 0 ? red[0] : 1.0;
+}
+
+// features (K, H, W, 4) at level lg from the fine g (stride s = 2^(lg-1))
+__global__ void features_kernel(Level F, Level Cl, int K, int H, int W, int s, const cx<double> *G,
+                                const double *sf, const double *yc, double ytop, double beta,
+                                float kzscale, float *feat)
+{
+    const long total = static_cast<long>(K) * H * W;
+    const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= total) return;
+    const int w = static_cast<int>(e % W), h = static_cast<int>((e / W) % H);
+    const int k = static_cast<int>(e / (static_cast<long>(W) * H));
+    float a = 0.f, b = 0.f, kz = 0.f, yy = 0.f;
+    if (w < Cl.nx && h < Cl.ny) {
+        const cx<double> g = G[static_cast<size_t>(k) * F.plane + static_cast<size_t>(w * s) * F.ld + h * s];
+        a = static_cast<float>(g.re / sf[k]);
+        b = static_cast<float>(g.im / sf[k]);
+        kz = static_cast<float>(k * beta) * kzscale;
+        yy = static_cast<float>(yc[h] / ytop);
+    }
+    float *o = feat + e * 4;
+    o[0] = a; o[1] = b; o[2] = kz; o[3] = yy;
+}
+
+// targets (K, H, W, 2): v at level lg points / sv
+__global__ void targets_kernel(Level F, Level Cl, int K, int H, int W, int s, const cx<double> *V,
+                               const double *sf, double beta, double kappa2, float *tgt)
+{
+    const long total = static_cast<long>(K) * H * W;
+    const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= total) return;
+    const int w = static_cast<int>(e % W), h = static_cast<int>((e / W) % H);
+    const int k = static_cast<int>(e / (static_cast<long>(W) * H));
+    float a = 0.f, b = 0.f;
+    if (w < Cl.nx && h < Cl.ny) {
+        const double kz = k * beta;
+        const double sv = sf[k] / (kz * kz + kappa2);
+        const cx<double> v = V[static_cast<size_t>(k) * F.plane + static_cast<size_t>(w * s) * F.ld + h * s];
+        a = static_cast<float>(v.re / sv);
+        b = static_cast<float>(v.im / sv);
+    }
+    tgt[e * 2] = a;
+    tgt[e * 2 + 1] = b;
+}
+
+// prediction -> V at level lg (interior only; wall row and side columns 0)
+__global__ void place_kernel(Level Cl, int K, int H, int W, const float *pred, const double *sf,
+                             double beta, double kappa2, cx<double> *V)
+{
+    const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= Cl.plane * K) return;
+    const int k = static_cast<int>(idx / Cl.plane);
+    const size_t rem = idx - static_cast<size_t>(k) * Cl.plane;
+    const int i = static_cast<int>(rem / Cl.ld), j = static_cast<int>(rem % Cl.ld);
+    cx<double> v = {0, 0};
+    if (i >= 1 && i <= Cl.nx - 2 && j >= 1 && j < Cl.ny) {
+        const double kz = k * beta;
+        const double sv = sf[k] / (kz * kz + kappa2);
+        const float *p = pred + ((static_cast<long>(k) * H + j) * W + i) * 2;
+        v = {static_cast<double>(p[0]) * sv, static_cast<double>(p[1]) * sv};
+    }
+    V[idx] = v;
+}
+
+// modes whose guess residual is not below the zero start's go back to zero
+__global__ void fallback_kernel(Level F, int K, const double *res0, const double *resg,
+                                cx<double> *V, int *count)
+{
+    const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= F.plane * K) return;
+    const int k = static_cast<int>(idx / F.plane);
+    if (!(resg[k] < res0[k])) {
+        V[idx] = {0, 0};
+        if (idx == static_cast<size_t>(k) * F.plane) atomicAdd(count, 1);
+    }
+}
+
+unsigned nblk(size_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+}  // namespace
+
+int guess_free(tbl_ctx *c)
+{
+    Guess *g = c->guess;
+    if (!g) return TBL_OK;
+    auto fr = [](void *p) { if (p) (void)hipFree(p); };
+    fr(g->feat); fr(g->pred); fr(g->sf); fr(g->res0); fr(g->resg); fr(g->fallback); fr(g->ycoarse);
+    delete g;
+    c->guess = nullptr;
+    return TBL_OK;
+}
+
+static int guess_setup(tbl_ctx *c, tbl_net *net, int lvl)
+{
+    if (lvl < 1 || lvl > c->L) return TBL_ERR_ARG;
+    guess_free(c);
+    Guess *g = new Guess();
+    c->guess = g;
+    g->net = net;
+    g->lvl = lvl;
+    const Level &Cl = c->lv[lvl - 1];
+    g->H = round_up(Cl.ny, 8);
+    g->W = round_up(Cl.nx, 8);
+    std::vector<double> yc(Cl.ny);
+    const int s = 1 << (lvl - 1);
+    for (int j = 0; j < Cl.ny; ++j) {
+        const int jf = j * s;
+        yc[j] = (c->g.stf == 1.0) ? c->g.dy0 * jf
+                                  : c->g.dy0 * (std::pow(c->g.stf, jf) - 1.0) / (c->g.stf - 1.0);
+    }
+    g->ytop = (c->g.stf == 1.0) ? c->g.dy0 * (c->g.ny - 1)
+                                : c->g.dy0 * (std::pow(c->g.stf, c->g.ny - 1) - 1.0) / (c->g.stf - 1.0);
+    g->kappa2 = 4.0 / (g->ytop * g->ytop);
+    const size_t px = static_cast<size_t>(c->K) * g->H * g->W;
+    TBL_HIP(hipMalloc(&g->feat, px * 4 * sizeof(float)));
+    TBL_HIP(hipMalloc(&g->pred, px * 2 * sizeof(float)));
+    TBL_HIP(hipMemset(g->feat, 0, px * 4 * sizeof(float)));
+    TBL_HIP(hipMalloc(&g->sf, c->K * sizeof(double)));
+    TBL_HIP(hipMalloc(&g->res0, c->K * sizeof(double)));
+    TBL_HIP(hipMalloc(&g->resg, c->K * sizeof(double)));
+    TBL_HIP(hipMalloc(&g->fallback, sizeof(int)));
+    TBL_HIP(hipMalloc(&g->ycoarse, Cl.ny * sizeof(double)));
+    TBL_HIP(hipMemcpy(g->ycoarse, yc.data(), Cl.ny * sizeof(double), hipMemcpyHostToDevice));
+    return TBL_OK;
+}
+
+static void build_features(tbl_ctx *c, hipStream_t st)
+{
+    Guess *g = c->guess;
+    const Level &F = c->lv[0], &Cl = c->lv[g->lvl - 1];
+    hipLaunchKernelGGL(mode_absmax_kernel, dim3(c->K), dim3(256), 0, st, F, c->K, c->R[0], g->sf);
+    const long n = static_cast<long>(c->K) * g->H * g->W;
+    hipLaunchKernelGGL(features_kernel, dim3(nblk(n)), dim3(256), 0, st, F, Cl, c->K, g->H, g->W,
+                       1 << (g->lvl - 1), c->R[0], g->sf, g->ycoarse, g->ytop, c->g.beta,
+                       static_cast<float>(std::sqrt(1.0 / g->kappa2)), g->feat);
+}
+
+// Called by the solver after gather+source (R[0] holds g, S[0] the source):
+// writes the guess into V[0] and the fallback count into *fallback.
+int guess_apply(tbl_ctx *c, hipStream_t st, int *fallback, double *guess_res_max)
+{
+    Guess *g = c->guess;
+    if (!g || !g->net) return TBL_ERR_STATE;
+    const Level &F = c->lv[0];
+    const int K = c->K;
+    build_features(c, st);
+    TBL_HIP(hipStreamSynchronize(st));
+    int rc = tbl_net_forward(g->net, g->feat, g->pred, K);
+    if (rc != TBL_OK) return rc;
+    const int L = g->lvl - 1;
+    const Level &Cl = c->lv[L];
+    hipLaunchKernelGGL(place_kernel, dim3(nblk(Cl.plane * K)), dim3(256), 0, st, Cl, K, g->H, g->W,
+                       g->pred, g->sf, c->g.beta, g->kappa2, c->V[L]);
+    for (int l = L; l >= 1; --l) {
+        fill(reinterpret_cast<double *>(c->V[l - 1]), c->lv[l - 1].plane * K * 2, 0.0, st);
+        prolong_level<double>(c->lv[l - 1], c->lv[l], K, c->V[l - 1], c->V[l], nullptr, 0, nullptr, st);
+    }
+    // residual of the guess and of the zero start
+    fill(g->resg, K, 0.0, st);
+    fill(g->res0, K, 0.0, st);
+    defect<double, double>(c->la_d[0], c->V[0], c->S[0], c->R[0], -1.0, nullptr, g->resg, st);
+    fill(reinterpret_cast<double *>(c->W[0]), F.plane * K * 2, 0.0, st);
+    defect<double, double>(c->la_d[0], c->W[0], c->S[0], c->R[0], -1.0, nullptr, g->res0, st);
+    fill_int(g->fallback, 1, 0, st);
+    hipLaunchKernelGGL(fallback_kernel, dim3(nblk(F.plane * K)), dim3(256), 0, st, F, K, g->res0,
+                       g->resg, c->V[0], g->fallback);
+    TBL_HIP(hipGetLastError());
+    std::vector<double> rg(K);
+    TBL_HIP(hipMemcpyAsync(fallback, g->fallback, sizeof(int), hipMemcpyDeviceToHost, st));
+    TBL_HIP(hipMemcpyAsync(rg.data(), g->resg, K * sizeof(double), hipMemcpyDeviceToHost, st));
+    TBL_HIP(hipStreamSynchronize(st));
+    *guess_res_max = *std::max_element(rg.begin(), rg.end());
+    return TBL_OK;
+}
+
+}  // namespace tbl
+
+using namespace tbl;
+
+extern "C" int tbl_guess_attach(tbl_ctx *c, tbl_net *net, int level)
+{
+    if (!c) return TBL_ERR_ARG;
+    if (!net) return guess_free(c);
+    return guess_setup(c, net, level);
+}
+
+extern "C" int tbl_guess_shape(tbl_ctx *c, int *H, int *W)
+{
+    if (!c || !c->guess || !H || !W) return TBL_ERR_STATE;
+    *H = c->guess->H;
+    *W = c->guess->W;
+    return TBL_OK;
+}
+
+// Training pairs from a solved batch: f and v in the user layout (device),
+// features (K, H, W, 4) and targets (K, H, W, 2) written as fp32 (device).
+extern "C" int tbl_guess_pairs(tbl_ctx *c, const double *d_f, const double *d_v, float *d_x,
+                               float *d_t)
+{
+    if (!c || !c->guess || !d_f || !d_v || !d_x || !d_t) return TBL_ERR_ARG;
+    Guess *g = c->guess;
+    hipStream_t st = c->stream;
+    const Level &F = c->lv[0], &Cl = c->lv[g->lvl - 1];
+    gather(c->g.nx, c->g.ny, c->K, F, nullptr, d_f, d_v, c->g.beta, c->R[0], c->V[0], 1, st);
+    build_features(c, st);
+    const long n = static_cast<long>(c->K) * g->H * g->W;
+    hipLaunchKernelGGL(targets_kernel, dim3(nblk(n)), dim3(256), 0, st, F, Cl, c->K, g->H, g->W,
+                       1 << (g->lvl - 1), c->V[0], g->sf, c->g.beta, g->kappa2, d_t);
+    TBL_HIP(hipMemcpyAsync(d_x, g->feat, n * 4 * sizeof(float), hipMemcpyDeviceToDevice, st));
+    TBL_HIP(hipGetLastError());
+    TBL_HIP(hipStreamSynchronize(st));
+    return TBL_OK;
+}

@@ -29,6 +29,7 @@ inline Level make_level(int nx, int ny, double dx2)
 }
 
 int line_rows_for(int G, int ny);
+int set_kernel_attributes();
 
 template <typename T>
 int smooth_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,

@@ -63,10 +63,11 @@ __device__ __forceinline__ cx<T> xfull(const cx<T> *__restrict__ col, int ld, in
 // ---------------------------------------------------------------------------
 // one y-line relaxation by a group of G lanes
 // ---------------------------------------------------------------------------
-template <typename T, int G, int R>
-__device__ __forceinline__ void relax_line(const LevelArgs<T> &L, cx<T> *col,
-                                           const cx<T> *__restrict__ Scol, const T *__restrict__ lu,
-                                           int side, T rf, int lane)
+// The line's data is reached through accessors: getX(j) = off-diagonal x
+// term of row j, getV(j) = current v, getS(j) = source, putV(j, v) = store.
+template <typename T, int G, int R, class FX, class FV, class FS, class FW>
+__device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const T *__restrict__ lu, T rf,
+                                           int lane, FX getX, FV getV, FS getS, FW putV)
 {
     const int ny = L.ny, ld = L.ld;
     const int j0 = lane * R;
@@ -80,7 +81,7 @@ __device__ __forceinline__ void relax_line(const LevelArgs<T> &L, cx<T> *col,
     for (int r = -1; r <= R; ++r) {
         const int j = j0 + r;
         cx<T> x = {T(0), T(0)};
-        if (j >= 0 && j < ny) x = xoff<T>(col, ld, j, side, side == 0 && j >= ny - 2);
+        if (j >= 0 && j < ny) x = getX(j);
         X[r + 1] = x;
     }
     // right-hand sides (rhs_border1/2, rhs_center)
@@ -89,15 +90,15 @@ __device__ __forceinline__ void relax_line(const LevelArgs<T> &L, cx<T> *col,
         const int j = j0 + r;
         cx<T> v = {T(0), T(0)}, rhs = {T(0), T(0)};
         if (j < ny) {
-            v = col[j];
+            v = getV(j);
             if (j == 0) {
                 rhs = v;
             } else if (j == 1) {
-                rhs = Scol[j] - qx * (L.sp[2] * X[r + 2] + L.sp[1] * X[r + 1]);
+                rhs = getS(j) - qx * (L.sp[2] * X[r + 2] + L.sp[1] * X[r + 1]);
             } else if (j <= ny - 3) {
-                rhs = Scol[j] - qx * (L.cp[0] * X[r] + L.cp[1] * X[r + 1] + L.cp[2] * X[r + 2]);
+                rhs = getS(j) - qx * (L.cp[0] * X[r] + L.cp[1] * X[r + 1] + L.cp[2] * X[r + 2]);
             } else {
-                rhs = Scol[j] - qx * X[r + 1];
+                rhs = getS(j) - qx * X[r + 1];
             }
         }
         vo[r] = v;
@@ -190,8 +191,22 @@ __device__ __forceinline__ void relax_line(const LevelArgs<T> &L, cx<T> *col,
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int j = j0 + r;
-        if (j >= 1 && j < ny) col[j] = vo[r] + rf * (y[r] - vo[r]);
+        if (j >= 1 && j < ny) putV(j, vo[r] + rf * (y[r] - vo[r]));
     }
+}
+
+// one y-line relaxation on a column reached directly (global or LDS)
+template <typename T, int G, int R>
+__device__ __forceinline__ void relax_line(const LevelArgs<T> &L, cx<T> *col,
+                                           const cx<T> *__restrict__ Scol, const T *__restrict__ lu,
+                                           int side, T rf, int lane)
+{
+    const int ny = L.ny, ld = L.ld;
+    relax_core<T, G, R>(
+        L, lu, rf, lane,
+        [&](int j) { return xoff<T>(col, ld, j, side, side == 0 && j >= ny - 2); },
+        [&](int j) { return col[j]; }, [&](int j) { return Scol[j]; },
+        [&](int j, cx<T> v) { col[j] = v; });
 }
 
 // column of line c in a phase: phase 0 = the two border columns (Fortran 2
@@ -226,6 +241,58 @@ __global__ void __launch_bounds__(256) smooth_phase_kernel(LevelArgs<T> L, cx<T>
     const size_t base = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * L.ld;
     const T *lut = lu + (static_cast<size_t>(k) * 2 + (side != 0)) * 5 * L.ld;
     relax_line<T, G, R>(L, V + base, S + base, lut, side, rf, lane);
+}
+
+constexpr int kStagedLines = 4;   // lines per workgroup of the staged smoother
+
+// Same relaxation with coalesced global traffic: the group first reads its
+// line lane-strided (row t, t+G, ...: contiguous across the wavefront),
+// forming the off-diagonal x terms, v and S in LDS; the scan then works on
+// lane-contiguous segments from LDS; the relaxed v goes back through LDS and
+// is stored lane-strided.  Arithmetic identical to smooth_phase_kernel.
+template <typename T, int G, int R>
+__global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T> *V,
+                                                            const cx<T> *S, const T *lu,
+                                                            int phase, T rf, const int *active)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int LDW = L.ld;                                // rows held per line
+    cx<T> *base_s = reinterpret_cast<cx<T> *>(smem_raw);
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+    cx<T> *Xs = base_s + static_cast<size_t>(grp) * 3 * LDW;
+    cx<T> *Vs = Xs + LDW, *Ss = Vs + LDW;
+    const int nlines = phase_lines(phase, L.nx);
+    const int gid = blockIdx.x * kStagedLines + grp;
+    const int k = gid / nlines;
+    const bool live = k < L.K && !(active && !active[k]);
+    const int ny = L.ny, ld = L.ld;
+    int side = 0;
+    size_t base = 0;
+    if (live) {
+        const int c = gid - k * nlines;
+        const int i = phase_column(phase, c, L.nx);
+        side = (phase == 0) ? (c == 0 ? -1 : 1) : 0;
+        base = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * ld;
+        const cx<T> *col = V + base;
+        for (int j = lane; j < ny; j += G) {
+            Xs[j] = xoff<T>(col, ld, j, side, side == 0 && j >= ny - 2);
+            Vs[j] = col[j];
+            Ss[j] = S[base + j];
+        }
+    }
+    __syncthreads();
+    if (live) {
+        const T *lut = lu + (static_cast<size_t>(k) * 2 + (side != 0)) * 5 * ld;
+        relax_core<T, G, R>(L, lut, rf, lane, [&](int j) { return Xs[j]; },
+                            [&](int j) { return Vs[j]; }, [&](int j) { return Ss[j]; },
+                            [&](int j, cx<T> v) { Vs[j] = v; });
+    }
+    __syncthreads();
+    if (live) {
+        cx<T> *col = V + base;
+        for (int j = lane; j < ny; j += G)
+            if (j >= 1) col[j] = Vs[j];
+    }
 }
 
 // Coarsest level: one workgroup per mode keeps the whole level in LDS and
@@ -578,16 +645,22 @@ static void launch_phase(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const 
 }
 
 template <typename T, int G, int R>
+static void launch_staged(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int phase,
+                          T rf, const int *active, hipStream_t st)
+{
+    const size_t lines = static_cast<size_t>(phase_lines(phase, L.nx)) * L.K;
+    if (!lines) return;
+    const size_t shm = static_cast<size_t>(kStagedLines) * 3 * L.ld * sizeof(cx<T>);
+    hipLaunchKernelGGL((smooth_staged_kernel<T, G, R>),
+                       dim3(static_cast<unsigned>((lines + kStagedLines - 1) / kStagedLines)),
+                       dim3(kStagedLines * G), shm, st, L, V, S, lu, phase, rf, active);
+}
+
+template <typename T, int G, int R>
 static void launch_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu,
                             int sweeps, T rf, const int *active, hipStream_t st, int threads)
 {
     const size_t shm = L.plane * sizeof(cx<T>);
-    static const bool attr_set = [] {
-        return hipFuncSetAttribute(reinterpret_cast<const void *>(&smooth_coarsest_kernel<T, G, R>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) ==
-               hipSuccess;
-    }();
-    (void)attr_set;
     hipLaunchKernelGGL((smooth_coarsest_kernel<T, G, R>), dim3(L.K), dim3(threads), shm, st, L, V,
                        S, lu, sweeps, rf, active);
 }
@@ -600,6 +673,25 @@ static void launch_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, con
 
 static const int kRs[] = {1, 2, 3, 4, 5, 7, 9, 13, 17};
 
+// Raise the dynamic-LDS limit of every LDS-resident smoother instantiation
+// once, before any stream capture can happen.
+int set_kernel_attributes()
+{
+    hipError_t e = hipSuccess;
+#define X(GG, RR)                                                                             \
+    for (const void *f : {reinterpret_cast<const void *>(&smooth_staged_kernel<double, GG, RR>), \
+                          reinterpret_cast<const void *>(&smooth_staged_kernel<float, GG, RR>),  \
+                          reinterpret_cast<const void *>(&smooth_coarsest_kernel<double, GG, RR>), \
+                          reinterpret_cast<const void *>(&smooth_coarsest_kernel<float, GG, RR>)}) { \
+        hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,     \
+                                           160 * 1024);                                      \
+        if (r != hipSuccess) e = r;                                                           \
+    }
+    TBL_LINE_SHAPES(X)
+#undef X
+    return e == hipSuccess ? TBL_OK : TBL_ERR_HIP;
+}
+
 int line_rows_for(int G, int ny)
 {
     const int need = (ny + G - 1) / G;
@@ -611,18 +703,25 @@ int line_rows_for(int G, int ny)
     return -1;
 }
 
+// G > 0: lanes per line with direct (lane-contiguous) global access;
+// G < 0: |G| lanes per line with the LDS-staged, coalesced kernel
 template <typename T>
 int smooth_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
                   const int *active, int G, hipStream_t st)
 {
+    const bool staged = G < 0;
+    if (staged) G = -G;
     const int R = line_rows_for(G, L.ny);
     if (R < 0) return TBL_ERR_GRID;
+    if (staged && static_cast<size_t>(kStagedLines) * 3 * L.ld * sizeof(cx<T>) > 160 * 1024)
+        return TBL_ERR_GRID;
     for (int it = 0; it < sweeps; ++it)
         for (int ph = 0; ph < 4; ++ph) {
             bool done = false;
 #define X(GG, RR)                                                             \
     if (!done && G == GG && R == RR) {                                        \
-        launch_phase<T, GG, RR>(L, V, S, lu, ph, rf, active, st);             \
+        if (staged) launch_staged<T, GG, RR>(L, V, S, lu, ph, rf, active, st); \
+        else launch_phase<T, GG, RR>(L, V, S, lu, ph, rf, active, st);        \
         done = true;                                                          \
     }
             TBL_LINE_SHAPES(X)
@@ -638,6 +737,7 @@ int smooth_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu
 {
     const size_t shm = L.plane * sizeof(cx<T>);
     if (shm > 160 * 1024) return smooth_sweeps<T>(L, V, S, lu, sweeps, rf, active, G, st);
+    if (G < 0) G = -G;
     const int R = line_rows_for(G, L.ny);
     if (R < 0) return TBL_ERR_GRID;
     bool done = false;

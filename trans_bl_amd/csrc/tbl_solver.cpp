@@ -85,6 +85,10 @@ LevelArgs<T> level_args(const tbl_ctx &c, int l, const T *qdy2)
 // ---------------------------------------------------------------------------
 static int create_impl(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx *c)
 {
+    static std::once_flag attr_once;
+    static int attr_rc = TBL_OK;
+    std::call_once(attr_once, [] { attr_rc = set_kernel_attributes(); });
+    if (attr_rc != TBL_OK) return attr_rc;
     c->g = *grid;
     TRY(check_grid(c->g));
     if (coefs) c->c = *coefs;
@@ -209,6 +213,7 @@ extern "C" int tbl_coefs_generate(double stf, int levels, tbl_coefs *out)
 extern "C" int tbl_destroy(tbl_ctx *c)
 {
     if (!c) return TBL_ERR_ARG;
+    guess_free(c);
     auto fr = [](void *p) { if (p) (void)hipFree(p); };
     for (auto *v : {&c->V, &c->S, &c->W, &c->R})
         for (auto *p : *v) fr(p);
@@ -427,6 +432,9 @@ int solve_dev(tbl_ctx *c, const double *wx, const double *dz, double *uy, int di
     gather(c->g.nx, c->g.ny, K, F, wx, dz, zero_start ? nullptr : uy, c->g.beta, c->R[0], c->V[0],
            direct_rhs, st);
     source(c->la_d[0], c->R[0], c->S[0], st);
+    int fallback = 0;
+    double guess_res = 0.0;
+    if (opts && opts->guess == 2) TRY(guess_apply(c, st, &fallback, &guess_res));
     fill_int(c->d_active, K, 1, st);
     fill_int(c->d_cycles, K, 0, st);
     fill_int(c->d_cycle, 1, 0, st);
@@ -459,6 +467,8 @@ int solve_dev(tbl_ctx *c, const double *wx, const double *dz, double *uy, int di
             stats->nconverged += res[k] < tol ? 1 : 0;
             stats->residual_max = std::max(stats->residual_max, res[k]);
         }
+        stats->fallback = fallback;
+        stats->guess_residual_max = guess_res;
         float ms = 0.f;
         TBL_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
         stats->device_ms = ms;
@@ -560,9 +570,10 @@ extern "C" int tbl_residual_dev(tbl_ctx *c, const double *d_wx, const double *d_
 // smoother, 1: FP32 smoother.
 extern "C" int tbl_set_line_shape(tbl_ctx *c, int prec, int level, int lanes)
 {
-    if (!c || level < 1 || level > c->L || (lanes != 16 && lanes != 32 && lanes != 64))
+    const int a = lanes < 0 ? -lanes : lanes;
+    if (!c || level < 1 || level > c->L || (a != 16 && a != 32 && a != 64))
         return TBL_ERR_ARG;
-    if (line_rows_for(lanes, c->lv[level - 1].ny) < 0) return TBL_ERR_GRID;
+    if (line_rows_for(a, c->lv[level - 1].ny) < 0) return TBL_ERR_GRID;
     (prec ? c->G32 : c->G64)[level - 1] = lanes;
     return TBL_OK;
 }

@@ -1,0 +1,407 @@
+// BF16 U-Net kernels for the learned multigrid initial guess (gfx950).
+//
+// Activations are NHWC (channels fastest).  A 3x3 convolution is an
+// im2col gather followed by a GEMM on the matrix cores
+// (v_mfma_f32_32x32x8_bf16: bf16 operands, fp32 accumulation); the
+// backward pass is two more GEMMs (weights: col^T x dZ, data: dZ x W^T)
+// and a col2im gather.  Weights keep an fp32 master copy for Adam.
+#include "tbl_nn.h"
+
+namespace tbl {
+namespace nn {
+
+typedef short short4v __attribute__((ext_vector_type(4)));
+typedef float float16v __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float bf2f(bf16 h)
+{
+    return __uint_as_float(static_cast<unsigned>(h) << 16);
+}
+
+__device__ __forceinline__ bf16 f2bf(float f)
+{
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return static_cast<bf16>((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+    u += 0x7fffu + ((u >> 16) & 1u);     // round to nearest even
+    return static_cast<bf16>(u >> 16);
+}
+
+// ---------------------------------------------------------------------------
+// GEMM  C[M,N] = alpha * A[M,K] B[K,N] + beta * C   (A, B bf16; C fp32)
+// element (m,k) of A at A[m*sam + k*sak], (k,n) of B at B[k*sbk + n*sbn]
+// Block: 256 threads = 4 wavefronts, tile 64 x 64, K step 32 through LDS;
+// each wavefront owns a 32 x 32 sub-tile computed with 4 MFMAs per K step.
+// ---------------------------------------------------------------------------
+constexpr int BM = 64, BN = 64, BK = 32, PADK = BK + 4;
+
+__global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, const bf16 *A, long sam,
+                                                   long sak, const bf16 *B, long sbk, long sbn,
+                                                   float *C, long ldc, float alpha, float beta,
+                                                   int kchunk)
+{
+    __shared__ bf16 As[BM][PADK];
+    __shared__ bf16 Bs[BN][PADK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    float16v acc;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+    const bool a_kfast = (sak == 1), b_kfast = (sbk == 1);
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = min(K, kbeg + kchunk);
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        // stage A (64 x 32) and B (32 x 64, stored transposed) in LDS
+#pragma unroll
+        for (int e = 0; e < (BM * BK) / 256; ++e) {
+            const int idx = e * 256 + tid;
+            int mm, kk;
+            if (a_kfast) { mm = idx / BK; kk = idx % BK; }
+            else         { kk = idx / BM; mm = idx % BM; }
+            const int gm = m0 + mm, gk = k0 + kk;
+            As[mm][kk] = (gm < M && gk < kend) ? A[gm * sam + gk * sak] : static_cast<bf16>(0);
+        }
+#pragma unroll
+        for (int e = 0; e < (BN * BK) / 256; ++e) {
+            const int idx = e * 256 + tid;
+            int nn, kk;
+            if (b_kfast) { nn = idx / BK; kk = idx % BK; }
+            else         { kk = idx / BN; nn = idx % BN; }
+            const int gn = n0 + nn, gk = k0 + kk;
+            Bs[nn][kk] = (gn < N && gk < kend) ? B[gk * sbk + gn * sbn] : static_cast<bf16>(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 8) {
+            const int kb = kk + (lane >> 5) * 4;
+            short4v a, b;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                a[t] = static_cast<short>(As[wm + (lane & 31)][kb + t]);
+                b[t] = static_cast<short>(Bs[wn + (lane & 31)][kb + t]);
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // D layout: lane holds column (lane & 31); register i holds row
+    // 8*(i/4) + 4*(lane/32) + (i%4)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int row = m0 + wm + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+        const int col = n0 + wn + (lane & 31);
+        if (row < M && col < N) {
+            float *p = C + static_cast<long>(row) * ldc + col;
+            const float v = alpha * acc[i];
+            if (gridDim.z > 1) atomicAdd(p, v);
+            else *p = (beta != 0.f) ? v + beta * *p : v;
+        }
+    }
+}
+
+__global__ void zero2d_kernel(int M, int N, float *C, long ldc)
+{
+    const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= static_cast<long>(M) * N) return;
+    C[(e / N) * ldc + e % N] = 0.f;
+}
+
+// splitk > 1 partitions K over blockIdx.z and accumulates with atomics
+// (C is cleared first; beta must be 0)
+void gemm(int M, int N, int K, const bf16 *A, long sam, long sak, const bf16 *B, long sbk,
+          long sbn, float *C, long ldc, float alpha, float beta, hipStream_t st, int splitk)
+{
+    if (splitk < 1) splitk = 1;
+    int kchunk = (K + splitk - 1) / splitk;
+    kchunk = (kchunk + BK - 1) / BK * BK;
+    splitk = (K + kchunk - 1) / kchunk;
+    if (splitk > 1) {
+        const long mn = static_cast<long>(M) * N;
+        hipLaunchKernelGGL(zero2d_kernel, dim3(static_cast<unsigned>((mn + 255) / 256)), dim3(256), 0,
+                           st, M, N, C, ldc);
+    }
+    dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, splitk);
+    hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, st, M, N, K, A, sam, sak, B, sbk, sbn, C,
+                       ldc, alpha, beta, kchunk);
+}
+
+// ---------------------------------------------------------------------------
+// element-wise and layout kernels
+// ---------------------------------------------------------------------------
+static inline unsigned nb(long n) { return static_cast<unsigned>((n + 255) / 256); }
+
+// col[p, t*C + c] = X[n, h+dh, w+dw, c] (zero outside), taps t = (dh+1)*3 + (dw+1)
+__global__ void im2col_kernel(int Nn, int H, int W, int Cc, const bf16 *X, bf16 *col)
+{
+    const long total = static_cast<long>(Nn) * H * W * 9 * Cc;
+    const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= total) return;
+    const int c = static_cast<int>(e % Cc);
+    long r = e / Cc;
+    const int t = static_cast<int>(r % 9);
+    const long p = r / 9;
+    const int w = static_cast<int>(p % W), h = static_cast<int>((p / W) % H);
+    const int n = static_cast<int>(p / (static_cast<long>(W) * H));
+    const int hh = h + t / 3 - 1, ww = w + t % 3 - 1;
+    bf16 v = 0;
+    if (hh >= 0 && hh < H && ww >= 0 && ww < W)
+        v = X[((static_cast<long>(n) * H + hh) * W + ww) * Cc + c];
+    col[e] = v;
+}
+
+// dX[n,h,w,c] = sum_t dcol[(n, h-dh, w-dw), t*C + c]
+__global__ void col2im_kernel(int Nn, int H, int W, int Cc, const float *dcol, float *dX, int accumulate)
+{
+    const long total = static_cast<long>(Nn) * H * W * Cc;
+    const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= total) return;
+    const int c = static_cast<int>(e % Cc);
+    const long p = e / Cc;
+    const int w = static_cast<int>(p % W), h = static_cast<int>((p / W) % H);
+    const int n = static_cast<int>(p / (static_cast<long>(W) * H));
+    float s = 0.f;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int hh = h - (t / 3 - 1), ww = w - (t % 3 - 1);
+        if (hh >= 0 && hh < H && ww >= 0 && ww < W)
+            s += dcol[(((static_cast<long>(n) * H + hh) * W + ww) * 9 + t) * Cc + c];
+    }
+    dX[e] = accumulate ? dX[e] + s : s;
+}
+
+// y = relu(z + b) -> bf16 (relu = 0: identity, fp32 output kept in z)
+__global__ void bias_act_kernel(long P, int Cc, float *z, const float *b, bf16 *y, int relu)
+{
+    const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= P * Cc) return;
+    float v = z[e] + b[e % Cc];
+    if (relu) v = fmaxf(v, 0.f);
+    z[e] = v;
+    if (y) y[e] = f2bf(v);
+}
+
+// dZ = dY * (y > 0) (relu) or dY, written as bf16 for the GEMMs
+__global__ void relu_back_kernel(long n, const float *dY, const bf16 *y, bf16 *dZ, int relu)
+{
+    const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= n) return;
+    float g = dY[e];
+    if (relu && !(bf2f(y[e]) > 0.f)) g = 0.f;
+    dZ[e] = f2bf(g);
+}
+
+// db[c] = sum_p dZ[p, c]
+__global__ void bias_grad_kernel(long P, int Cc, const bf16 *dZ, float *db)
+{
+    const int c = blockIdx.x;
+    float s = 0.f;
+    for (long p = threadIdx.x; p < P; p += 256) s += bf2f(dZ[p * Cc + c]);
+    __shared__ float red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[c] = red[0];
+}
+
+// 2x2 average pooling, NHWC
+__global__ void pool_kernel(int Nn, int H, int W, int Cc, const bf16 *X, bf16 *Y)
+{
+    const int h2 = H / 2, w2 = W / 2;
+    const long total = static_cast<long>(Nn) * h2 * w2 * Cc;
+    const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= total) return;
+    const int c = static_cast<int>(e % Cc);
+    long p = e / Cc;
+    const int w = static_cast<int>(p % w2), h = static_cast<int>((p / w2) % h2);
+    const int n = static_cast<int>(p / (static_cast<long>(w2) * h2));
+    auto at = [&](int hh, int ww) {
+        return bf2f(X[((static_cast<long>(n) * H + hh) * W + ww) * Cc + c]);
+    };
+    Y[e] = f2bf(0.25f * (at(2 * h, 2 * w) + at(2 * h, 2 * w + 1) + at(2 * h + 1, 2 * w) +
+                         at(2 * h + 1, 2 * w + 1)));
+}
+
+__global__ void pool_back_kernel(int Nn, int H, int W, int Cc, const float *dY, float *dX, int accumulate)
+{
+    const long total = static_cast<long>(Nn) * H * W * Cc;
+    const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= total) return;
+    const int c = static_cast<int>(e % Cc);
+    long p = e / Cc;
+    const int w = static_cast<int>(p % W), h = static_cast<int>((p / W) % H);
+    const int n = static_cast<int>(p / (static_cast<long>(W) * H));
+    const float g = 0.25f * dY[((static_cast<long>(n) * (H / 2) + h / 2) * (W / 2) + w / 2) * Cc + c];
+    dX[e] = accumulate ? dX[e] + g : g;
+}
+
+// Y[:, :, :, 0:C1] = up2(A) (nearest), Y[:, :, :, C1:C1+C2] = Bskip
+__global__ void up_concat_kernel(int Nn, int H, int W, int C1, int C2, const bf16 *A, const bf16 *Bk,
+                                 bf16 *Y)
+{
+    const int Ct = C1 + C2;
+    const long total = static_cast<long>(Nn) * H * W * Ct;
+    const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= total) return;
+    const int c = static_cast<int>(e % Ct);
+    long p = e / Ct;
+    const int w = static_cast<int>(p % W), h = static_cast<int>((p / W) % H);
+    const int n = static_cast<int>(p / (static_cast<long>(W) * H));
+    if (c < C1)
+        Y[e] = A[((static_cast<long>(n) * (H / 2) + h / 2) * (W / 2) + w / 2) * C1 + c];
+    else
+        Y[e] = Bk[p * C2 + (c - C1)];
+}
+
+// split the gradient of up_concat: dA (low res, sum of 4) and dB (skip)
+__global__ void up_concat_back_a_kernel(int Nn, int h, int w, int C1, int C2, const float *dY,
+                                        float *dA)
+{
+    const int Ct = C1 + C2, H = 2 * h, W = 2 * w;
+    const long total = static_cast<long>(Nn) * h * w * C1;
+    const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= total) return;
+    const int c = static_cast<int>(e % C1);
+    long p = e / C1;
+    const int x = static_cast<int>(p % w), y = static_cast<int>((p / w) % h);
+    const int n = static_cast<int>(p / (static_cast<long>(w) * h));
+    float s = 0.f;
+    for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx)
+            s += dY[((static_cast<long>(n) * H + 2 * y + dy) * W + 2 * x + dx) * Ct + c];
+    dA[e] = s;
+}
+
+__global__ void up_concat_back_b_kernel(long P, int C1, int C2, const float *dY, float *dB, int accumulate)
+{
+    const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= P * C2) return;
+    const long p = e / C2;
+    const int c = static_cast<int>(e % C2);
+    const float g = dY[p * (C1 + C2) + C1 + c];
+    dB[e] = accumulate ? dB[e] + g : g;
+}
+
+__global__ void f32_to_bf16_kernel(long n, const float *a, bf16 *b)
+{
+    const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e < n) b[e] = f2bf(a[e]);
+}
+
+// mean squared error over n values; grad = 2 (y - t) / n ; loss into *loss
+__global__ void mse_kernel(long n, const float *y, const float *t, float *grad, float *loss)
+{
+    float s = 0.f;
+    for (long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x; e < n;
+         e += static_cast<long>(gridDim.x) * 256) {
+        const float d = y[e] - t[e];
+        grad[e] = 2.f * d / static_cast<float>(n);
+        s += d * d;
+    }
+    __shared__ float red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0] / static_cast<float>(n));
+}
+
+// Adam on fp32 master weights, refreshed bf16 copy
+__global__ void adam_kernel(long n, float *w, bf16 *wb, const float *g, float *m, float *v,
+                            float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= n) return;
+    const float gg = g[e];
+    const float mm = b1 * m[e] + (1.f - b1) * gg;
+    const float vv = b2 * v[e] + (1.f - b2) * gg * gg;
+    m[e] = mm;
+    v[e] = vv;
+    const float upd = lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    const float nw = w[e] - upd;
+    w[e] = nw;
+    if (wb) wb[e] = f2bf(nw);
+}
+
+__global__ void zero_kernel(long n, float *a)
+{
+    const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e < n) a[e] = 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void im2col(int n, int h, int w, int c, const bf16 *x, bf16 *col, hipStream_t st)
+{
+    const long t = static_cast<long>(n) * h * w * 9 * c;
+    hipLaunchKernelGGL(im2col_kernel, dim3(nb(t)), dim3(256), 0, st, n, h, w, c, x, col);
+}
+void col2im(int n, int h, int w, int c, const float *dcol, float *dx, int acc, hipStream_t st)
+{
+    const long t = static_cast<long>(n) * h * w * c;
+    hipLaunchKernelGGL(col2im_kernel, dim3(nb(t)), dim3(256), 0, st, n, h, w, c, dcol, dx, acc);
+}
+void bias_act(long P, int c, float *z, const float *b, bf16 *y, int relu, hipStream_t st)
+{
+    hipLaunchKernelGGL(bias_act_kernel, dim3(nb(P * c)), dim3(256), 0, st, P, c, z, b, y, relu);
+}
+void relu_back(long n, const float *dy, const bf16 *y, bf16 *dz, int relu, hipStream_t st)
+{
+    hipLaunchKernelGGL(relu_back_kernel, dim3(nb(n)), dim3(256), 0, st, n, dy, y, dz, relu);
+}
+void bias_grad(long P, int c, const bf16 *dz, float *db, hipStream_t st)
+{
+    hipLaunchKernelGGL(bias_grad_kernel, dim3(c), dim3(256), 0, st, P, c, dz, db);
+}
+void pool(int n, int h, int w, int c, const bf16 *x, bf16 *y, hipStream_t st)
+{
+    const long t = static_cast<long>(n) * (h / 2) * (w / 2) * c;
+    hipLaunchKernelGGL(pool_kernel, dim3(nb(t)), dim3(256), 0, st, n, h, w, c, x, y);
+}
+void pool_back(int n, int h, int w, int c, const float *dy, float *dx, int acc, hipStream_t st)
+{
+    const long t = static_cast<long>(n) * h * w * c;
+    hipLaunchKernelGGL(pool_back_kernel, dim3(nb(t)), dim3(256), 0, st, n, h, w, c, dy, dx, acc);
+}
+void up_concat(int n, int h, int w, int c1, int c2, const bf16 *a, const bf16 *b, bf16 *y,
+               hipStream_t st)
+{
+    const long t = static_cast<long>(n) * h * w * (c1 + c2);
+    hipLaunchKernelGGL(up_concat_kernel, dim3(nb(t)), dim3(256), 0, st, n, h, w, c1, c2, a, b, y);
+}
+void up_concat_back(int n, int h, int w, int c1, int c2, const float *dy, float *da, float *db,
+                    int acc_b, hipStream_t st)
+{
+    const long ta = static_cast<long>(n) * (h / 2) * (w / 2) * c1;
+    hipLaunchKernelGGL(up_concat_back_a_kernel, dim3(nb(ta)), dim3(256), 0, st, n, h / 2, w / 2,
+                       c1, c2, dy, da);
+    const long P = static_cast<long>(n) * h * w;
+    hipLaunchKernelGGL(up_concat_back_b_kernel, dim3(nb(P * c2)), dim3(256), 0, st, P, c1, c2, dy,
+                       db, acc_b);
+}
+void to_bf16(long n, const float *a, bf16 *b, hipStream_t st)
+{
+    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(nb(n)), dim3(256), 0, st, n, a, b);
+}
+void mse(long n, const float *y, const float *t, float *grad, float *loss, hipStream_t st)
+{
+    hipLaunchKernelGGL(mse_kernel, dim3(256), dim3(256), 0, st, n, y, t, grad, loss);
+}
+void adam(long n, float *w, bf16 *wb, const float *g, float *m, float *v, float lr, float b1,
+          float b2, float eps, float c1, float c2, hipStream_t st)
+{
+    hipLaunchKernelGGL(adam_kernel, dim3(nb(n)), dim3(256), 0, st, n, w, wb, g, m, v, lr, b1, b2,
+                       eps, c1, c2);
+}
+void zero(long n, float *a, hipStream_t st)
+{
+    if (n > 0) hipLaunchKernelGGL(zero_kernel, dim3(nb(n)), dim3(256), 0, st, n, a);
+}
+
+}  // namespace nn
+}  // namespace tbl
