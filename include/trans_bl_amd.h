@@ -105,6 +105,12 @@ const char *tbl_last_error(void);
    and reading coefs.bin. */
 int tbl_coefs_generate(double stf, int levels, tbl_coefs *out);
 
+/* Read the tables of the path from a coefs.bin (20 sequential unformatted
+   records, coef_generation.f90:39-60; the same records main_par.f:852-873
+   reads).  Rejects a file whose first record is the older 7-entry implicit
+   first-point scheme instead of the 6-entry explicit one. */
+int tbl_coefs_read(const char *path, int levels, tbl_coefs *out);
+
 /* Create a context on the current HIP device: checks the grid like
    verifica (main_par.f:60-134), builds the multigrid metrics (create_ctes),
    the derivative LHS (derivs_k) and the line-LU factors, and allocates the
@@ -174,6 +180,27 @@ int tbl_get_line_shape(tbl_ctx *ctx, int prec, int level);
    the FP64 smoother, 1 the FP32 one. */
 int tbl_time_smoother(tbl_ctx *ctx, int prec, int level, int sweeps, double *ms,
                       int *launches);
+
+/* ---- uneven batches and the captured RK4 stage (no reference counterpart:
+   the reference runs one compile-time grid per executable) ---------------- */
+typedef struct tbl_batch tbl_batch;
+
+/* One context per problem (grids may differ in nx, ny, modes, levels). */
+int tbl_batch_create(int n, const tbl_grid *grids, tbl_batch **out);
+int tbl_batch_destroy(tbl_batch *batch);
+/* Context of problem p (for line shapes, options); owned by the batch. */
+tbl_ctx *tbl_batch_context(tbl_batch *batch, int p);
+/* One RK4 stage of every problem, concurrently: derparx, derparxx, dery,
+   deryy of v (into d[4p+0..3]), then the v-Poisson solve of all modes warm-
+   started from v and written back to v, with `cycles` V-cycles (modes stop
+   at the tolerance through device-side masks).  Device pointers: f[p], v[p],
+   d[4p+q] are complex (nx_p, ny_p, K_p).  use_graph: capture the whole stage
+   once as one hipGraph and replay it.  *ms: GPU time of the stage. */
+int tbl_batch_stage(tbl_batch *batch, const double *const *f, double *const *v,
+                    double *const *d, double tol, int cycles, int precision, int use_graph,
+                    double *ms);
+/* Per-mode completed V-cycles and last residual of problem p's last stage. */
+int tbl_batch_results(tbl_batch *batch, int p, int *cycles, double *residual);
 
 /* ---- learned initial guess (no reference counterpart: the reference
    warm-starts from the previous stage, poisson_par.f:157-161) ------------ */

@@ -52,3 +52,31 @@ def test_coefs_roundtrip_through_c_struct():
     back = tb.coefs_to_dict(tb.coefs_from_dict(d), 4)
     for k in d:
         np.testing.assert_array_equal(d[k], back[k])
+
+
+def test_coefs_bin_reader(tmp_path):
+    """The reader takes the reference's record layout; the shipped fixture
+    has the stale 7-entry first record and is rejected, a file with the
+    current 6-entry record is read back exactly."""
+    import struct
+    from oracle import coefgen
+    fix = os.path.join(os.path.dirname(__file__), "golden", "coefs_stf1.01_lvl4.bin")
+    with pytest.raises(tb.TblError):
+        tb.coefs_read(fix, 4)
+    recs = coefgen.read_coefs_bin(fix)
+    g = coefgen.generate(1.01, 4)
+    recs[0] = g["fp_fd_e"]
+    path = tmp_path / "coefs.bin"
+    with open(path, "wb") as f:
+        for r in recs:
+            b = np.asarray(r, dtype="<f8").tobytes()
+            f.write(struct.pack("<i", len(b)) + b + struct.pack("<i", len(b)))
+    d = tb.coefs_read(str(path), 4)
+    np.testing.assert_array_equal(d["fp_fd_e"], g["fp_fd_e"])
+    for i, k in enumerate(["sp_fd", "cp_fd", "pp_fd", "lp_fd", "fp_sd", "sp_sd", "cp_sd", "pp_sd",
+                           "lp_sd"]):
+        np.testing.assert_array_equal(d[k], recs[i + 1])
+    for i, k in enumerate(["sp_poi", "cp_poi", "pp_poi", "lp_poi"]):
+        np.testing.assert_array_equal(d[k].ravel(), recs[10 + i])
+    with pytest.raises(tb.TblError):
+        tb.coefs_read(str(path), 3)        # wrong level count

@@ -67,14 +67,15 @@ class SolveStats(C.Structure):
 
 # every extern "C" symbol include/trans_bl_amd.h declares
 EXPORTS = [
-    "tbl_version", "tbl_last_error", "tbl_coefs_generate", "tbl_create", "tbl_destroy",
+    "tbl_version", "tbl_last_error", "tbl_coefs_generate", "tbl_coefs_read", "tbl_create", "tbl_destroy",
     "tbl_set_stream", "tbl_synchronize", "tbl_poi_uy", "tbl_poi_uy_dev", "tbl_solve_rhs_dev", "tbl_derparx",
     "tbl_derparxx", "tbl_dery", "tbl_deryy", "tbl_deryfv", "tbl_deriv_dev", "tbl_residual_dev",
     "tbl_set_line_shape", "tbl_get_line_shape", "tbl_time_smoother",
     "tbl_net_create", "tbl_net_destroy", "tbl_net_num_params", "tbl_net_get_params",
     "tbl_net_set_params", "tbl_net_forward", "tbl_net_gradients", "tbl_net_train_step",
     "tbl_net_synchronize", "tbl_mfma_gemm_bf16", "tbl_guess_attach", "tbl_guess_shape",
-    "tbl_guess_pairs",
+    "tbl_guess_pairs", "tbl_batch_create", "tbl_batch_destroy", "tbl_batch_context",
+    "tbl_batch_stage", "tbl_batch_results",
 ]
 
 _lib = None
@@ -89,25 +90,34 @@ def lib():
         raise TblError(f"{LIB_PATH} is missing: run `python -m trans_bl_amd.build` first")
     L = C.CDLL(LIB_PATH)
     vp, ip, dp = C.c_void_p, C.c_int, C.c_double
-    L.tbl_version.restype = C.c_char_p
-    L.tbl_last_error.restype = C.c_char_p
-    L.tbl_coefs_generate.argtypes = [dp, ip, C.POINTER(CCoefs)]
-    L.tbl_create.argtypes = [C.POINTER(CGrid), C.POINTER(CCoefs), C.POINTER(vp)]
-    L.tbl_destroy.argtypes = [vp]
-    L.tbl_set_stream.argtypes = [vp, vp]
-    L.tbl_synchronize.argtypes = [vp]
     sig = [vp, vp, vp, vp, dp, C.POINTER(SolveOpts), vp, vp, C.POINTER(SolveStats)]
-    L.tbl_poi_uy.argtypes = sig
-    L.tbl_poi_uy_dev.argtypes = sig
-    L.tbl_solve_rhs_dev.argtypes = [vp, vp, vp, dp, C.POINTER(SolveOpts), vp, vp,
-                                    C.POINTER(SolveStats)]
+    types = {
+        "tbl_version": ([], C.c_char_p),
+        "tbl_last_error": ([], C.c_char_p),
+        "tbl_coefs_generate": ([dp, ip, C.POINTER(CCoefs)], C.c_int),
+        "tbl_coefs_read": ([C.c_char_p, ip, C.POINTER(CCoefs)], C.c_int),
+        "tbl_create": ([C.POINTER(CGrid), C.POINTER(CCoefs), C.POINTER(vp)], C.c_int),
+        "tbl_destroy": ([vp], C.c_int),
+        "tbl_set_stream": ([vp, vp], C.c_int),
+        "tbl_synchronize": ([vp], C.c_int),
+        "tbl_poi_uy": (sig, C.c_int),
+        "tbl_poi_uy_dev": (sig, C.c_int),
+        "tbl_solve_rhs_dev": ([vp, vp, vp, dp, C.POINTER(SolveOpts), vp, vp,
+                               C.POINTER(SolveStats)], C.c_int),
+        "tbl_deriv_dev": ([vp, ip, vp, vp], C.c_int),
+        "tbl_residual_dev": ([vp, vp, vp, vp, vp, vp], C.c_int),
+        "tbl_set_line_shape": ([vp, ip, ip, ip], C.c_int),
+        "tbl_get_line_shape": ([vp, ip, ip], C.c_int),
+        "tbl_time_smoother": ([vp, ip, ip, ip, C.POINTER(dp), C.POINTER(ip)], C.c_int),
+    }
     for n in ("tbl_derparx", "tbl_derparxx", "tbl_dery", "tbl_deryy", "tbl_deryfv"):
-        getattr(L, n).argtypes = [vp, vp, vp]
-    L.tbl_deriv_dev.argtypes = [vp, ip, vp, vp]
-    L.tbl_residual_dev.argtypes = [vp, vp, vp, vp, vp, vp]
-    L.tbl_set_line_shape.argtypes = [vp, ip, ip, ip]
-    L.tbl_get_line_shape.argtypes = [vp, ip, ip]
-    L.tbl_time_smoother.argtypes = [vp, ip, ip, ip, C.POINTER(dp), C.POINTER(ip)]
+        types[n] = ([vp, vp, vp], C.c_int)
+    # a symbol missing from an older build fails when it is called, not here
+    for name, (args, res) in types.items():
+        if hasattr(L, name):
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
     _lib = L
     return L
 
@@ -123,6 +133,13 @@ def coefs_generate(stf: float, levels: int) -> dict:
     library's host generator."""
     c = CCoefs()
     _check(lib().tbl_coefs_generate(stf, levels, C.byref(c)), "tbl_coefs_generate")
+    return coefs_to_dict(c, levels)
+
+
+def coefs_read(path: str, levels: int) -> dict:
+    """Tables of a coefs.bin file (the reference's coefficient file)."""
+    c = CCoefs()
+    _check(lib().tbl_coefs_read(path.encode(), levels, C.byref(c)), "tbl_coefs_read")
     return coefs_to_dict(c, levels)
 
 

@@ -4,7 +4,9 @@
 #include "tbl_host.h"
 
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace tbl {
@@ -404,6 +406,83 @@ void deriv_lhs(const tbl_grid &g, const tbl_coefs &c, int kind, std::vector<doub
     a[1] = sp[0]; b[1] = sp[1]; cc[1] = sp[2];
     a[n - 2] = pp[2]; b[n - 2] = pp[1]; cc[n - 2] = pp[0];
     a[n - 1] = lp[1]; b[n - 1] = lp[0]; cc[n - 1] = 0;
+}
+
+}  // namespace tbl
+
+// ---------------------------------------------------------------------------
+// coefs.bin reader: the 20 sequential unformatted records written by
+// pre_processing/coef_generation.f90:39-60 and read by main_par.f:852-873
+// (4-byte record markers, little endian).  Records 1-14 are the tables of
+// the hot path; records 15-20 (wall vorticity, integration) are skipped.
+// ---------------------------------------------------------------------------
+#include <cstdio>
+
+namespace tbl {
+
+int read_coefs_bin(const char *path, int levels, tbl_coefs *c, std::string *err)
+{
+    std::FILE *f = std::fopen(path, "rb");
+    if (!f) {
+        *err = "cannot open coefs file";
+        return TBL_ERR_ARG;
+    }
+    std::vector<std::vector<double>> rec;
+    for (;;) {
+        int32_t n1 = 0, n2 = 0;
+        if (std::fread(&n1, 4, 1, f) != 1) break;
+        if (n1 < 0 || n1 % 8) {
+            std::fclose(f);
+            *err = "bad record marker";
+            return TBL_ERR_ARG;
+        }
+        std::vector<double> r(n1 / 8);
+        if (std::fread(r.data(), 1, n1, f) != static_cast<size_t>(n1) ||
+            std::fread(&n2, 4, 1, f) != 1 || n2 != n1) {
+            std::fclose(f);
+            *err = "truncated record";
+            return TBL_ERR_ARG;
+        }
+        rec.push_back(std::move(r));
+    }
+    std::fclose(f);
+    if (rec.size() < 14) {
+        *err = "fewer than 14 records";
+        return TBL_ERR_ARG;
+    }
+    if (rec[0].size() != 6) {
+        *err = "record 1 must hold the 6-entry explicit first-point scheme (a 7-entry record "
+               "is the older implicit scheme)";
+        return TBL_ERR_ARG;
+    }
+    const size_t want[10] = {6, 9, 8, 9, 7, 8, 10, 8, 10, 8};
+    double *dst[10] = {c->fp_fd_e, c->sp_fd, c->cp_fd, c->pp_fd, c->lp_fd,
+                       c->fp_sd, c->sp_sd, c->cp_sd, c->pp_sd, c->lp_sd};
+    std::memset(c, 0, sizeof(*c));
+    for (int i = 0; i < 10; ++i) {
+        if (rec[i].size() != want[i]) {
+            *err = "derivative record " + std::to_string(i + 1) + " has the wrong length";
+            return TBL_ERR_ARG;
+        }
+        std::memcpy(dst[i], rec[i].data(), want[i] * sizeof(double));
+    }
+    const int per[4] = {7, 8, 4, 5};
+    for (int t = 0; t < 4; ++t) {
+        const auto &r = rec[10 + t];
+        if (r.size() != static_cast<size_t>(per[t]) * levels) {
+            *err = "Poisson record " + std::to_string(11 + t) + " does not hold `levels` levels";
+            return TBL_ERR_ARG;
+        }
+        for (int l = 0; l < levels; ++l)
+            for (int e = 0; e < per[t]; ++e) {
+                const double v = r[static_cast<size_t>(l) * per[t] + e];
+                if (t == 0) c->sp_poi[l][e] = v;
+                else if (t == 1) c->cp_poi[l][e] = v;
+                else if (t == 2) c->pp_poi[l][e] = v;
+                else c->lp_poi[l][e] = v;
+            }
+    }
+    return TBL_OK;
 }
 
 }  // namespace tbl

@@ -1,6 +1,7 @@
 // Host-side setup routines of trans_bl_amd (see tbl_host.cpp).
 #pragma once
 
+#include <string>
 #include <vector>
 
 #include "../../include/trans_bl_amd.h"
@@ -25,6 +26,7 @@ void line_lu(const tbl_grid &g, const tbl_coefs &c, const Metrics &m, int lvl, i
 void thomas_factors(const std::vector<double> &a, const std::vector<double> &b,
                     const std::vector<double> &c, std::vector<double> &gam,
                     std::vector<double> &bet);
+int read_coefs_bin(const char *path, int levels, tbl_coefs *c, std::string *err);
 void deriv_lhs(const tbl_grid &g, const tbl_coefs &c, int kind, std::vector<double> &a,
                std::vector<double> &b, std::vector<double> &cc);
 

@@ -296,40 +296,69 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
 }
 
 // Coarsest level: one workgroup per mode keeps the whole level in LDS and
-// runs every sweep (20*itera, poisson_par.f:45) without leaving the CU.
+// runs every sweep (20*itera, poisson_par.f:45) without leaving the CU.  LDS
+// holds v with a compact odd column stride, the mode's line-LU factors and,
+// when it fits, the source.
+__host__ __device__ inline int coarse_stride(int ny) { return ny | 1; }
+
+template <typename T>
+__host__ __device__ inline size_t coarse_lds_bytes(int nx, int ny, bool with_s)
+{
+    const size_t col = static_cast<size_t>(coarse_stride(ny)) * nx * sizeof(cx<T>);
+    return col * (with_s ? 2 : 1) + static_cast<size_t>(10) * ny * sizeof(T);
+}
+
 template <typename T, int G, int R>
 __global__ void __launch_bounds__(1024) smooth_coarsest_kernel(LevelArgs<T> L, cx<T> *V,
                                                                const cx<T> *S, const T *lu,
                                                                int sweeps, T rf,
-                                                               const int *active)
+                                                               const int *active, int s_in_lds)
 {
     extern __shared__ unsigned char smem_raw[];
-    cx<T> *sv = reinterpret_cast<cx<T> *>(smem_raw);
     const int k = blockIdx.x;
     if (active && !active[k]) return;
-    const size_t plane = L.plane;
-    cx<T> *gv = V + static_cast<size_t>(k) * plane;
-    const cx<T> *gs = S + static_cast<size_t>(k) * plane;
-    for (size_t e = threadIdx.x; e < plane; e += blockDim.x) sv[e] = gv[e];
+    const int nx = L.nx, ny = L.ny, ld = L.ld, ls = coarse_stride(ny);
+    cx<T> *sv = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *ss = sv + static_cast<size_t>(ls) * nx;
+    T *slu = reinterpret_cast<T *>(s_in_lds ? ss + static_cast<size_t>(ls) * nx : ss);
+    cx<T> *gv = V + static_cast<size_t>(k) * L.plane;
+    const cx<T> *gs = S + static_cast<size_t>(k) * L.plane;
+    const T *glu = lu + (static_cast<size_t>(k) * 2) * 5 * ld;
+    for (int e = threadIdx.x; e < nx * ny; e += blockDim.x) {
+        const int i = e / ny, j = e % ny;
+        sv[i * ls + j] = gv[static_cast<size_t>(i) * ld + j];
+        if (s_in_lds) ss[i * ls + j] = gs[static_cast<size_t>(i) * ld + j];
+    }
+    for (int e = threadIdx.x; e < 10 * ny; e += blockDim.x)
+        slu[e] = glu[static_cast<size_t>(e / ny) * ld + e % ny];
     __syncthreads();
+    // the line solve indexes the factors with stride L.ld: give it ny here
+    LevelArgs<T> Ls = L;
+    Ls.ld = ny;
     const int groups = blockDim.x / G;
     const int grp = threadIdx.x / G, lane = threadIdx.x % G;
-    const T *luc = lu + (static_cast<size_t>(k) * 2) * 5 * L.ld;
-    const T *lub = luc + 5 * L.ld;
     for (int it = 0; it < sweeps; ++it) {
         for (int phase = 0; phase < 4; ++phase) {
-            const int nl = phase_lines(phase, L.nx);
+            const int nl = phase_lines(phase, nx);
             for (int c = grp; c < nl; c += groups) {
-                const int i = phase_column(phase, c, L.nx);
+                const int i = phase_column(phase, c, nx);
                 const int side = (phase == 0) ? (c == 0 ? -1 : 1) : 0;
-                relax_line<T, G, R>(L, sv + static_cast<size_t>(i) * L.ld,
-                                    gs + static_cast<size_t>(i) * L.ld, side ? lub : luc, side,
-                                    rf, lane);
+                cx<T> *col = sv + static_cast<size_t>(i) * ls;
+                const cx<T> *scol = s_in_lds ? ss + static_cast<size_t>(i) * ls
+                                             : gs + static_cast<size_t>(i) * ld;
+                relax_core<T, G, R>(
+                    Ls, slu + (side ? 5 * ny : 0), rf, lane,
+                    [&](int j) { return xoff<T>(col, ls, j, side, side == 0 && j >= ny - 2); },
+                    [&](int j) { return col[j]; }, [&](int j) { return scol[j]; },
+                    [&](int j, cx<T> v) { col[j] = v; });
             }
             __syncthreads();
         }
     }
-    for (size_t e = threadIdx.x; e < plane; e += blockDim.x) gv[e] = sv[e];
+    for (int e = threadIdx.x; e < nx * ny; e += blockDim.x) {
+        const int i = e / ny, j = e % ny;
+        gv[static_cast<size_t>(i) * ld + j] = sv[i * ls + j];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -660,9 +689,10 @@ template <typename T, int G, int R>
 static void launch_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu,
                             int sweeps, T rf, const int *active, hipStream_t st, int threads)
 {
-    const size_t shm = L.plane * sizeof(cx<T>);
+    const int s_in_lds = coarse_lds_bytes<T>(L.nx, L.ny, true) <= 160 * 1024;
+    const size_t shm = coarse_lds_bytes<T>(L.nx, L.ny, s_in_lds);
     hipLaunchKernelGGL((smooth_coarsest_kernel<T, G, R>), dim3(L.K), dim3(threads), shm, st, L, V,
-                       S, lu, sweeps, rf, active);
+                       S, lu, sweeps, rf, active, s_in_lds);
 }
 
 // (G, R) table shared by both smoother forms
@@ -735,8 +765,8 @@ template <typename T>
 int smooth_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps,
                     T rf, const int *active, int G, hipStream_t st)
 {
-    const size_t shm = L.plane * sizeof(cx<T>);
-    if (shm > 160 * 1024) return smooth_sweeps<T>(L, V, S, lu, sweeps, rf, active, G, st);
+    if (coarse_lds_bytes<T>(L.nx, L.ny, false) > 160 * 1024)
+        return smooth_sweeps<T>(L, V, S, lu, sweeps, rf, active, G, st);
     if (G < 0) G = -G;
     const int R = line_rows_for(G, L.ny);
     if (R < 0) return TBL_ERR_GRID;

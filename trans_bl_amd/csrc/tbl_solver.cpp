@@ -210,6 +210,15 @@ extern "C" int tbl_coefs_generate(double stf, int levels, tbl_coefs *out)
     return generate_coefs(stf, levels, out) ? TBL_OK : TBL_ERR_ARG;
 }
 
+extern "C" int tbl_coefs_read(const char *path, int levels, tbl_coefs *out)
+{
+    if (!path || !out || levels < 1 || levels > TBL_MAX_LEVELS) return TBL_ERR_ARG;
+    std::string err;
+    const int rc = read_coefs_bin(path, levels, out, &err);
+    if (rc != TBL_OK) tbl_set_last_error(err.c_str(), __FILE__, __LINE__);
+    return rc;
+}
+
 extern "C" int tbl_destroy(tbl_ctx *c)
 {
     if (!c) return TBL_ERR_ARG;
@@ -606,3 +615,30 @@ extern "C" int tbl_time_smoother(tbl_ctx *c, int prec, int level, int sweeps, do
     if (launches) *launches = 4 * sweeps;
     return TBL_OK;
 }
+
+namespace tbl {
+
+// One RK4 stage of one context, issued asynchronously on its stream (no host
+// synchronisation, so it can be captured): the four derivatives of v, then
+// the v-Poisson solve with a fixed number of V-cycles, warm-started from v
+// and written back into v.
+int stage_issue(tbl_ctx *c, const double *f, double *v, double *const *d, double tol,
+                const tbl_solve_opts *opts)
+{
+    hipStream_t st = c->stream;
+    const Sched s = sched_of(opts);
+    const int prec = opts ? opts->precision : 0;
+    const int K = c->K;
+    for (int q = 0; q < 4; ++q) TRY(deriv(c->dargs, q, v, d[q], st));
+    gather(c->g.nx, c->g.ny, K, c->lv[0], nullptr, f, v, c->g.beta, c->R[0], c->V[0], 1, st);
+    source(c->la_d[0], c->R[0], c->S[0], st);
+    fill_int(c->d_active, K, 1, st);
+    fill_int(c->d_cycles, K, 0, st);
+    fill_int(c->d_cycle, 1, 0, st);
+    fill(c->d_lastres, K, 0.0, st);
+    for (int cyc = 1; cyc <= s.maxc; ++cyc) TRY(prec ? mixed_cycle(c, s, tol) : fas_cycle(c, s, tol));
+    scatter(c->g.nx, c->g.ny, K, c->lv[0], c->V[0], v, st);
+    return TBL_OK;
+}
+
+}  // namespace tbl

@@ -22,20 +22,26 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 def _setup():
     L = lib()
     vp, ip, fp = C.c_void_p, C.c_int, C.c_float
-    L.tbl_net_create.argtypes = [ip, ip, ip, C.c_uint, C.POINTER(vp)]
-    L.tbl_net_destroy.argtypes = [vp]
-    L.tbl_net_num_params.argtypes = [vp]
-    L.tbl_net_num_params.restype = C.c_long
-    L.tbl_net_get_params.argtypes = [vp, vp]
-    L.tbl_net_set_params.argtypes = [vp, vp]
-    L.tbl_net_forward.argtypes = [vp, vp, vp, ip]
-    L.tbl_net_gradients.argtypes = [vp, vp, vp, ip, C.POINTER(fp), vp, vp]
-    L.tbl_net_train_step.argtypes = [vp, vp, vp, ip, fp, C.POINTER(fp)]
-    L.tbl_net_synchronize.argtypes = [vp]
-    L.tbl_mfma_gemm_bf16.argtypes = [ip, ip, ip, vp, vp, vp, ip, ip, ip]
-    L.tbl_guess_attach.argtypes = [vp, vp, ip]
-    L.tbl_guess_shape.argtypes = [vp, C.POINTER(ip), C.POINTER(ip)]
-    L.tbl_guess_pairs.argtypes = [vp, vp, vp, vp, vp]
+    types = {
+        "tbl_net_create": ([ip, ip, ip, C.c_uint, C.POINTER(vp)], C.c_int),
+        "tbl_net_destroy": ([vp], C.c_int),
+        "tbl_net_num_params": ([vp], C.c_long),
+        "tbl_net_get_params": ([vp, vp], C.c_int),
+        "tbl_net_set_params": ([vp, vp], C.c_int),
+        "tbl_net_forward": ([vp, vp, vp, ip], C.c_int),
+        "tbl_net_gradients": ([vp, vp, vp, ip, C.POINTER(fp), vp, vp], C.c_int),
+        "tbl_net_train_step": ([vp, vp, vp, ip, fp, C.POINTER(fp)], C.c_int),
+        "tbl_net_synchronize": ([vp], C.c_int),
+        "tbl_mfma_gemm_bf16": ([ip, ip, ip, vp, vp, vp, ip, ip, ip], C.c_int),
+        "tbl_guess_attach": ([vp, vp, ip], C.c_int),
+        "tbl_guess_shape": ([vp, C.POINTER(ip), C.POINTER(ip)], C.c_int),
+        "tbl_guess_pairs": ([vp, vp, vp, vp, vp], C.c_int),
+    }
+    for name, (args, res) in types.items():
+        if hasattr(L, name):
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
     return L
 
 

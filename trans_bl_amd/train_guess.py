@@ -77,6 +77,9 @@ def generate(torch, ctx, grid, npairs, H, W, tol, seed0=1000, steps_per_seq=8):
         cyc, res, st = ctx.solve_rhs_dev(f.data_ptr(), u.data_ptr(), tol, precision=1,
                                          max_cycles=40, guess=1)
         cyc_total += int(cyc.sum())
+        if b % 25 == 0:
+            print(json.dumps({"phase": "data", "batch": b, "of": nb, "cycles_mean": float(cyc.mean()),
+                              "residual_max": float(res.max())}), flush=True)
         G.pairs(ctx, f.data_ptr(), u.data_ptr(), X[b * K:].data_ptr(), T[b * K:].data_ptr())
     return X[:npairs], T[:npairs], cyc_total
 
@@ -92,6 +95,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-9)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--minutes", type=float, default=0.0, help="stop training after this long")
     a = ap.parse_args()
     grid = GRIDS[a.grid]
     ctx = tb.Context(grid)
@@ -123,13 +127,21 @@ def main():
             lr = a.lr * 0.5 * (1 + math.cos(math.pi * step / steps))
             losses.append(net.train_step(xb.data_ptr(), tb_.data_ptr(), a.batch, lr))
             step += 1
+            if step % 200 == 0:
+                print(json.dumps({"phase": "train", "step": step, "of": steps,
+                                  "loss": float(np.mean(losses[-50:]))}), flush=True)
+            if a.minutes and time.time() - t0 > 60 * a.minutes:
+                break
         vl = []
         for i in range(0, nval - a.batch + 1, a.batch):
             xb, tb_ = Xv[i:i + a.batch].contiguous(), Tv[i:i + a.batch].contiguous()
             vl.append(net.gradients(xb.data_ptr(), tb_.data_ptr(), a.batch)[0])
+        stop = a.minutes and time.time() - t0 > 60 * a.minutes
         print(json.dumps({"phase": "train", "epoch": ep, "loss": float(np.mean(losses[-50:])),
                           "val_loss": float(np.mean(vl)), "target_var": float(Tv.var()),
                           "seconds": round(time.time() - t1, 1)}), flush=True)
+        if stop:
+            break
     out = a.out or G.weights_path(grid)
     os.makedirs(os.path.dirname(out), exist_ok=True)
     net.save(out)
