@@ -108,8 +108,10 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    from trans_bl_amd import autotune
     K = grid.nmodes
     ctx = tb.Context(grid)
+    tuned = autotune.cached(ctx, sweeps=3)
     fd = torch.from_numpy(f).to("cuda")
     ud = torch.zeros_like(fd)
     torch.cuda.synchronize()
@@ -202,6 +204,8 @@ def main():
                            ms_per_step=round(wall64 / a.steps * 1e3, 3)),
         "speedup_vs_plain_fp64": round(ratem / rate64, 3),
         "flagship_vs_fp64_rel_l2": diff,
+        "line_shapes": {k: {"fp64": v.get("fp64"), "fp32": v.get("fp32")}
+                        for k, v in sorted(tuned.items())},
         "roofline": roof[1],
         "roofline_fp64_smoother": roof[0],
         "cpu_baseline": cpu,

@@ -118,8 +118,9 @@ int tbl_coefs_read(const char *path, int levels, tbl_coefs *out);
 int tbl_create(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx **out);
 int tbl_destroy(tbl_ctx *ctx);
 
-/* Stream for all following work of the context (hipStream_t as void*;
-   NULL = the null stream). */
+/* Stream for all following work of the context (hipStream_t as void*).
+   NULL selects the context's own stream, a blocking stream: it is ordered
+   after work the caller queued on the legacy default stream. */
 int tbl_set_stream(tbl_ctx *ctx, void *hip_stream);
 
 /* Wait until all work the context queued on its stream has finished. */
@@ -167,7 +168,10 @@ int tbl_residual_dev(tbl_ctx *ctx, const double *d_wx, const double *d_dwzdx,
 
 /* Line-solve shape of one multigrid level's smoother: `lanes` (16, 32 or
    64) lanes of a wavefront share one y-line; a negative value selects the
-   same shape with LDS-staged, coalesced loads and stores.  prec 0 = FP64 smoother,
+   same shape with LDS-staged, coalesced loads and stores; 1000 + lanes
+   selects the fused-sweep strip smoother (a whole sweep per launch, one
+   workgroup per strip of columns in LDS; not on the coarsest level).
+   All forms give the same colour-ordered sweep.  prec 0 = FP64 smoother,
    1 = FP32 smoother.  Chosen by the autotuner; no reference counterpart
    (the reference solves each line sequentially, penta_par.f:367-404). */
 int tbl_set_line_shape(tbl_ctx *ctx, int prec, int level, int lanes);

@@ -197,6 +197,52 @@ def test_staged_smoother_matches_direct(torch_mod, precision):
     assert np.array_equal(a, b) and np.array_equal(ca, cb)
 
 
+def _strip_shapes(ctx, g):
+    """Strip form on every level and precision that accepts one (not the
+    coarsest level; rows per lane bounded)."""
+    used = 0
+    for lvl in range(1, g.levels):
+        for prec in (0, 1):
+            for lanes in (1064, 1032, 1016):
+                try:
+                    ctx.set_line_shape(prec, lvl, lanes)
+                    used += 1
+                    break
+                except tb.TblError:
+                    continue
+    assert used >= 2
+
+
+@pytest.mark.parametrize("grid", [SMALL, CFG1], ids=["small", "385x97x8"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_strip_smoother_matches_direct(torch_mod, precision, grid):
+    """The fused-sweep strip smoother (a whole sweep per workgroup in LDS, the
+    halo lines recomputed from a snapshot) gives the phase-by-phase sweep."""
+    f = wl.wavepacket_rhs(grid, tag="strip")
+    z = np.zeros_like(f)
+    with context(grid) as ctx:
+        a, ca, _, _ = ctx.poi_uy(None, -f, z, 1e-9, precision=precision, max_cycles=20)
+        _strip_shapes(ctx, grid)
+        b, cb, _, _ = ctx.poi_uy(None, -f, z, 1e-9, precision=precision, max_cycles=20)
+    assert np.array_equal(ca, cb)
+    assert rel(b, a) < (1e-12 if precision == 0 else 1e-8), rel(b, a)
+
+
+def test_strip_smoother_iterates_match_colour_oracle(torch_mod):
+    """Strip smoother with nonzero boundary data and wx, iterate by iterate
+    against the oracle (the halo snapshot must carry the boundary columns)."""
+    g = SMALL
+    rng = np.random.default_rng(11)
+    f = wl.wavepacket_rhs(g, tag="stripbd")
+    wx = 0.1 * wl.wavepacket_rhs(g, tag="stripwx")
+    uy = 0.01 * (rng.normal(size=f.shape) + 1j * rng.normal(size=f.shape))
+    ref, _, _ = oracle.poi_uy(oparams(g), wx, -f, uy, 0.0, max_cycles=2, order=1)
+    with context(g) as ctx:
+        _strip_shapes(ctx, g)
+        got, _, _, _ = ctx.poi_uy(wx, -f, uy, 0.0, max_cycles=2)
+    assert rel(got, ref) < 1e-12, rel(got, ref)
+
+
 def test_autotuned_shapes_keep_parity(torch_mod):
     from trans_bl_amd import autotune
     g = SMALL

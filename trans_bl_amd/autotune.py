@@ -2,7 +2,8 @@
 
 For every multigrid level and both smoother precisions it times each valid
 line shape -- 16, 32 or 64 lanes of a wavefront per y-line, with direct
-(lane-contiguous) or LDS-staged (coalesced) global access -- on the
+(lane-contiguous) or LDS-staged (coalesced) global access per colour phase,
+or the fused-sweep strip form (whole sweep per launch in LDS) -- on the
 context's own grid with hipEvents (tbl_time_smoother), and keeps the fastest.
 Results are per size class (level shape), so contexts of the same grid reuse
 them through the JSON cache.
@@ -14,7 +15,8 @@ import os
 
 from . import TblError
 
-CANDIDATES = (64, 32, 16, -64, -32, -16)
+# direct, LDS-staged (negative) and fused-sweep strip (1000 + lanes) forms
+CANDIDATES = (64, 32, 16, -64, -32, -16, 1064, 1032, 1016)
 
 
 def size_class(ctx, level):

@@ -67,7 +67,7 @@ extern "C" int tbl_batch_create(int n, const tbl_grid *grids, tbl_batch **out)
         rc = tbl_create(&grids[p], nullptr, &c);
         if (rc == TBL_OK) b->ctx.push_back(c);
     }
-    if (rc == TBL_OK && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    if (rc == TBL_OK && hipStreamCreateWithFlags(&b->stream, hipStreamDefault) != hipSuccess)
         rc = TBL_ERR_HIP;
     for (int e = 0; e < 2 * n + 1 && rc == TBL_OK; ++e) {
         hipEvent_t ev;

@@ -31,6 +31,9 @@ struct tbl_ctx {
     std::vector<tbl::LevelArgs<double>> la_d;
     std::vector<tbl::LevelArgs<float>> la_f;
     std::vector<int> G64, G32;    // lanes per line for each level
+    // halo snapshots of the fused-sweep strip smoother per level
+    std::vector<tbl::cx<double> *> H64;
+    std::vector<tbl::cx<float> *> H32;
 
     int *d_active = nullptr, *d_cycles = nullptr, *d_nactive = nullptr, *d_cycle = nullptr;
     double *d_resmax = nullptr, *d_lastres = nullptr;

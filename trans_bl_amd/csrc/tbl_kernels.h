@@ -5,10 +5,15 @@
 
 namespace tbl {
 
+// fields of the line-LU table per (mode, centre/border), each ld long:
+// l1, l2 (forward multipliers), u1, u2 (super-diagonals of U), pivot,
+// -u1/pivot, -u2/pivot, 1/pivot
+constexpr int kLuFields = 8;
+
 // Per-level constants of the Poisson operator (poisson_par.f:229-793)
 template <typename T>
 struct LevelArgs {
-    int nx, ny, ld, K, jblocks;
+    int nx, ny, ld, K;
     size_t plane;                 // nx * ld
     T sp[7], cp[8], pp[4], lp[5]; // sp_poi_coef(:,l) ... lp_poi_coef(:,l)
     T qx;                         // 1 / (12 dx_l^2 fac_y)
@@ -29,11 +34,26 @@ inline Level make_level(int nx, int ny, double dx2)
 }
 
 int line_rows_for(int G, int ny);
+
+// Fused-sweep strip smoother: line shape kStripShape + G selects it (G lanes
+// per line); halo of kStripHalo columns each side of a strip.
+constexpr int kStripShape = 1000;
+constexpr int kStripHalo = 6;
+struct StripGeom {
+    int width, nstrips;   // strip width in interior columns, number of strips
+};
+// nstrips == 0: the level's columns do not fit the workgroup's LDS
+StripGeom strip_geom(int nx, int ny, int K, size_t esize);
+// snapshot buffer of the strip smoother, in complex elements
+inline size_t strip_halo_elems(const StripGeom &g, int ld, int K)
+{
+    return g.nstrips > 1 ? static_cast<size_t>(K) * (g.nstrips - 1) * 2 * kStripHalo * ld : 0;
+}
 int set_kernel_attributes();
 
 template <typename T>
 int smooth_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
-                  const int *active, int G, hipStream_t st);
+                  const int *active, int G, cx<T> *halo, hipStream_t st);
 template <typename T>
 int smooth_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps,
                     T rf, const int *active, int G, hipStream_t st);

@@ -42,37 +42,49 @@ __device__ __forceinline__ cx<T> xoff(const cx<T> *__restrict__ col, int ld, int
            - T(1) * col[j + 2 * ld];
 }
 
-// Full x stencil of the operator (defect).
-template <typename T>
-__device__ __forceinline__ cx<T> xfull(const cx<T> *__restrict__ col, int ld, int j,
-                                       int side, bool grouped)
-{
-    if (side < 0)
-        return T(10) * col[j - ld] - T(15) * col[j] - T(4) * col[j + ld]
-               + T(14) * col[j + 2 * ld] - T(6) * col[j + 3 * ld] + col[j + 4 * ld];
-    if (side > 0)
-        return T(10) * col[j + ld] - T(15) * col[j] - T(4) * col[j - ld]
-               + T(14) * col[j - 2 * ld] - T(6) * col[j - 3 * ld] + col[j - 4 * ld];
-    if (grouped)
-        return T(-1) * (col[j - 2 * ld] + col[j + 2 * ld])
-               + T(16) * (col[j - ld] + col[j + ld]) - T(30) * col[j];
-    return T(-1) * col[j - 2 * ld] + T(16) * col[j - ld] - T(30) * col[j]
-           + T(16) * col[j + ld] - T(1) * col[j + 2 * ld];
-}
-
 // ---------------------------------------------------------------------------
 // one y-line relaxation by a group of G lanes
 // ---------------------------------------------------------------------------
+// The line-LU factors of the rows a lane owns, held in registers: l1, l2
+// (forward multipliers), u1, u2, pivot (back substitution, as banbk5_poi
+// uses them) and -u1/p, -u2/p, 1/p (the scan's map composition).  Rows past
+// ny get the identity (zeros, unit pivot).
+template <typename T, int R>
+struct LineLU {
+    T l1[R], l2[R], u1[R], u2[R], pv[R], m1[R], m2[R], rp[R];
+};
+
+template <typename T, int R>
+__device__ __forceinline__ LineLU<T, R> load_line_lu(const T *__restrict__ lu, int ld, int j0,
+                                                    int ny)
+{
+    LineLU<T, R> f;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        const bool ok = j < ny;
+        f.l1[r] = ok ? lu[j] : T(0);
+        f.l2[r] = ok ? lu[ld + j] : T(0);
+        f.u1[r] = ok ? lu[2 * ld + j] : T(0);
+        f.u2[r] = ok ? lu[3 * ld + j] : T(0);
+        f.pv[r] = ok ? lu[4 * ld + j] : T(1);
+        f.m1[r] = ok ? lu[5 * ld + j] : T(0);
+        f.m2[r] = ok ? lu[6 * ld + j] : T(0);
+        f.rp[r] = ok ? lu[7 * ld + j] : T(1);
+    }
+    return f;
+}
+
 // The line's data is reached through accessors: getX(j) = off-diagonal x
-// term of row j, getV(j) = current v, getS(j) = source, putV(j, v) = store.
+// term of row j, getV(j) = current v, getS(j, r) = source of row j = lane*R + r,
+// putV(j, v) = store.
 template <typename T, int G, int R, class FX, class FV, class FS, class FW>
-__device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const T *__restrict__ lu, T rf,
+__device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const LineLU<T, R> &f, T rf,
                                            int lane, FX getX, FV getV, FS getS, FW putV)
 {
-    const int ny = L.ny, ld = L.ld;
+    const int ny = L.ny;
     const int j0 = lane * R;
     const T qx = L.qx;
-    const T *l1 = lu, *l2 = lu + ld, *u1 = lu + 2 * ld, *u2 = lu + 3 * ld, *pv = lu + 4 * ld;
 
     cx<T> vo[R], y[R];
     // x off-diagonal terms for rows j0-1 .. j0+R
@@ -94,11 +106,11 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const T *__res
             if (j == 0) {
                 rhs = v;
             } else if (j == 1) {
-                rhs = getS(j) - qx * (L.sp[2] * X[r + 2] + L.sp[1] * X[r + 1]);
+                rhs = getS(j, r) - qx * (L.sp[2] * X[r + 2] + L.sp[1] * X[r + 1]);
             } else if (j <= ny - 3) {
-                rhs = getS(j) - qx * (L.cp[0] * X[r] + L.cp[1] * X[r + 1] + L.cp[2] * X[r + 2]);
+                rhs = getS(j, r) - qx * (L.cp[0] * X[r] + L.cp[1] * X[r + 1] + L.cp[2] * X[r + 2]);
             } else {
-                rhs = getS(j) - qx * X[r + 1];
+                rhs = getS(j, r) - qx * X[r + 1];
             }
         }
         vo[r] = v;
@@ -111,8 +123,7 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const T *__res
     cx<T> c0 = {0, 0}, c1 = {0, 0};
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int j = j0 + r;
-        const T a1 = (j < ny) ? l1[j] : T(0), a2 = (j < ny) ? l2[j] : T(0);
+        const T a1 = f.l1[r], a2 = f.l2[r];
         const T n10 = -a2 * p00 - a1 * p10, n11 = -a2 * p01 - a1 * p11;
         p00 = p10; p01 = p11; p10 = n10; p11 = n11;
         const cx<T> nc1 = (y[r] - a2 * c0) - a1 * c1;
@@ -137,8 +148,7 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const T *__res
         if (lane == 0) { a = {0, 0}; b = {0, 0}; }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int j = j0 + r;
-            const T a1 = (j < ny) ? l1[j] : T(0), a2 = (j < ny) ? l2[j] : T(0);
+            const T a1 = f.l1[r], a2 = f.l2[r];
             const cx<T> yy = (y[r] - a2 * a) - a1 * b;
             a = b; b = yy;
             y[r] = yy;
@@ -151,13 +161,12 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const T *__res
     c0 = {0, 0}; c1 = {0, 0};
 #pragma unroll
     for (int r = R - 1; r >= 0; --r) {
-        const int j = j0 + r;
-        const bool ok = j < ny;
-        const T pj = ok ? pv[j] : T(1), w1 = ok ? u1[j] : T(0), w2 = ok ? u2[j] : T(0);
-        const T m0 = -w1 / pj, m1 = -w2 / pj;
+        // composition only: precomputed -u1/p, -u2/p, 1/p (the replay below
+        // divides by the pivot like the reference)
+        const T m0 = f.m1[r], m1 = f.m2[r], rp = f.rp[r];
         const T n00 = m0 * p00 + m1 * p10, n01 = m0 * p01 + m1 * p11;
         p10 = p00; p11 = p01; p00 = n00; p01 = n01;
-        const cx<T> nc0 = y[r] / pj + m0 * c0 + m1 * c1;
+        const cx<T> nc0 = rp * y[r] + m0 * c0 + m1 * c1;
         c1 = c0; c0 = nc0;
     }
 #pragma unroll
@@ -179,9 +188,7 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const T *__res
         if (lane == G - 1) { a = {0, 0}; b = {0, 0}; }
 #pragma unroll
         for (int r = R - 1; r >= 0; --r) {
-            const int j = j0 + r;
-            const bool ok = j < ny;
-            const T pj = ok ? pv[j] : T(1), w1 = ok ? u1[j] : T(0), w2 = ok ? u2[j] : T(0);
+            const T pj = f.pv[r], w1 = f.u1[r], w2 = f.u2[r];
             const cx<T> xx = ((y[r] - w1 * a) - w2 * b) / pj;
             b = a; a = xx;
             y[r] = xx;
@@ -202,10 +209,11 @@ __device__ __forceinline__ void relax_line(const LevelArgs<T> &L, cx<T> *col,
                                            int side, T rf, int lane)
 {
     const int ny = L.ny, ld = L.ld;
+    const LineLU<T, R> f = load_line_lu<T, R>(lu, ld, lane * R, ny);
     relax_core<T, G, R>(
-        L, lu, rf, lane,
+        L, f, rf, lane,
         [&](int j) { return xoff<T>(col, ld, j, side, side == 0 && j >= ny - 2); },
-        [&](int j) { return col[j]; }, [&](int j) { return Scol[j]; },
+        [&](int j) { return col[j]; }, [&](int j, int) { return Scol[j]; },
         [&](int j, cx<T> v) { col[j] = v; });
 }
 
@@ -239,7 +247,7 @@ __global__ void __launch_bounds__(256) smooth_phase_kernel(LevelArgs<T> L, cx<T>
     const int i = phase_column(phase, c, L.nx);
     const int side = (phase == 0) ? (c == 0 ? -1 : 1) : 0;
     const size_t base = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * L.ld;
-    const T *lut = lu + (static_cast<size_t>(k) * 2 + (side != 0)) * 5 * L.ld;
+    const T *lut = lu + (static_cast<size_t>(k) * 2 + (side != 0)) * kLuFields * L.ld;
     relax_line<T, G, R>(L, V + base, S + base, lut, side, rf, lane);
 }
 
@@ -282,9 +290,10 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
     }
     __syncthreads();
     if (live) {
-        const T *lut = lu + (static_cast<size_t>(k) * 2 + (side != 0)) * 5 * ld;
-        relax_core<T, G, R>(L, lut, rf, lane, [&](int j) { return Xs[j]; },
-                            [&](int j) { return Vs[j]; }, [&](int j) { return Ss[j]; },
+        const T *lut = lu + (static_cast<size_t>(k) * 2 + (side != 0)) * kLuFields * ld;
+        const LineLU<T, R> f = load_line_lu<T, R>(lut, ld, lane * R, ny);
+        relax_core<T, G, R>(L, f, rf, lane, [&](int j) { return Xs[j]; },
+                            [&](int j) { return Vs[j]; }, [&](int j, int) { return Ss[j]; },
                             [&](int j, cx<T> v) { Vs[j] = v; });
     }
     __syncthreads();
@@ -295,41 +304,182 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
     }
 }
 
+// Fused sweep over column strips: one workgroup owns a strip [a, b) of the
+// interior columns of one mode and runs a whole sweep (border phase and the
+// three colour phases) in LDS.  It loads the columns [a-6, b+6) (widened by
+// one to the boundary column when that range stops one short): each colour
+// phase needs its neighbours two columns either side as of the previous
+// phase, so phase 1 relaxes its lines on [a-4, b+4), phase 2 on [a-2, b+2)
+// and phase 3 on [a, b) -- the halo lines are recomputed, not exchanged --
+// and the result on [a, b) equals the phase-by-phase sweep.  Halo columns
+// come from a snapshot taken before the sweep (strip_halo_kernel): the
+// neighbouring workgroups overwrite those columns of v during the sweep.
+__host__ __device__ inline int strip_stride(int ny) { return ny | 1; }
+
+template <typename T>
+__global__ void __launch_bounds__(256) strip_halo_kernel(LevelArgs<T> L, const cx<T> *V,
+                                                         cx<T> *H, int width, int nb,
+                                                         const int *active)
+{
+    const int k = blockIdx.z, s = blockIdx.y;
+    if (active && !active[k]) return;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= kStripHalo * 2 * L.ny) return;
+    const int c = e / L.ny, j = e % L.ny;
+    const int i = 1 + (s + 1) * width - kStripHalo + c;
+    if (i < 1 || i > L.nx - 2) return;
+    H[((static_cast<size_t>(k) * nb + s) * 2 * kStripHalo + c) * L.ld + j] =
+        V[static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * L.ld + j];
+}
+
+template <typename T, int G, int R>
+__global__ void __launch_bounds__(512) smooth_strip_kernel(LevelArgs<T> L, cx<T> *V,
+                                                            const cx<T> *S, const T *lu,
+                                                            const cx<T> *H, T rf,
+                                                            const int *active, int width,
+                                                            int nstrips)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *sv = reinterpret_cast<cx<T> *>(smem_raw);
+    const int k = blockIdx.x / nstrips, sc = blockIdx.x % nstrips;
+    if (active && !active[k]) return;
+    const int nx = L.nx, ny = L.ny, ld = L.ld, ls = strip_stride(ny);
+    const int a = 1 + sc * width, b = min(a + width, nx - 1);
+    int lo = max(a - kStripHalo, 0), hi = min(b + kStripHalo, nx);
+    // a loaded border column (Fortran 2 / nx-1) is relaxed in the border
+    // phase, so its whole one-sided stencil must be loaded too
+    if (lo == 1) lo = 0;
+    if (hi == nx - 1) hi = nx;
+    const int nb = nstrips - 1;
+    cx<T> *gv = V + static_cast<size_t>(k) * L.plane;
+    const cx<T> *gs = S + static_cast<size_t>(k) * L.plane;
+    for (int e = threadIdx.x; e < (hi - lo) * ny; e += blockDim.x) {
+        const int i = lo + e / ny, j = e % ny;
+        const cx<T> *src;
+        if ((i >= a && i < b) || i == 0 || i == nx - 1)
+            src = gv + static_cast<size_t>(i) * ld;
+        else if (i < a)
+            src = H + ((static_cast<size_t>(k) * nb + (sc - 1)) * 2 * kStripHalo
+                       + (i - (a - kStripHalo))) * ld;
+        else
+            src = H + ((static_cast<size_t>(k) * nb + sc) * 2 * kStripHalo
+                       + (i - (b - kStripHalo))) * ld;
+        sv[(i - lo) * ls + j] = src[j];
+    }
+    __syncthreads();
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G, ngrp = blockDim.x / G;
+    const int j0 = lane * R;
+    // lines of each phase: first column, count, column step
+    int i0s[4], cnts[4], steps[4];
+#pragma unroll
+    for (int phase = 0; phase < 4; ++phase) {
+        if (phase == 0) {
+            // border lines: column 2 / nx-1 (Fortran) when their stencils are loaded
+            i0s[0] = (lo == 0) ? 1 : nx - 2;
+            cnts[0] = (lo == 0) + (hi == nx);
+            steps[0] = nx - 3;
+        } else {
+            const int m = 2 * (3 - phase);
+            const int from = max(a - m, 2), to = min(b + m, nx - 2);
+            const int cls = (phase + 1) % 3;
+            i0s[phase] = from + ((cls - from) % 3 + 3) % 3;
+            cnts[phase] = i0s[phase] < to ? (to - 1 - i0s[phase]) / 3 + 1 : 0;
+            steps[phase] = 3;
+        }
+    }
+    // this group's lines in sweep order: (phase, c) with c = grp, grp+ngrp, ...
+    auto advance = [&](int &ph, int &c) {
+        c += ngrp;
+        while (ph < 4 && c >= cnts[ph]) {
+            ++ph;
+            c = grp;
+        }
+    };
+    auto load_s = [&](int ph, int c, cx<T> *out) {
+        const int i = i0s[ph] + c * steps[ph];
+        const cx<T> *scol = gs + static_cast<size_t>(i) * ld;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            out[r] = (j0 + r < ny) ? scol[j0 + r] : cx<T>{T(0), T(0)};
+    };
+    const LineLU<T, R> fc = load_line_lu<T, R>(
+        lu + static_cast<size_t>(k) * 2 * kLuFields * ld, ld, j0, ny);
+    int ph = 0, c = grp;
+    if (c >= cnts[0]) {
+        c -= ngrp;
+        advance(ph, c);
+    }
+    cx<T> Scur[R] = {}, Snext[R] = {};
+    if (ph < 4) load_s(ph, c, Scur);
+    for (int phase = 0; phase < 4; ++phase) {
+        while (ph == phase) {
+            // prefetch the source of this group's next line (read-only)
+            int phn = ph, cn = c;
+            advance(phn, cn);
+            if (phn < 4) load_s(phn, cn, Snext);
+            const int i = i0s[ph] + c * steps[ph];
+            const int side = (ph == 0) ? (i == 1 ? -1 : 1) : 0;
+            cx<T> *col = sv + static_cast<size_t>(i - lo) * ls;
+            auto run = [&](const LineLU<T, R> &f) {
+                relax_core<T, G, R>(
+                    L, f, rf, lane,
+                    [&](int j) { return xoff<T>(col, ls, j, side, side == 0 && j >= ny - 2); },
+                    [&](int j) { return col[j]; }, [&](int, int r) { return Scur[r]; },
+                    [&](int j, cx<T> v) { col[j] = v; });
+            };
+            if (side == 0) {
+                run(fc);
+            } else {
+                run(load_line_lu<T, R>(
+                    lu + (static_cast<size_t>(k) * 2 + 1) * kLuFields * ld, ld, j0, ny));
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) Scur[r] = Snext[r];
+            ph = phn;
+            c = cn;
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < (b - a) * ny; e += blockDim.x) {
+        const int i = a + e / ny, j = e % ny;
+        if (j >= 1) gv[static_cast<size_t>(i) * ld + j] = sv[(i - lo) * ls + j];
+    }
+}
+
 // Coarsest level: one workgroup per mode keeps the whole level in LDS and
 // runs every sweep (20*itera, poisson_par.f:45) without leaving the CU.  LDS
-// holds v with a compact odd column stride, the mode's line-LU factors and,
-// when it fits, the source.
+// holds v with a compact odd column stride and the mode's line-LU factors;
+// the source is read from global memory.  (Scan form, for levels whose lines
+// are too long for the one-thread-per-line kernel below.)
 __host__ __device__ inline int coarse_stride(int ny) { return ny | 1; }
 
 template <typename T>
 __host__ __device__ inline size_t coarse_lds_bytes(int nx, int ny, bool with_s)
 {
     const size_t col = static_cast<size_t>(coarse_stride(ny)) * nx * sizeof(cx<T>);
-    return col * (with_s ? 2 : 1) + static_cast<size_t>(10) * ny * sizeof(T);
+    return col * (with_s ? 2 : 1) + static_cast<size_t>(2 * kLuFields) * ny * sizeof(T);
 }
 
 template <typename T, int G, int R>
-__global__ void __launch_bounds__(1024) smooth_coarsest_kernel(LevelArgs<T> L, cx<T> *V,
+__global__ void __launch_bounds__(512) smooth_coarsest_kernel(LevelArgs<T> L, cx<T> *V,
                                                                const cx<T> *S, const T *lu,
                                                                int sweeps, T rf,
-                                                               const int *active, int s_in_lds)
+                                                               const int *active)
 {
     extern __shared__ unsigned char smem_raw[];
     const int k = blockIdx.x;
     if (active && !active[k]) return;
     const int nx = L.nx, ny = L.ny, ld = L.ld, ls = coarse_stride(ny);
     cx<T> *sv = reinterpret_cast<cx<T> *>(smem_raw);
-    cx<T> *ss = sv + static_cast<size_t>(ls) * nx;
-    T *slu = reinterpret_cast<T *>(s_in_lds ? ss + static_cast<size_t>(ls) * nx : ss);
+    T *slu = reinterpret_cast<T *>(sv + static_cast<size_t>(ls) * nx);
     cx<T> *gv = V + static_cast<size_t>(k) * L.plane;
     const cx<T> *gs = S + static_cast<size_t>(k) * L.plane;
-    const T *glu = lu + (static_cast<size_t>(k) * 2) * 5 * ld;
+    const T *glu = lu + (static_cast<size_t>(k) * 2) * kLuFields * ld;
     for (int e = threadIdx.x; e < nx * ny; e += blockDim.x) {
         const int i = e / ny, j = e % ny;
         sv[i * ls + j] = gv[static_cast<size_t>(i) * ld + j];
-        if (s_in_lds) ss[i * ls + j] = gs[static_cast<size_t>(i) * ld + j];
     }
-    for (int e = threadIdx.x; e < 10 * ny; e += blockDim.x)
+    for (int e = threadIdx.x; e < 2 * kLuFields * ny; e += blockDim.x)
         slu[e] = glu[static_cast<size_t>(e / ny) * ld + e % ny];
     __syncthreads();
     // the line solve indexes the factors with stride L.ld: give it ny here
@@ -344,12 +494,13 @@ __global__ void __launch_bounds__(1024) smooth_coarsest_kernel(LevelArgs<T> L, c
                 const int i = phase_column(phase, c, nx);
                 const int side = (phase == 0) ? (c == 0 ? -1 : 1) : 0;
                 cx<T> *col = sv + static_cast<size_t>(i) * ls;
-                const cx<T> *scol = s_in_lds ? ss + static_cast<size_t>(i) * ls
-                                             : gs + static_cast<size_t>(i) * ld;
+                const cx<T> *scol = gs + static_cast<size_t>(i) * ld;
+                const LineLU<T, R> f =
+                    load_line_lu<T, R>(slu + (side ? kLuFields * ny : 0), ny, lane * R, ny);
                 relax_core<T, G, R>(
-                    Ls, slu + (side ? 5 * ny : 0), rf, lane,
+                    Ls, f, rf, lane,
                     [&](int j) { return xoff<T>(col, ls, j, side, side == 0 && j >= ny - 2); },
-                    [&](int j) { return col[j]; }, [&](int j) { return scol[j]; },
+                    [&](int j) { return col[j]; }, [&](int j, int) { return scol[j]; },
                     [&](int j, cx<T> v) { col[j] = v; });
             }
             __syncthreads();
@@ -361,46 +512,102 @@ __global__ void __launch_bounds__(1024) smooth_coarsest_kernel(LevelArgs<T> L, c
     }
 }
 
+// Coarsest level, one thread per line: each line of a phase is relaxed by one
+// thread with the sequential banded substitution of banbk5_poi
+// (penta_par.f:367-404).  v (and the source when it fits) stay in LDS; the
+// forward-substitution values stay in registers (NY >= ny rows, unrolled);
+// the factors are read with wave-uniform global loads, which cannot alias
+// the LDS stores, so the row loads are scheduled off the serial chain.  The
+// back substitution multiplies by the precomputed -u1/p, -u2/p, 1/p instead
+// of dividing by the pivot (last-bit rounding differs from the division).
+template <typename T>
+__host__ __device__ inline size_t coarse_reg_lds_bytes(int nx, int ny, bool with_s)
+{
+    return static_cast<size_t>(coarse_stride(ny)) * nx * sizeof(cx<T>) * (with_s ? 2 : 1);
+}
+
+// SL: the source is staged in LDS (else read from global memory); a
+// compile-time choice, so every access has a known address space.
+template <typename T, int NY, bool SL>
+__global__ void __launch_bounds__(256) smooth_coarsest_reg_kernel(LevelArgs<T> L, cx<T> *V,
+                                                                   const cx<T> *S, const T *lu,
+                                                                   int sweeps, T rf,
+                                                                   const int *active)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int k = blockIdx.x;
+    if (active && !active[k]) return;
+    const int nx = L.nx, ny = L.ny, ld = L.ld, ls = coarse_stride(ny);
+    cx<T> *sv = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *ss = sv + static_cast<size_t>(ls) * nx;
+    cx<T> *gv = V + static_cast<size_t>(k) * L.plane;
+    const cx<T> *gs = S + static_cast<size_t>(k) * L.plane;
+    for (int e = threadIdx.x; e < nx * ny; e += blockDim.x) {
+        const int i = e / ny, j = e % ny;
+        sv[i * ls + j] = gv[static_cast<size_t>(i) * ld + j];
+        if (SL) ss[i * ls + j] = gs[static_cast<size_t>(i) * ld + j];
+    }
+    __syncthreads();
+    const T *lu_centre = lu + (static_cast<size_t>(k) * 2) * kLuFields * ld;
+    const T *lu_border = lu_centre + kLuFields * ld;
+    const T qx = L.qx;
+    const int c = threadIdx.x;
+    for (int it = 0; it < sweeps; ++it) {
+        for (int phase = 0; phase < 4; ++phase) {
+            if (c < phase_lines(phase, nx)) {
+                const int i = phase_column(phase, c, nx);
+                const int side = (phase == 0) ? (c == 0 ? -1 : 1) : 0;
+                const T *f = phase == 0 ? lu_border : lu_centre;
+                const T *l1 = f, *l2 = f + ld, *m1 = f + 5 * ld, *m2 = f + 6 * ld, *rp = f + 7 * ld;
+                cx<T> *col = sv + static_cast<size_t>(i) * ls;
+                const cx<T> *scol = SL ? ss + static_cast<size_t>(i) * ls
+                                       : gs + static_cast<size_t>(i) * ld;
+                auto X = [&](int j) { return xoff<T>(col, ls, j, side, side == 0 && j >= ny - 2); };
+                cx<T> Y[NY];
+                cx<T> xm = {T(0), T(0)}, x0 = X(0), xp = X(1);
+                cx<T> y1 = {T(0), T(0)}, y2 = {T(0), T(0)};
+#pragma unroll
+                for (int j = 0; j < NY; ++j) {
+                    if (j < ny) {
+                        cx<T> rhs;
+                        if (j == 0) rhs = col[0];
+                        else if (j == 1) rhs = scol[j] - qx * (L.sp[2] * xp + L.sp[1] * x0);
+                        else if (j <= ny - 3)
+                            rhs = scol[j] - qx * (L.cp[0] * xm + L.cp[1] * x0 + L.cp[2] * xp);
+                        else rhs = scol[j] - qx * x0;
+                        const cx<T> yy = (rhs - l2[j] * y2) - l1[j] * y1;
+                        Y[j] = yy;
+                        y2 = y1; y1 = yy;
+                        xm = x0; x0 = xp;
+                        xp = (j + 2 < ny) ? X(j + 2) : cx<T>{T(0), T(0)};
+                    }
+                }
+                cx<T> a = {T(0), T(0)}, b = {T(0), T(0)};
+#pragma unroll
+                for (int j = NY - 1; j >= 0; --j) {
+                    if (j < ny) {
+                        const cx<T> xx = (rp[j] * Y[j] + m2[j] * b) + m1[j] * a;
+                        if (j >= 1) col[j] = col[j] + rf * (xx - col[j]);
+                        b = a; a = xx;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int e = threadIdx.x; e < nx * ny; e += blockDim.x) {
+        const int i = e / ny, j = e % ny;
+        gv[static_cast<size_t>(i) * ld + j] = sv[i * ls + j];
+    }
+}
+
+#define TBL_COARSE_NY(X) X(16) X(32) X(48)
+
 // ---------------------------------------------------------------------------
 // defect (poisson_par.f:543-793): out = S + si * L v on i=2..nx-1, j=2..ny.
 // With resmax != NULL the max |out| of each mode is accumulated (the
 // verifica_poi / get_max_value test, poisson_par.f:67-108, 945-963).
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ cx<T> apply_L(const LevelArgs<T> &L, const cx<T> *col, int j,
-                                         int side, T k2b2, T alpha2)
-{
-    const int ld = L.ld, ny = L.ny;
-    const T scx = T(12) * L.dx2 * L.fac_y;
-    cx<T> b, c, d;
-    T q;
-    if (j == 1) {
-        b = L.sp[2] * xfull<T>(col, ld, j + 1, side, false) + L.sp[1] * xfull<T>(col, ld, j, side, false);
-        c = L.sp[3] * col[j - 1] + L.sp[4] * col[j] + L.sp[5] * col[j + 1] + L.sp[6] * col[j + 2];
-        d = k2b2 * (L.sp[1] * col[j] + L.sp[2] * col[j + 1]);
-        q = L.qdy2[0];
-    } else if (j <= ny - 3) {
-        b = L.cp[0] * xfull<T>(col, ld, j - 1, side, false) + L.cp[1] * xfull<T>(col, ld, j, side, false)
-            + L.cp[2] * xfull<T>(col, ld, j + 1, side, false);
-        c = L.cp[3] * col[j - 2] + L.cp[4] * col[j - 1] + L.cp[5] * col[j] + L.cp[6] * col[j + 1]
-            + L.cp[7] * col[j + 2];
-        d = k2b2 * (L.cp[0] * col[j - 1] + L.cp[1] * col[j] + L.cp[2] * col[j + 1]);
-        q = L.qdy2[j - 2];
-    } else if (j == ny - 2) {
-        b = xfull<T>(col, ld, j, side, side == 0);
-        c = L.pp[3] * col[j + 1] + L.pp[2] * col[j] + L.pp[1] * col[j - 1];
-        d = k2b2 * col[j];
-        q = L.qdy2[j - 1];
-    } else {
-        b = xfull<T>(col, ld, j, side, side == 0);
-        c = L.lp[2] * col[j] + L.lp[3] * col[j - 1] + L.lp[4] * col[j - 2];
-        d = k2b2 * col[j];
-        q = L.qdy2[j - 2];
-        return b / scx + c * q + d / L.fac_y - L.lp[1] * alpha2 * col[j] * L.sqq_top;
-    }
-    return b / scx + c * q + d / L.fac_y;
-}
-
 template <typename T>
 __device__ __forceinline__ void atomic_max_nonneg(double *addr, double v)
 {
@@ -408,37 +615,100 @@ __device__ __forceinline__ void atomic_max_nonneg(double *addr, double v)
     atomicMax(reinterpret_cast<unsigned long long *>(addr), __double_as_longlong(v));
 }
 
+// Defect by column marching: one wavefront owns kDefRows consecutive output
+// rows of one strip of columns of one mode.  Its 64 lanes hold rows
+// j0-2 .. j0+61 (two halo rows each side); it walks the strip in x keeping
+// the five columns i-2..i+2 of its rows in registers (one new column load per
+// step) and takes the y-neighbours from lane shuffles, so every value of v is
+// loaded about once.  Row forms: Fortran row 2 (sp_poi_coef), interior rows
+// (cp_poi_coef), row ny-1 (pp_poi_coef) and row ny (lp_poi_coef, with the
+// top boundary term); x part: the border-column one-sided stencils on
+// columns 2 and nx-1, the centred fourth-order stencil elsewhere (grouped
+// on the two top rows).
+constexpr int kDefRows = 60;
+
+__host__ __device__ inline int defect_strip(int nx) { return nx >= 512 ? 32 : 16; }
+
 template <typename T, typename TO>
-__global__ void __launch_bounds__(256) defect_kernel(LevelArgs<T> L, const cx<T> *V,
-                                                     const cx<T> *S, cx<TO> *out, T si,
-                                                     const int *active, double *resmax)
+__global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
+                                                           const cx<T> *__restrict__ V,
+                                                           const cx<T> *S, cx<TO> *out, T si,
+                                                           const int *active, double *resmax,
+                                                           int strip, int nstrips, int jchunks)
 {
-    // block: 64 rows (j) x 4 columns (i); grid.y = mode
-    const int j = blockIdx.x % L.jblocks * 64 + threadIdx.x % 64;
-    const int i = (blockIdx.x / L.jblocks) * 4 + threadIdx.x / 64;
+    const int lane = threadIdx.x & 63;
+    const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int k = blockIdx.y;
+    if (item >= jchunks * nstrips) return;          // whole wavefront
+    if (active && !active[k]) return;
+    const int nx = L.nx, ny = L.ny, ld = L.ld;
+    const int jc = item % jchunks, sc = item / jchunks;
+    const int j = 1 + jc * kDefRows - 2 + lane;
+    const bool jin = j >= 0 && j < ny;
+    const bool own = lane >= 2 && lane < 2 + kDefRows && j >= 1 && j < ny;
+    const int ia = 1 + sc * strip, ib = min(ia + strip, nx - 1);
+    const cx<T> *vk = V + static_cast<size_t>(k) * L.plane;
+    auto vload = [&](int i) -> cx<T> {
+        return (jin && i >= 0 && i < nx) ? vk[static_cast<size_t>(i) * ld + j] : cx<T>{T(0), T(0)};
+    };
+    const T kk = static_cast<T>(k);
+    const T k2b2 = -kk * kk * L.beta * L.beta;
+    const T alpha2 = sqrt((L.alpha * L.alpha - k2b2) / L.fac_y);
+    const T scx = T(12) * L.dx2 * L.fac_y;
+    T q = T(0);
+    if (own) q = (j == 1) ? L.qdy2[0] : (j == ny - 2) ? L.qdy2[j - 1] : L.qdy2[j - 2];
+    cx<T> w0 = vload(ia - 2), w1 = vload(ia - 1), w2 = vload(ia), w3 = vload(ia + 1),
+          w4 = vload(ia + 2);
     double amax = 0.0;
-    const bool live = !(active && !active[k]);
-    if (live && i >= 1 && i <= L.nx - 2 && j >= 1 && j < L.ny) {
-        const size_t base = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * L.ld;
-        const int side = (i == 1) ? -1 : (i == L.nx - 2) ? 1 : 0;
-        const T kk = static_cast<T>(k);
-        const T k2b2 = -kk * kk * L.beta * L.beta;
-        const T alpha2 = sqrt((L.alpha * L.alpha - k2b2) / L.fac_y);
-        const cx<T> lv = apply_L<T>(L, V + base, j, side, k2b2, alpha2);
-        const cx<T> r = S[base + j] + si * lv;
-        out[base + j] = cx<TO>{static_cast<TO>(r.re), static_cast<TO>(r.im)};
-        if (resmax) amax = hypot(static_cast<double>(r.re), static_cast<double>(r.im));
+    for (int i = ia; i < ib; ++i) {
+        const cx<T> nxt = vload(i + 3);
+        cx<T> xs, xo;   // x part at row j: shared (ungrouped) and own
+        if (i == 1) {
+            xs = T(10) * w1 - T(15) * w2 - T(4) * w3 + T(14) * w4 - T(6) * vload(4) + vload(5);
+            xo = xs;
+        } else if (i == nx - 2) {
+            xs = T(10) * w3 - T(15) * w2 - T(4) * w1 + T(14) * w0 - T(6) * vload(nx - 5)
+                 + vload(nx - 6);
+            xo = xs;
+        } else {
+            xs = T(-1) * w0 + T(16) * w1 - T(30) * w2 + T(16) * w3 - T(1) * w4;
+            xo = (j >= ny - 2) ? T(-1) * (w0 + w4) + T(16) * (w1 + w3) - T(30) * w2 : xs;
+        }
+        const cx<T> xm = shfl_up(xs, 1, 64), xp = shfl_down(xs, 1, 64);
+        const cx<T> vm1 = shfl_up(w2, 1, 64), vm2 = shfl_up(w2, 2, 64);
+        const cx<T> vp1 = shfl_down(w2, 1, 64), vp2 = shfl_down(w2, 2, 64);
+        if (own) {
+            cx<T> lv;
+            if (j == 1) {
+                const cx<T> b = L.sp[2] * xp + L.sp[1] * xs;
+                const cx<T> c = L.sp[3] * vm1 + L.sp[4] * w2 + L.sp[5] * vp1 + L.sp[6] * vp2;
+                const cx<T> d = k2b2 * (L.sp[1] * w2 + L.sp[2] * vp1);
+                lv = b / scx + c * q + d / L.fac_y;
+            } else if (j <= ny - 3) {
+                const cx<T> b = L.cp[0] * xm + L.cp[1] * xs + L.cp[2] * xp;
+                const cx<T> c = L.cp[3] * vm2 + L.cp[4] * vm1 + L.cp[5] * w2 + L.cp[6] * vp1
+                                + L.cp[7] * vp2;
+                const cx<T> d = k2b2 * (L.cp[0] * vm1 + L.cp[1] * w2 + L.cp[2] * vp1);
+                lv = b / scx + c * q + d / L.fac_y;
+            } else if (j == ny - 2) {
+                const cx<T> c = L.pp[3] * vp1 + L.pp[2] * w2 + L.pp[1] * vm1;
+                const cx<T> d = k2b2 * w2;
+                lv = xo / scx + c * q + d / L.fac_y;
+            } else {
+                const cx<T> c = L.lp[2] * w2 + L.lp[3] * vm1 + L.lp[4] * vm2;
+                const cx<T> d = k2b2 * w2;
+                lv = xo / scx + c * q + d / L.fac_y - L.lp[1] * alpha2 * w2 * L.sqq_top;
+            }
+            const size_t o = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * ld + j;
+            const cx<T> r = S[o] + si * lv;
+            out[o] = cx<TO>{static_cast<TO>(r.re), static_cast<TO>(r.im)};
+            if (resmax) amax = fmax(amax, hypot(static_cast<double>(r.re), static_cast<double>(r.im)));
+        }
+        w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = nxt;
     }
     if (resmax) {
-        __shared__ double red[4];
         for (int o = 32; o > 0; o >>= 1) amax = fmax(amax, __shfl_down(amax, o, 64));
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = amax;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-            if (live) atomic_max_nonneg<T>(resmax + k, m);
-        }
+        if (lane == 0) atomic_max_nonneg<T>(resmax + k, amax);
     }
 }
 
@@ -686,13 +956,30 @@ static void launch_staged(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const
 }
 
 template <typename T, int G, int R>
+static void launch_strip(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, cx<T> *H,
+                         const StripGeom &sg, T rf, const int *active, hipStream_t st)
+{
+    const int nb = sg.nstrips - 1;
+    if (nb > 0)
+        hipLaunchKernelGGL((strip_halo_kernel<T>),
+                           dim3((2 * kStripHalo * L.ny + 255) / 256, nb, L.K), dim3(256), 0, st,
+                           L, V, H, sg.width, nb, active);
+    const int lines = (sg.width + 4 * 2 + 2) / 3 + 1;
+    const int groups = std::min(512 / G, std::max(lines, 1));
+    const int threads = (groups * G + 63) / 64 * 64;
+    const size_t shm = static_cast<size_t>(std::min(sg.width + 2 * kStripHalo + 2, L.nx))
+                       * strip_stride(L.ny) * sizeof(cx<T>);
+    hipLaunchKernelGGL((smooth_strip_kernel<T, G, R>), dim3(L.K * sg.nstrips), dim3(threads), shm,
+                       st, L, V, S, lu, H, rf, active, sg.width, sg.nstrips);
+}
+
+template <typename T, int G, int R>
 static void launch_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu,
                             int sweeps, T rf, const int *active, hipStream_t st, int threads)
 {
-    const int s_in_lds = coarse_lds_bytes<T>(L.nx, L.ny, true) <= 160 * 1024;
-    const size_t shm = coarse_lds_bytes<T>(L.nx, L.ny, s_in_lds);
+    const size_t shm = coarse_lds_bytes<T>(L.nx, L.ny, false);
     hipLaunchKernelGGL((smooth_coarsest_kernel<T, G, R>), dim3(L.K), dim3(threads), shm, st, L, V,
-                       S, lu, sweeps, rf, active, s_in_lds);
+                       S, lu, sweeps, rf, active);
 }
 
 // (G, R) table shared by both smoother forms
@@ -712,14 +999,46 @@ int set_kernel_attributes()
     for (const void *f : {reinterpret_cast<const void *>(&smooth_staged_kernel<double, GG, RR>), \
                           reinterpret_cast<const void *>(&smooth_staged_kernel<float, GG, RR>),  \
                           reinterpret_cast<const void *>(&smooth_coarsest_kernel<double, GG, RR>), \
-                          reinterpret_cast<const void *>(&smooth_coarsest_kernel<float, GG, RR>)}) { \
+                          reinterpret_cast<const void *>(&smooth_coarsest_kernel<float, GG, RR>),  \
+                          reinterpret_cast<const void *>(&smooth_strip_kernel<double, GG, RR>),    \
+                          reinterpret_cast<const void *>(&smooth_strip_kernel<float, GG, RR>)}) { \
         hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,     \
                                            160 * 1024);                                      \
         if (r != hipSuccess) e = r;                                                           \
     }
     TBL_LINE_SHAPES(X)
 #undef X
+#define X(NY)                                                                                 \
+    for (const void *f :                                                                      \
+         {reinterpret_cast<const void *>(&smooth_coarsest_reg_kernel<double, NY, true>),       \
+          reinterpret_cast<const void *>(&smooth_coarsest_reg_kernel<double, NY, false>),      \
+          reinterpret_cast<const void *>(&smooth_coarsest_reg_kernel<float, NY, true>),        \
+          reinterpret_cast<const void *>(&smooth_coarsest_reg_kernel<float, NY, false>)}) {    \
+        hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,     \
+                                           160 * 1024);                                      \
+        if (r != hipSuccess) e = r;                                                           \
+    }
+    TBL_COARSE_NY(X)
+#undef X
     return e == hipSuccess ? TBL_OK : TBL_ERR_HIP;
+}
+
+StripGeom strip_geom(int nx, int ny, int K, size_t esize)
+{
+    StripGeom g{0, 0};
+    const size_t col = static_cast<size_t>(strip_stride(ny)) * 2 * esize;
+    const int maxcols = static_cast<int>(160 * 1024 / col);
+    const int wmax = maxcols - 2 * kStripHalo - 2;
+    const int n_int = nx - 2;
+    if (wmax < 2 * kStripHalo) return g;
+    int n = (n_int + wmax - 1) / wmax;
+    // enough workgroups for the chip, strips at least 2*halo wide
+    const int want = (512 + K - 1) / K;
+    n = std::max(n, std::min(want, n_int / (2 * kStripHalo)));
+    n = std::max(n, 1);
+    g.width = (n_int + n - 1) / n;
+    g.nstrips = (n_int + g.width - 1) / g.width;
+    return g;
 }
 
 int line_rows_for(int G, int ny)
@@ -737,8 +1056,27 @@ int line_rows_for(int G, int ny)
 // G < 0: |G| lanes per line with the LDS-staged, coalesced kernel
 template <typename T>
 int smooth_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
-                  const int *active, int G, hipStream_t st)
+                  const int *active, int G, cx<T> *halo, hipStream_t st)
 {
+    if (G > kStripShape) {
+        // fused-sweep strips
+        G -= kStripShape;
+        const int R = line_rows_for(G, L.ny);
+        const StripGeom sg = strip_geom(L.nx, L.ny, L.K, sizeof(T));
+        if (R < 0 || sg.nstrips < 1 || (sg.nstrips > 1 && !halo)) return TBL_ERR_GRID;
+        for (int it = 0; it < sweeps; ++it) {
+            bool done = false;
+#define X(GG, RR)                                                                       \
+    if (!done && G == GG && R == RR) {                                                  \
+        launch_strip<T, GG, RR>(L, V, S, lu, halo, sg, rf, active, st);                 \
+        done = true;                                                                    \
+    }
+            TBL_LINE_SHAPES(X)
+#undef X
+            if (!done) return TBL_ERR_GRID;
+        }
+        return TBL_OK;
+    }
     const bool staged = G < 0;
     if (staged) G = -G;
     const int R = line_rows_for(G, L.ny);
@@ -765,15 +1103,37 @@ template <typename T>
 int smooth_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps,
                     T rf, const int *active, int G, hipStream_t st)
 {
+    if (L.ny <= 48 && coarse_reg_lds_bytes<T>(L.nx, L.ny, false) <= 160 * 1024 &&
+        phase_lines(1, L.nx) <= 256) {
+        const int s_in = coarse_reg_lds_bytes<T>(L.nx, L.ny, true) <= 160 * 1024;
+        const int threads = (phase_lines(1, L.nx) + 63) / 64 * 64;
+        const size_t shm = coarse_reg_lds_bytes<T>(L.nx, L.ny, s_in);
+        bool done = false;
+#define X(NY)                                                                                 \
+    if (!done && L.ny <= NY) {                                                                \
+        if (s_in)                                                                             \
+            hipLaunchKernelGGL((smooth_coarsest_reg_kernel<T, NY, true>), dim3(L.K),          \
+                               dim3(threads), shm, st, L, V, S, lu, sweeps, rf, active);      \
+        else                                                                                  \
+            hipLaunchKernelGGL((smooth_coarsest_reg_kernel<T, NY, false>), dim3(L.K),         \
+                               dim3(threads), shm, st, L, V, S, lu, sweeps, rf, active);      \
+        done = true;                                                                          \
+    }
+        TBL_COARSE_NY(X)
+#undef X
+        return TBL_OK;
+    }
     if (coarse_lds_bytes<T>(L.nx, L.ny, false) > 160 * 1024)
-        return smooth_sweeps<T>(L, V, S, lu, sweeps, rf, active, G, st);
+        return smooth_sweeps<T>(L, V, S, lu, sweeps, rf, active, G > kStripShape ? G - kStripShape : G,
+                                nullptr, st);
+    if (G > kStripShape) G -= kStripShape;
     if (G < 0) G = -G;
     const int R = line_rows_for(G, L.ny);
     if (R < 0) return TBL_ERR_GRID;
     bool done = false;
 #define X(GG, RR)                                                                 \
     if (!done && G == GG && R == RR) {                                            \
-        launch_coarsest<T, GG, RR>(L, V, S, lu, sweeps, rf, active, st, 1024);    \
+        launch_coarsest<T, GG, RR>(L, V, S, lu, sweeps, rf, active, st, 512);     \
         done = true;                                                              \
     }
     TBL_LINE_SHAPES(X)
@@ -785,9 +1145,12 @@ template <typename T, typename TO>
 void defect(const LevelArgs<T> &L, const cx<T> *V, const cx<T> *S, cx<TO> *out, T si,
             const int *active, double *resmax, hipStream_t st)
 {
-    const int ib = (L.nx + 3) / 4;
-    hipLaunchKernelGGL((defect_kernel<T, TO>), dim3(ib * L.jblocks, L.K), dim3(256), 0, st, L, V, S,
-                       out, si, active, resmax);
+    const int strip = defect_strip(L.nx);
+    const int nstrips = (L.nx - 2 + strip - 1) / strip;
+    const int jchunks = (L.ny - 1 + kDefRows - 1) / kDefRows;
+    const int blocks = (nstrips * jchunks + 3) / 4;
+    hipLaunchKernelGGL((defect_march_kernel<T, TO>), dim3(blocks, L.K), dim3(256), 0, st, L, V, S,
+                       out, si, active, resmax, strip, nstrips, jchunks);
 }
 
 template <typename T>
@@ -863,9 +1226,11 @@ void convert(size_t n, const cx<TA> *a, cx<TB> *b, int zero, hipStream_t st)
 
 // explicit instantiations
 template int smooth_sweeps<double>(const LevelArgs<double> &, cx<double> *, const cx<double> *,
-                                   const double *, int, double, const int *, int, hipStream_t);
+                                   const double *, int, double, const int *, int, cx<double> *,
+                                   hipStream_t);
 template int smooth_sweeps<float>(const LevelArgs<float> &, cx<float> *, const cx<float> *,
-                                  const float *, int, float, const int *, int, hipStream_t);
+                                  const float *, int, float, const int *, int, cx<float> *,
+                                  hipStream_t);
 template int smooth_coarsest<double>(const LevelArgs<double> &, cx<double> *, const cx<double> *,
                                      const double *, int, double, const int *, int, hipStream_t);
 template void defect<double, double>(const LevelArgs<double> &, const cx<double> *,

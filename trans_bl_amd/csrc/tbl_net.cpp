@@ -284,7 +284,7 @@ extern "C" int tbl_net_create(int H, int W, int max_batch, unsigned seed, tbl_ne
         init_params(n, seed, h);
         if (hipMemcpy(n->w, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice) != hipSuccess)
             rc = TBL_ERR_HIP;
-        if (rc == TBL_OK && hipStreamCreateWithFlags(&n->own, hipStreamNonBlocking) != hipSuccess)
+        if (rc == TBL_OK && hipStreamCreateWithFlags(&n->own, hipStreamDefault) != hipSuccess)
             rc = TBL_ERR_HIP;
         n->stream = n->own;
         if (rc == TBL_OK) {

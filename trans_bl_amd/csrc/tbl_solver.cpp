@@ -62,7 +62,6 @@ LevelArgs<T> level_args(const tbl_ctx &c, int l, const T *qdy2)
     A.ny = lv.ny;
     A.ld = lv.ld;
     A.K = c.K;
-    A.jblocks = (lv.ld + 63) / 64;
     A.plane = lv.plane;
     for (int i = 0; i < 7; ++i) A.sp[i] = static_cast<T>(c.c.sp_poi[l][i]);
     for (int i = 0; i < 8; ++i) A.cp[i] = static_cast<T>(c.c.cp_poi[l][i]);
@@ -105,6 +104,7 @@ static int create_impl(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx *c)
     c->qdy2_d.assign(L, nullptr); c->qdy2_f.assign(L, nullptr);
     c->lu_d.assign(L, nullptr); c->lu_f.assign(L, nullptr);
     c->G64.assign(L, 0); c->G32.assign(L, 0);
+    c->H64.assign(L, nullptr); c->H32.assign(L, nullptr);
     for (int l = 0; l < L; ++l) {
         const Level &lv = c->lv[l];
         const size_t n = lv.plane * K;
@@ -127,14 +127,21 @@ static int create_impl(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx *c)
         TBL_HIP(hipMemcpy(c->qdy2_d[l], q.data(), sizeof(double) * lv.ld, hipMemcpyHostToDevice));
         TBL_HIP(hipMemcpy(c->qdy2_f[l], qf.data(), sizeof(float) * lv.ld, hipMemcpyHostToDevice));
         // line LU factors per (mode, centre/border)
-        const size_t per = static_cast<size_t>(5) * lv.ld;
+        const size_t per = static_cast<size_t>(kLuFields) * lv.ld;
         std::vector<double> lu(per * 2 * K, 0.0);
         std::vector<float> luf(lu.size(), 0.0f);
         for (int k = 0; k < K; ++k)
             for (int t = 0; t < 2; ++t) {
                 double *b = &lu[(static_cast<size_t>(k) * 2 + t) * per];
-                line_lu(c->g, c->c, c->m, l + 1, k, t == 1, b, b + lv.ld, b + 2 * lv.ld,
-                        b + 3 * lv.ld, b + 4 * lv.ld);
+                const int d = lv.ld;
+                line_lu(c->g, c->c, c->m, l + 1, k, t == 1, b, b + d, b + 2 * d, b + 3 * d,
+                        b + 4 * d);
+                for (int j = 0; j < lv.ny; ++j) {
+                    const double p = b[4 * d + j];
+                    b[5 * d + j] = -b[2 * d + j] / p;
+                    b[6 * d + j] = -b[3 * d + j] / p;
+                    b[7 * d + j] = 1.0 / p;
+                }
             }
         for (size_t e = 0; e < lu.size(); ++e) luf[e] = static_cast<float>(lu[e]);
         TRY(dalloc(&c->lu_d[l], lu.size()));
@@ -143,6 +150,13 @@ static int create_impl(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx *c)
         TBL_HIP(hipMemcpy(c->lu_f[l], luf.data(), sizeof(float) * luf.size(), hipMemcpyHostToDevice));
         c->G64[l] = default_G(lv.ny);
         c->G32[l] = default_G(lv.ny);
+        // snapshot buffers for the fused-sweep strip smoother (not on the coarsest level)
+        if (l < L - 1) {
+            const size_t h64 = strip_halo_elems(strip_geom(lv.nx, lv.ny, K, sizeof(double)), lv.ld, K);
+            const size_t h32 = strip_halo_elems(strip_geom(lv.nx, lv.ny, K, sizeof(float)), lv.ld, K);
+            if (h64) TRY(dalloc(&c->H64[l], h64));
+            if (h32) TRY(dalloc(&c->H32[l], h32));
+        }
     }
     c->la_d.resize(L);
     c->la_f.resize(L);
@@ -196,7 +210,7 @@ static int create_impl(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx *c)
     A.qdy2 = c->qdy2_d[0];
     A.thomas = c->d_thomas;
     A.nmax = nmax;
-    TBL_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+    TBL_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamDefault));
     c->stream = c->own_stream;
     TBL_HIP(hipEventCreate(&c->ev0));
     TBL_HIP(hipEventCreate(&c->ev1));
@@ -232,6 +246,8 @@ extern "C" int tbl_destroy(tbl_ctx *c)
     for (auto *p : c->qdy2_f) fr(p);
     for (auto *p : c->lu_d) fr(p);
     for (auto *p : c->lu_f) fr(p);
+    for (auto *p : c->H64) fr(p);
+    for (auto *p : c->H32) fr(p);
     fr(c->d_active); fr(c->d_cycles); fr(c->d_nactive); fr(c->d_cycle);
     fr(c->d_resmax); fr(c->d_lastres); fr(c->d_qdy); fr(c->d_thomas);
     fr(c->d_u0); fr(c->d_u1); fr(c->d_u2);
@@ -332,7 +348,7 @@ int fas_cycle(tbl_ctx *c, const Sched &s, double tol)
     hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(256), 0, st, c->d_cycle, c->d_resmax, K, c->d_nactive);
     for (int l = 0; l < L - 1; ++l) {
         TRY(smooth_sweeps<double>(c->la_d[l], c->V[l], c->S[l], c->lu_d[l], s.nu1, rf_of(s.nu1),
-                                  c->d_active, c->G64[l], st));
+                                  c->d_active, c->G64[l], c->H64[l], st));
         defect<double, double>(c->la_d[l], c->V[l], c->S[l], c->R[l], -1.0, c->d_active,
                                l == 0 ? c->d_resmax : nullptr, st);
         if (l == 0)
@@ -350,7 +366,7 @@ int fas_cycle(tbl_ctx *c, const Sched &s, double tol)
         prolong_level<double>(c->lv[l - 1], c->lv[l], K, c->V[l - 1], c->V[l], c->W[l], 1,
                               c->d_active, st);
         TRY(smooth_sweeps<double>(c->la_d[l - 1], c->V[l - 1], c->S[l - 1], c->lu_d[l - 1], s.nu2,
-                                  rf_of(s.nu2), c->d_active, c->G64[l - 1], st));
+                                  rf_of(s.nu2), c->d_active, c->G64[l - 1], c->H64[l - 1], st));
     }
     return TBL_OK;
 }
@@ -368,7 +384,8 @@ int mixed_cycle(tbl_ctx *c, const Sched &s, double tol)
     convert<float, float>(c->lv[0].plane * K, c->E32[0], c->E32[0], 1, st);
     for (int l = 0; l < L - 1; ++l) {
         TRY(smooth_sweeps<float>(c->la_f[l], c->E32[l], c->S32[l], c->lu_f[l], s.nu1,
-                                 static_cast<float>(rf_of(s.nu1)), c->d_active, c->G32[l], st));
+                                 static_cast<float>(rf_of(s.nu1)), c->d_active, c->G32[l],
+                                 c->H32[l], st));
         defect<float, float>(c->la_f[l], c->E32[l], c->S32[l], c->R32[l], -1.0f, c->d_active,
                              nullptr, st);
         restrict_level<float>(c->lv[l], c->lv[l + 1], K, c->E32[l], c->R32[l], c->E32[l + 1],
@@ -386,7 +403,7 @@ int mixed_cycle(tbl_ctx *c, const Sched &s, double tol)
                              c->d_active, st);
         TRY(smooth_sweeps<float>(c->la_f[l - 1], c->E32[l - 1], c->S32[l - 1], c->lu_f[l - 1],
                                  s.nu2, static_cast<float>(rf_of(s.nu2)), c->d_active,
-                                 c->G32[l - 1], st));
+                                 c->G32[l - 1], c->H32[l - 1], st));
     }
     update(c->lv[0], K, c->V[0], c->E32[0], c->d_active, st);
     return TBL_OK;
@@ -579,10 +596,17 @@ extern "C" int tbl_residual_dev(tbl_ctx *c, const double *d_wx, const double *d_
 // smoother, 1: FP32 smoother.
 extern "C" int tbl_set_line_shape(tbl_ctx *c, int prec, int level, int lanes)
 {
-    const int a = lanes < 0 ? -lanes : lanes;
+    const bool strip = lanes > kStripShape;
+    const int a = strip ? lanes - kStripShape : lanes < 0 ? -lanes : lanes;
     if (!c || level < 1 || level > c->L || (a != 16 && a != 32 && a != 64))
         return TBL_ERR_ARG;
-    if (line_rows_for(a, c->lv[level - 1].ny) < 0) return TBL_ERR_GRID;
+    const Level &lv = c->lv[level - 1];
+    if (line_rows_for(a, lv.ny) < 0) return TBL_ERR_GRID;
+    // strip form: not on the coarsest level, and only for rows per lane that
+    // stay in registers with two lines in flight (FP64 <= 3, FP32 <= 5)
+    if (strip && (level == c->L || line_rows_for(a, lv.ny) > (prec ? 5 : 3) ||
+                  strip_geom(lv.nx, lv.ny, c->K, prec ? 4 : 8).nstrips < 1))
+        return TBL_ERR_GRID;
     (prec ? c->G32 : c->G64)[level - 1] = lanes;
     return TBL_OK;
 }
@@ -602,10 +626,10 @@ extern "C" int tbl_time_smoother(tbl_ctx *c, int prec, int level, int sweeps, do
     TBL_HIP(hipEventRecord(c->ev0, st));
     if (prec)
         TRY(smooth_sweeps<float>(c->la_f[l], c->E32[l], c->S32[l], c->lu_f[l], sweeps, 1.2f,
-                                 nullptr, c->G32[l], st));
+                                 nullptr, c->G32[l], c->H32[l], st));
     else
         TRY(smooth_sweeps<double>(c->la_d[l], c->V[l], c->S[l], c->lu_d[l], sweeps, 1.2,
-                                  nullptr, c->G64[l], st));
+                                  nullptr, c->G64[l], c->H64[l], st));
     TBL_HIP(hipGetLastError());
     TBL_HIP(hipEventRecord(c->ev1, st));
     TBL_HIP(hipEventSynchronize(c->ev1));

@@ -124,6 +124,7 @@ def main():
         for i in range(ntr // a.batch):
             idx = order[i * a.batch:(i + 1) * a.batch]
             xb, tb_ = Xt[idx].contiguous(), Tt[idx].contiguous()
+            torch.cuda.synchronize()
             lr = a.lr * 0.5 * (1 + math.cos(math.pi * step / steps))
             losses.append(net.train_step(xb.data_ptr(), tb_.data_ptr(), a.batch, lr))
             step += 1
@@ -135,6 +136,7 @@ def main():
         vl = []
         for i in range(0, nval - a.batch + 1, a.batch):
             xb, tb_ = Xv[i:i + a.batch].contiguous(), Tv[i:i + a.batch].contiguous()
+            torch.cuda.synchronize()
             vl.append(net.gradients(xb.data_ptr(), tb_.data_ptr(), a.batch)[0])
         stop = a.minutes and time.time() - t0 > 60 * a.minutes
         print(json.dumps({"phase": "train", "epoch": ep, "loss": float(np.mean(losses[-50:])),
