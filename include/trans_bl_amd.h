@@ -74,8 +74,12 @@ typedef struct tbl_solve_opts {
     int use_graph;    /* 1: replay the V-cycle as a captured hipGraph */
     int guess;        /* 0: start from uy (the reference's warm start,
                             poisson_par.f:157-161); 1: start from zero;
-                         2: learned initial guess (tbl_guess_*), falling back to
-                            uy when its residual is not smaller */
+                         2: learned initial guess (tbl_guess_*) on the interior,
+                            uy's boundary values kept; falls back to uy when
+                            its residual on the network's multigrid level
+                            (against the restricted source, after two line
+                            sweeps) exceeds 10x uy's or is not finite;
+                         3: learned initial guess for every mode (no test) */
     int reserved[5];
 } tbl_solve_opts;
 
@@ -87,7 +91,8 @@ typedef struct tbl_solve_stats {
     int nconverged;       /* modes whose last checked residual < erromax */
     int fallback;         /* modes where the learned guess was rejected */
     double residual_max;  /* largest last-checked max |S - L v| */
-    double guess_residual_max; /* max residual of the initial guess (guess=2) */
+    double guess_residual_max; /* guess=2: max residual of the learned prediction
+                                  on the network's level */
     double device_ms;     /* GPU time of the solve (hipEvents) */
 } tbl_solve_stats;
 

@@ -32,6 +32,9 @@ SOURCES = [
     "tbl_host.cpp",
     "tbl_solver.cpp",
     "tbl_mg_kernels.hip",
+    "tbl_smooth_phase.hip",
+    "tbl_smooth_strip.hip",
+    "tbl_smooth_coarse.hip",
     "tbl_deriv_kernels.hip",
     "tbl_unet.hip",
     "tbl_net.cpp",

@@ -5,9 +5,16 @@
 // the mode's max |g|), kz and the normalised height y/ytop; its output is v
 // at those points scaled by sv = max|g| / (kz^2 + kappa^2).  The prediction
 // is prolongated to the fine grid with the multigrid's own bilinear
-// interpolation (poisson_par.f:877-920).  A mode whose guess has a larger
-// max residual than the zero start falls back to zero; multigrid then runs
-// to the same tolerance either way.
+// interpolation (poisson_par.f:877-920) onto the interior of the fine grid;
+// the boundary columns and the wall row keep the warm start's values.  The
+// candidates are compared on the network's level: after two line sweeps of
+// each (for the fully weighted restriction of the source), the max residual
+// of the prediction against that of the injected warm start.  (The residual
+// of the raw prediction, on any level, is dominated by its pixel-scale
+// noise, which the operator amplifies and the first sweeps remove.)  A mode
+// whose prediction's residual is more than kGuessAccept times the warm
+// start's keeps the warm start; multigrid then runs to the same tolerance
+// either way.
 #include <algorithm>
 #include <cmath>
 
@@ -112,20 +119,31 @@ __global__ void place_kernel(Level Cl, int K, int H, int W, const float *pred, c
     V[idx] = v;
 }
 
-// modes whose guess residual is not below the zero start's go back to zero
+// line sweeps applied on the network's level before the comparison
+constexpr int kGuessSweeps = 2;
+// a prediction is kept unless its residual exceeds the warm start's by this
+// factor (or is not finite): even after a few sweeps the residual weights
+// the prediction's rough, quickly smoothed error far above the smooth error
+// that decides how many V-cycles remain
+constexpr double kGuessAccept = 10.0;
+
+// accepted modes: the prolongated prediction on the interior, the warm
+// start W on the boundary columns and the wall row; other modes: W
 __global__ void fallback_kernel(Level F, int K, const double *res0, const double *resg,
-                                cx<double> *V, int *count)
+                                const cx<double> *W, cx<double> *V, int accept_all, int *count)
 {
     const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
     if (idx >= F.plane * K) return;
     const int k = static_cast<int>(idx / F.plane);
-    if (!(resg[k] < res0[k])) {
-        V[idx] = {0, 0};
-        if (idx == static_cast<size_t>(k) * F.plane) atomicAdd(count, 1);
-    }
+    const size_t rem = idx - static_cast<size_t>(k) * F.plane;
+    const int i = static_cast<int>(rem / F.ld), j = static_cast<int>(rem % F.ld);
+    const bool accept = accept_all || resg[k] < kGuessAccept * res0[k];
+    if (!accept || i == 0 || i == F.nx - 1 || j == 0) V[idx] = W[idx];
+    if (!accept && rem == 0) atomicAdd(count, 1);
 }
 
 unsigned nblk(size_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
 
 }  // namespace
 
@@ -187,7 +205,8 @@ static void build_features(tbl_ctx *c, hipStream_t st)
 
 // Called by the solver after gather+source (R[0] holds g, S[0] the source):
 // writes the guess into V[0] and the fallback count into *fallback.
-int guess_apply(tbl_ctx *c, hipStream_t st, int *fallback, double *guess_res_max)
+int guess_apply(tbl_ctx *c, hipStream_t st, bool accept_all, int *fallback,
+                double *guess_res_max)
 {
     Guess *g = c->guess;
     if (!g || !g->net) return TBL_ERR_STATE;
@@ -199,21 +218,41 @@ int guess_apply(tbl_ctx *c, hipStream_t st, int *fallback, double *guess_res_max
     if (rc != TBL_OK) return rc;
     const int L = g->lvl - 1;
     const Level &Cl = c->lv[L];
+    // warm start (V[0] after gather) kept in W[0]; injected down to level L
+    // (W[l]) with the source fully weighted alongside (S[l]); R[l] takes
+    // the injection's unused copy
+    convert<double, double>(F.plane * K, c->V[0], c->W[0], 0, st);
+    for (int l = 0; l < L; ++l)
+        restrict_level<double>(c->lv[l], c->lv[l + 1], K, c->W[l], l == 0 ? c->S[0] : c->S[l],
+                               c->W[l + 1], c->R[l + 1], c->S[l + 1], 1, nullptr, st);
     hipLaunchKernelGGL(place_kernel, dim3(nblk(Cl.plane * K)), dim3(256), 0, st, Cl, K, g->H, g->W,
                        g->pred, g->sf, c->g.beta, g->kappa2, c->V[L]);
+    // residuals on level L after kGuessSweeps line sweeps of each candidate
+    // (the prediction's pixel-scale noise is what the operator amplifies most;
+    // the sweeps remove it, and the smoothed prediction is the one used)
+    if (L > 0) {
+        for (cx<double> *cand : {c->V[L], c->W[L]}) {
+            rc = smooth_sweeps<double>(c->la_d[L], cand, c->S[L], c->lu_d[L], kGuessSweeps, 1.2,
+                                       nullptr, c->G64[L], c->H64[L], st);
+            if (rc != TBL_OK) return rc;
+        }
+    }
+    fill(g->resg, K, 0.0, st);
+    fill(g->res0, K, 0.0, st);
+    if (L > 0) {
+        defect<double, double>(c->la_d[L], c->V[L], c->S[L], c->R[L], -1.0, nullptr, g->resg, st);
+        defect<double, double>(c->la_d[L], c->W[L], c->S[L], c->R[L], -1.0, nullptr, g->res0, st);
+    } else {
+        defect<double, double>(c->la_d[0], c->V[0], c->S[0], c->R[0], -1.0, nullptr, g->resg, st);
+        defect<double, double>(c->la_d[0], c->W[0], c->S[0], c->R[0], -1.0, nullptr, g->res0, st);
+    }
     for (int l = L; l >= 1; --l) {
         fill(reinterpret_cast<double *>(c->V[l - 1]), c->lv[l - 1].plane * K * 2, 0.0, st);
         prolong_level<double>(c->lv[l - 1], c->lv[l], K, c->V[l - 1], c->V[l], nullptr, 0, nullptr, st);
     }
-    // residual of the guess and of the zero start
-    fill(g->resg, K, 0.0, st);
-    fill(g->res0, K, 0.0, st);
-    defect<double, double>(c->la_d[0], c->V[0], c->S[0], c->R[0], -1.0, nullptr, g->resg, st);
-    fill(reinterpret_cast<double *>(c->W[0]), F.plane * K * 2, 0.0, st);
-    defect<double, double>(c->la_d[0], c->W[0], c->S[0], c->R[0], -1.0, nullptr, g->res0, st);
     fill_int(g->fallback, 1, 0, st);
     hipLaunchKernelGGL(fallback_kernel, dim3(nblk(F.plane * K)), dim3(256), 0, st, F, K, g->res0,
-                       g->resg, c->V[0], g->fallback);
+                       g->resg, c->W[0], c->V[0], accept_all ? 1 : 0, g->fallback);
     TBL_HIP(hipGetLastError());
     std::vector<double> rg(K);
     TBL_HIP(hipMemcpyAsync(fallback, g->fallback, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -50,10 +50,19 @@ inline size_t strip_halo_elems(const StripGeom &g, int ld, int K)
     return g.nstrips > 1 ? static_cast<size_t>(K) * (g.nstrips - 1) * 2 * kStripHalo * ld : 0;
 }
 int set_kernel_attributes();
+int phase_kernel_attributes();
+int strip_kernel_attributes();
+int coarse_kernel_attributes();
 
 template <typename T>
 int smooth_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
                   const int *active, int G, cx<T> *halo, hipStream_t st);
+template <typename T>
+int phase_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
+                 const int *active, int G, hipStream_t st);
+template <typename T>
+int strip_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
+                 const int *active, int G, cx<T> *halo, hipStream_t st);
 template <typename T>
 int smooth_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps,
                     T rf, const int *active, int G, hipStream_t st);

@@ -1,0 +1,397 @@
+// Coarsest-level smoother: every sweep of the level in one launch, one
+// workgroup per mode with the level resident in LDS.
+#include "tbl_relax.h"
+
+namespace tbl {
+
+// Coarsest level: one workgroup per mode keeps the whole level in LDS and
+// runs every sweep (20*itera, poisson_par.f:45) without leaving the CU.  LDS
+// holds v with a compact odd column stride and the mode's line-LU factors;
+// the source is read from global memory.  (Scan form, for levels whose lines
+// are too long for the one-thread-per-line kernel below.)
+__host__ __device__ inline int coarse_stride(int ny) { return ny | 1; }
+
+template <typename T>
+__host__ __device__ inline size_t coarse_lds_bytes(int nx, int ny, bool with_s)
+{
+    const size_t col = static_cast<size_t>(coarse_stride(ny)) * nx * sizeof(cx<T>);
+    return col * (with_s ? 2 : 1) + static_cast<size_t>(2 * kLuFields) * ny * sizeof(T);
+}
+
+template <typename T, int G, int R>
+__global__ void __launch_bounds__(512) smooth_coarsest_kernel(LevelArgs<T> L, cx<T> *V,
+                                                               const cx<T> *S, const T *lu,
+                                                               int sweeps, T rf,
+                                                               const int *active)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int k = blockIdx.x;
+    if (active && !active[k]) return;
+    const int nx = L.nx, ny = L.ny, ld = L.ld, ls = coarse_stride(ny);
+    cx<T> *sv = reinterpret_cast<cx<T> *>(smem_raw);
+    T *slu = reinterpret_cast<T *>(sv + static_cast<size_t>(ls) * nx);
+    cx<T> *gv = V + static_cast<size_t>(k) * L.plane;
+    const cx<T> *gs = S + static_cast<size_t>(k) * L.plane;
+    const T *glu = lu + (static_cast<size_t>(k) * 2) * kLuFields * ld;
+    for (int e = threadIdx.x; e < nx * ny; e += blockDim.x) {
+        const int i = e / ny, j = e % ny;
+        sv[i * ls + j] = gv[static_cast<size_t>(i) * ld + j];
+    }
+    for (int e = threadIdx.x; e < 2 * kLuFields * ny; e += blockDim.x)
+        slu[e] = glu[static_cast<size_t>(e / ny) * ld + e % ny];
+    __syncthreads();
+    // the line solve indexes the factors with stride L.ld: give it ny here
+    LevelArgs<T> Ls = L;
+    Ls.ld = ny;
+    const int groups = blockDim.x / G;
+    const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+    for (int it = 0; it < sweeps; ++it) {
+        for (int phase = 0; phase < 4; ++phase) {
+            const int nl = phase_lines(phase, nx);
+            for (int c = grp; c < nl; c += groups) {
+                const int i = phase_column(phase, c, nx);
+                const int side = (phase == 0) ? (c == 0 ? -1 : 1) : 0;
+                cx<T> *col = sv + static_cast<size_t>(i) * ls;
+                const cx<T> *scol = gs + static_cast<size_t>(i) * ld;
+                const LineLU<T, R> f =
+                    load_line_lu<T, R>(slu + (side ? kLuFields * ny : 0), ny, lane * R, ny);
+                relax_core<T, G, R>(
+                    Ls, f, rf, lane,
+                    [&](int j) { return xoff<T>(col, ls, j, side, side == 0 && j >= ny - 2); },
+                    [&](int j) { return col[j]; }, [&](int j, int) { return scol[j]; },
+                    [&](int j, cx<T> v) { col[j] = v; });
+            }
+            __syncthreads();
+        }
+    }
+    for (int e = threadIdx.x; e < nx * ny; e += blockDim.x) {
+        const int i = e / ny, j = e % ny;
+        gv[static_cast<size_t>(i) * ld + j] = sv[i * ls + j];
+    }
+}
+
+// Coarsest level, one thread per line: each line of a phase is relaxed by one
+// thread with the sequential banded substitution of banbk5_poi
+// (penta_par.f:367-404).  v, the factors and (when it fits) the source stay
+// in LDS; the forward-substitution values stay in registers.  The row count
+// NY is a compile-time constant (one instantiation per supported coarsest
+// ny) and so is the line's border side, so a line solve is straight-line
+// code and its LDS loads are scheduled ahead of the serial chain.  The back
+// substitution multiplies by the precomputed -u1/p, -u2/p, 1/p instead of
+// dividing by the pivot (last-bit rounding differs from the division).
+template <typename T>
+__host__ __device__ inline size_t coarse_reg_lds_bytes(int nx, int ny, bool with_s)
+{
+    return static_cast<size_t>(coarse_stride(ny)) * nx * sizeof(cx<T>) * (with_s ? 2 : 1) +
+           static_cast<size_t>(2 * kLuFields) * ny * sizeof(T);
+}
+
+template <typename T, int NY, int SIDE>
+__device__ __forceinline__ void coarse_line(const LevelArgs<T> &L, cx<T> *col, const cx<T> *scol,
+                                            const T *f, T rf)
+{
+    constexpr int LS = NY | 1;
+    const T qx = L.qx;
+    auto X = [&](int j) { return xoff<T>(col, LS, j, SIDE, SIDE == 0 && j >= NY - 2); };
+    const T *l1 = f, *l2 = f + NY, *m1 = f + 5 * NY, *m2 = f + 6 * NY, *rp = f + 7 * NY;
+    cx<T> Y[NY];
+    cx<T> xm = {T(0), T(0)}, x0 = X(0), xp = X(1);
+    cx<T> y1 = {T(0), T(0)}, y2 = {T(0), T(0)};
+#pragma unroll
+    for (int j = 0; j < NY; ++j) {
+        cx<T> rhs;
+        if (j == 0) rhs = col[0];
+        else if (j == 1) rhs = scol[1] - qx * (L.sp[2] * xp + L.sp[1] * x0);
+        else if (j <= NY - 3) rhs = scol[j] - qx * (L.cp[0] * xm + L.cp[1] * x0 + L.cp[2] * xp);
+        else rhs = scol[j] - qx * x0;
+        const cx<T> yy = (rhs - l2[j] * y2) - l1[j] * y1;
+        Y[j] = yy;
+        y2 = y1; y1 = yy;
+        xm = x0; x0 = xp;
+        if (j + 2 < NY) xp = X(j + 2);
+    }
+    // back substitution into Y, then the relaxation stores: no LDS store on
+    // the serial chain, so the factor loads need not wait for one
+    cx<T> a = {T(0), T(0)}, b = {T(0), T(0)};
+#pragma unroll
+    for (int j = NY - 1; j >= 0; --j) {
+        const cx<T> xx = (rp[j] * Y[j] + m2[j] * b) + m1[j] * a;
+        Y[j] = xx;
+        b = a; a = xx;
+    }
+#pragma unroll
+    for (int j = 1; j < NY; ++j) col[j] = col[j] + rf * (Y[j] - col[j]);
+}
+
+// SL: the source is staged in LDS (else read from global memory); a
+// compile-time choice, so every access has a known address space.
+template <typename T, int NY, bool SL>
+__global__ void __launch_bounds__(256) smooth_coarsest_reg_kernel(LevelArgs<T> L, cx<T> *V,
+                                                                   const cx<T> *S, const T *lu,
+                                                                   int sweeps, T rf,
+                                                                   const int *active)
+{
+    constexpr int LS = NY | 1;
+    extern __shared__ unsigned char smem_raw[];
+    const int k = blockIdx.x;
+    if (active && !active[k]) return;
+    const int nx = L.nx, ld = L.ld;
+    cx<T> *sv = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *ss = sv + static_cast<size_t>(LS) * nx;
+    cx<T> *gv = V + static_cast<size_t>(k) * L.plane;
+    const cx<T> *gs = S + static_cast<size_t>(k) * L.plane;
+    T *slu = reinterpret_cast<T *>(SL ? ss + static_cast<size_t>(LS) * nx : ss);
+    const T *glu = lu + (static_cast<size_t>(k) * 2) * kLuFields * ld;
+    for (int e = threadIdx.x; e < nx * NY; e += blockDim.x) {
+        const int i = e / NY, j = e % NY;
+        sv[i * LS + j] = gv[static_cast<size_t>(i) * ld + j];
+        if (SL) ss[i * LS + j] = gs[static_cast<size_t>(i) * ld + j];
+    }
+    for (int e = threadIdx.x; e < 2 * kLuFields * NY; e += blockDim.x)
+        slu[e] = glu[static_cast<size_t>(e / NY) * ld + e % NY];
+    __syncthreads();
+    const T *lu_centre = slu;
+    const T *lu_border = slu + kLuFields * NY;
+    const int c = threadIdx.x;
+    for (int it = 0; it < sweeps; ++it) {
+        for (int phase = 0; phase < 4; ++phase) {
+            if (c < phase_lines(phase, nx)) {
+                const int i = phase_column(phase, c, nx);
+                cx<T> *col = sv + static_cast<size_t>(i) * LS;
+                const cx<T> *scol = SL ? ss + static_cast<size_t>(i) * LS
+                                       : gs + static_cast<size_t>(i) * ld;
+                if (phase != 0) coarse_line<T, NY, 0>(L, col, scol, lu_centre, rf);
+                else if (c == 0) coarse_line<T, NY, -1>(L, col, scol, lu_border, rf);
+                else coarse_line<T, NY, 1>(L, col, scol, lu_border, rf);
+            }
+            __syncthreads();
+        }
+    }
+    for (int e = threadIdx.x; e < nx * NY; e += blockDim.x) {
+        const int i = e / NY, j = e % NY;
+        gv[static_cast<size_t>(i) * ld + j] = sv[i * LS + j];
+    }
+}
+
+// Two-stage form of the same solve, used when its LDS fits: per phase, all
+// four wavefronts first form the right-hand sides of every (line, row) of
+// the phase into LDS (the bulk of the arithmetic, spread over the CU's
+// SIMDs); then two threads per line -- the real and the imaginary part, the
+// factors being real -- run the two substitutions and the relaxation.  The
+// substitutions are contracted to fused multiply-adds (one dependent
+// operation per row and direction); the right-hand sides keep the
+// reference's operation order.  v, the source, the right-hand sides and
+// the factors are all in LDS.
+template <typename T>
+__host__ __device__ inline size_t coarse_2s_lds_bytes(int nx, int ny)
+{
+    const size_t ls = static_cast<size_t>(coarse_stride(ny));
+    return (2 * ls * nx + ls * phase_lines(1, nx)) * sizeof(cx<T>) +
+           static_cast<size_t>(2 * kLuFields) * ny * sizeof(T);
+}
+
+// one part (real or imaginary; element stride 2) of one line
+template <typename T, int NY>
+__device__ __forceinline__ void coarse_chain(const T *rb, T *col, const T *f, T rf)
+{
+    const T *l1 = f, *l2 = f + NY, *m1 = f + 5 * NY, *m2 = f + 6 * NY, *rp = f + 7 * NY;
+    T Y[NY];
+    T y1 = T(0), y2 = T(0);
+    // y_j = (r_j - l2_j y_{j-2}) - l1_j y_{j-1}
+#pragma unroll
+    for (int j = 0; j < NY; ++j) {
+        const T yy = fma(-l1[j], y1, fma(-l2[j], y2, rb[2 * j]));
+        Y[j] = yy;
+        y2 = y1; y1 = yy;
+    }
+    // x_j = (y_j / p_j - (u2_j / p_j) x_{j+2}) - (u1_j / p_j) x_{j+1}
+    T a = T(0), b = T(0);
+#pragma unroll
+    for (int j = NY - 1; j >= 0; --j) {
+        const T xx = fma(m1[j], a, fma(m2[j], b, rp[j] * Y[j]));
+        Y[j] = xx;
+        b = a; a = xx;
+    }
+#pragma unroll
+    for (int j = 1; j < NY; ++j) col[2 * j] = fma(rf, Y[j] - col[2 * j], col[2 * j]);
+}
+
+template <typename T, int NY>
+__global__ void __launch_bounds__(256) smooth_coarsest_2s_kernel(LevelArgs<T> L, cx<T> *V,
+                                                                  const cx<T> *S, const T *lu,
+                                                                  int sweeps, T rf,
+                                                                  const int *active)
+{
+    constexpr int LS = NY | 1;
+    extern __shared__ unsigned char smem_raw[];
+    const int k = blockIdx.x;
+    if (active && !active[k]) return;
+    const int nx = L.nx, ld = L.ld;
+    cx<T> *sv = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *ss = sv + static_cast<size_t>(LS) * nx;
+    cx<T> *rbuf = ss + static_cast<size_t>(LS) * nx;
+    T *slu = reinterpret_cast<T *>(rbuf + static_cast<size_t>(LS) * phase_lines(1, nx));
+    cx<T> *gv = V + static_cast<size_t>(k) * L.plane;
+    const cx<T> *gs = S + static_cast<size_t>(k) * L.plane;
+    const T *glu = lu + (static_cast<size_t>(k) * 2) * kLuFields * ld;
+    for (int e = threadIdx.x; e < nx * NY; e += blockDim.x) {
+        const int i = e / NY, j = e % NY;
+        sv[i * LS + j] = gv[static_cast<size_t>(i) * ld + j];
+        ss[i * LS + j] = gs[static_cast<size_t>(i) * ld + j];
+    }
+    for (int e = threadIdx.x; e < 2 * kLuFields * NY; e += blockDim.x)
+        slu[e] = glu[static_cast<size_t>(e / NY) * ld + e % NY];
+    __syncthreads();
+    const T qx = L.qx;
+    for (int it = 0; it < sweeps; ++it) {
+        for (int phase = 0; phase < 4; ++phase) {
+            const int nl = phase_lines(phase, nx);
+            // stage 1: right-hand sides (rhs_border1/2, rhs_center)
+            for (int e = threadIdx.x; e < nl * NY; e += blockDim.x) {
+                const int c = e / NY, j = e % NY;
+                const int i = phase_column(phase, c, nx);
+                const int side = (phase == 0) ? (c == 0 ? -1 : 1) : 0;
+                const cx<T> *col = sv + static_cast<size_t>(i) * LS;
+                auto X = [&](int jj) {
+                    return xoff<T>(col, LS, jj, side, side == 0 && jj >= NY - 2);
+                };
+                cx<T> rhs;
+                if (j == 0) {
+                    rhs = col[0];
+                } else {
+                    const cx<T> sj = ss[static_cast<size_t>(i) * LS + j];
+                    if (j == 1) rhs = sj - qx * (L.sp[2] * X(2) + L.sp[1] * X(1));
+                    else if (j <= NY - 3)
+                        rhs = sj - qx * (L.cp[0] * X(j - 1) + L.cp[1] * X(j) + L.cp[2] * X(j + 1));
+                    else rhs = sj - qx * X(j);
+                }
+                rbuf[c * LS + j] = rhs;
+            }
+            __syncthreads();
+            // stage 2: two threads (real, imaginary part) per line
+            const int c = threadIdx.x >> 1, part = threadIdx.x & 1;
+            if (c < nl) {
+                const int i = phase_column(phase, c, nx);
+                coarse_chain<T, NY>(reinterpret_cast<const T *>(rbuf + c * LS) + part,
+                                    reinterpret_cast<T *>(sv + static_cast<size_t>(i) * LS) + part,
+                                    slu + (phase == 0 ? kLuFields * NY : 0), rf);
+            }
+            __syncthreads();
+        }
+    }
+    for (int e = threadIdx.x; e < nx * NY; e += blockDim.x) {
+        const int i = e / NY, j = e % NY;
+        gv[static_cast<size_t>(i) * ld + j] = sv[i * LS + j];
+    }
+}
+
+// coarsest row counts with a one-thread-per-line instantiation (ny of the
+// coarsest level = (ny-1)/2^(levels-1) + 1 for the usual grids)
+#define TBL_COARSE_NY(X) X(9) X(13) X(17) X(25) X(33)
+
+template <typename T, int G, int R>
+static void launch_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu,
+                            int sweeps, T rf, const int *active, hipStream_t st, int threads)
+{
+    const size_t shm = coarse_lds_bytes<T>(L.nx, L.ny, false);
+    hipLaunchKernelGGL((smooth_coarsest_kernel<T, G, R>), dim3(L.K), dim3(threads), shm, st, L, V,
+                       S, lu, sweeps, rf, active);
+}
+
+template <typename T>
+int smooth_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps,
+                    T rf, const int *active, int G, hipStream_t st)
+{
+    bool reg_ny = false;
+#define X(NY) reg_ny = reg_ny || L.ny == NY;
+    TBL_COARSE_NY(X)
+#undef X
+    if (reg_ny && coarse_2s_lds_bytes<T>(L.nx, L.ny) <= 160 * 1024 &&
+        2 * phase_lines(1, L.nx) <= 256) {
+        const size_t shm = coarse_2s_lds_bytes<T>(L.nx, L.ny);
+        bool done = false;
+#define X(NY)                                                                                 \
+    if (!done && L.ny == NY) {                                                                \
+        hipLaunchKernelGGL((smooth_coarsest_2s_kernel<T, NY>), dim3(L.K), dim3(256), shm, st, \
+                           L, V, S, lu, sweeps, rf, active);                                  \
+        done = true;                                                                          \
+    }
+        TBL_COARSE_NY(X)
+#undef X
+        return TBL_OK;
+    }
+    if (reg_ny && coarse_reg_lds_bytes<T>(L.nx, L.ny, false) <= 160 * 1024 &&
+        phase_lines(1, L.nx) <= 256) {
+        const int s_in = coarse_reg_lds_bytes<T>(L.nx, L.ny, true) <= 160 * 1024;
+        const int threads = (phase_lines(1, L.nx) + 63) / 64 * 64;
+        const size_t shm = coarse_reg_lds_bytes<T>(L.nx, L.ny, s_in);
+        bool done = false;
+#define X(NY)                                                                                 \
+    if (!done && L.ny == NY) {                                                                \
+        if (s_in)                                                                             \
+            hipLaunchKernelGGL((smooth_coarsest_reg_kernel<T, NY, true>), dim3(L.K),          \
+                               dim3(threads), shm, st, L, V, S, lu, sweeps, rf, active);      \
+        else                                                                                  \
+            hipLaunchKernelGGL((smooth_coarsest_reg_kernel<T, NY, false>), dim3(L.K),         \
+                               dim3(threads), shm, st, L, V, S, lu, sweeps, rf, active);      \
+        done = true;                                                                          \
+    }
+        TBL_COARSE_NY(X)
+#undef X
+        return TBL_OK;
+    }
+    if (coarse_lds_bytes<T>(L.nx, L.ny, false) > 160 * 1024)
+        return smooth_sweeps<T>(L, V, S, lu, sweeps, rf, active, G > kStripShape ? G - kStripShape : G,
+                                nullptr, st);
+    if (G > kStripShape) G -= kStripShape;
+    if (G < 0) G = -G;
+    const int R = line_rows_for(G, L.ny);
+    if (R < 0) return TBL_ERR_GRID;
+    bool done = false;
+#define X(GG, RR)                                                                 \
+    if (!done && G == GG && R == RR) {                                            \
+        launch_coarsest<T, GG, RR>(L, V, S, lu, sweeps, rf, active, st, 512);     \
+        done = true;                                                              \
+    }
+    TBL_LINE_SHAPES(X)
+#undef X
+    return done ? TBL_OK : TBL_ERR_GRID;
+}
+
+int coarse_kernel_attributes()
+{
+    hipError_t e = hipSuccess;
+#define X(GG, RR)                                                                             \
+    for (const void *f : {reinterpret_cast<const void *>(&smooth_coarsest_kernel<double, GG, RR>), \
+                          reinterpret_cast<const void *>(&smooth_coarsest_kernel<float, GG, RR>)}) { \
+        hipError_t r = allow_full_lds(f);                                                     \
+        if (r != hipSuccess) e = r;                                                           \
+    }
+    TBL_LINE_SHAPES(X)
+#undef X
+#define X(NY)                                                                                 \
+    for (const void *f :                                                                      \
+         {reinterpret_cast<const void *>(&smooth_coarsest_reg_kernel<double, NY, true>),       \
+          reinterpret_cast<const void *>(&smooth_coarsest_reg_kernel<double, NY, false>),      \
+          reinterpret_cast<const void *>(&smooth_coarsest_reg_kernel<float, NY, true>),        \
+          reinterpret_cast<const void *>(&smooth_coarsest_reg_kernel<float, NY, false>)}) {    \
+        hipError_t r = allow_full_lds(f);                                                     \
+        if (r != hipSuccess) e = r;                                                           \
+    }
+    TBL_COARSE_NY(X)
+#undef X
+#define X(NY)                                                                                 \
+    for (const void *f : {reinterpret_cast<const void *>(&smooth_coarsest_2s_kernel<double, NY>), \
+                          reinterpret_cast<const void *>(&smooth_coarsest_2s_kernel<float, NY>)}) { \
+        hipError_t r = allow_full_lds(f);                                                     \
+        if (r != hipSuccess) e = r;                                                           \
+    }
+    TBL_COARSE_NY(X)
+#undef X
+    return e == hipSuccess ? TBL_OK : TBL_ERR_HIP;
+}
+
+template int smooth_coarsest<double>(const LevelArgs<double> &, cx<double> *, const cx<double> *,
+                                     const double *, int, double, const int *, int, hipStream_t);
+
+}  // namespace tbl

@@ -447,6 +447,7 @@ int solve_dev(tbl_ctx *c, const double *wx, const double *dz, double *uy, int di
     const Sched s = sched_of(opts);
     const int prec = opts ? opts->precision : 0;
     if (prec < 0 || prec > 1) return TBL_ERR_ARG;
+    if (opts && (opts->guess < 0 || opts->guess > 3)) return TBL_ERR_ARG;
     const bool use_graph = opts && opts->use_graph;
     hipStream_t st = c->stream;
     const int K = c->K;
@@ -460,7 +461,8 @@ int solve_dev(tbl_ctx *c, const double *wx, const double *dz, double *uy, int di
     source(c->la_d[0], c->R[0], c->S[0], st);
     int fallback = 0;
     double guess_res = 0.0;
-    if (opts && opts->guess == 2) TRY(guess_apply(c, st, &fallback, &guess_res));
+    if (opts && (opts->guess == 2 || opts->guess == 3))
+        TRY(guess_apply(c, st, opts->guess == 3, &fallback, &guess_res));
     fill_int(c->d_active, K, 1, st);
     fill_int(c->d_cycles, K, 0, st);
     fill_int(c->d_cycle, 1, 0, st);
