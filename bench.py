@@ -33,6 +33,32 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# per-kernel HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs
+# (rocprofv3 --pmc ... -- python3 -m trans_bl_amd.prof_kernels, summarized by
+# trans_bl_amd.prof_kernels --summarize)
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_1025x193x64.json")
+ROWS = (1, 2, 3, 4, 5, 7, 9, 13, 17)
+
+
+def smoother_kernel(prec, lanes, ny):
+    """Name of the smoother kernel a line shape runs (as rocprofv3 prints it)."""
+    kind = "strip" if lanes > 1000 else "staged" if lanes < 0 else "phase"
+    g = lanes - 1000 if lanes > 1000 else abs(lanes)
+    need = -(-ny // g)
+    r = next(x for x in ROWS if x >= need)
+    return f"smooth_{kind}_kernel<{'float' if prec else 'double'}, {g}, {r}>"
+
+
+def measured_traffic(kernel):
+    try:
+        with open(TRAFFIC_JSON) as f:
+            table = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+    for name, e in table.items():
+        if kernel + "(" in name:
+            return e["hbm_bytes_per_launch"]
+    return None
 METRIC = "v-Poisson mode-solves/s to 1e-9 residual; learned+MG vs plain MG on 1 GPU"
 
 
@@ -148,6 +174,20 @@ def main():
     flag_learned = guess is not None
     wallm, stm = run(1, flag_learned, a.steps, a.warmup)
     vm = ud.clone()
+    mixed_zero = None
+    if flag_learned:
+        wallz, stz = run(1, False, a.steps, a.warmup)
+        mixed_zero = (wallz, stz)
+
+    # residual-tested form of the learned start (guess=2): fallback rate
+    fallback = None
+    if guess is not None:
+        ud.zero_()
+        _, _, stt = guess.solve(ctx, fd, ud, a.tol, test=True, precision=1, max_cycles=40,
+                                use_graph=1)
+        fallback = {"modes_falling_back": int(stt["fallback"]), "of": K,
+                    "rule": "prediction's max residual on its multigrid level after two line "
+                            "sweeps above 10x the zero start's"}
 
     def summar(stats):
         cyc = np.array([s[0] for s in stats])
@@ -167,17 +207,18 @@ def main():
     # the level (3 passes), complex FP64 = 16 B, FP32 = 8 B
     n1 = grid.nx * grid.ny * K
     roof = {}
-    for prec, esize, name in ((1, 8, "smooth_phase_kernel<float> level 1"),
-                              (0, 16, "smooth_phase_kernel<double> level 1")):
+    for prec, esize in ((1, 8), (0, 16)):
+        name = smoother_kernel(prec, ctx.line_shape(prec, 1), grid.ny)
         sweeps = 20
         ctx.time_smoother(prec, 1, 2)            # warm
         ms, launches = ctx.time_smoother(prec, 1, sweeps)
         per_launch_s = ms / 1e3 / launches
-        bytes_per_launch = 3 * esize * n1 / 4.0
+        bytes_per_launch = 3 * esize * n1 * sweeps / launches
         gbs = bytes_per_launch / per_launch_s / 1e9
         roof[prec] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                      "kernel": name, "avg_launch_us": round(per_launch_s * 1e6, 2),
+                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                      "traffic": measured_traffic(name),
+                      "kernel": name + " (finest level)", "avg_launch_us": round(per_launch_s * 1e6, 2),
                       "bytes_per_launch": int(bytes_per_launch),
                       "lanes_per_line": ctx.line_shape(prec, 1)}
 
@@ -203,6 +244,10 @@ def main():
         "plain_fp64": dict(s64, value=round(rate64, 2),
                            ms_per_step=round(wall64 / a.steps * 1e3, 3)),
         "speedup_vs_plain_fp64": round(ratem / rate64, 3),
+        "learned_guess_residual_test": fallback,
+        "mixed_zero_start": None if mixed_zero is None else dict(
+            summar(mixed_zero[1]), value=round(K * a.steps / mixed_zero[0], 2),
+            ms_per_step=round(mixed_zero[0] / a.steps * 1e3, 3)),
         "flagship_vs_fp64_rel_l2": diff,
         "line_shapes": {k: {"fp64": v.get("fp64"), "fp32": v.get("fp32")}
                         for k, v in sorted(tuned.items())},

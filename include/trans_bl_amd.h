@@ -186,7 +186,8 @@ int tbl_get_line_shape(tbl_ctx *ctx, int prec, int level);
    sweeps of one level on the context's current multigrid state, bracketed
    by hipEvents on the context's stream; *ms receives the elapsed time and
    *launches the number of smoother kernel launches it made.  prec 0 times
-   the FP64 smoother, 1 the FP32 one. */
+   the FP64 smoother, 1 the FP32 one.  On the coarsest level it times that
+   level's solve as the V-cycle runs it (all sweeps in one launch, FP64). */
 int tbl_time_smoother(tbl_ctx *ctx, int prec, int level, int sweeps, double *ms,
                       int *launches);
 
@@ -238,9 +239,9 @@ int tbl_mfma_gemm_bf16(int M, int N, int K, const void *d_a, const void *d_b, fl
                        int transpose_a, int transpose_b, int splitk);
 
 /* Attach a network to a context; it predicts v on multigrid level `level`
-   (opts.guess = 2 then starts each mode from its prolongated prediction,
-   falling back to zero when the prediction's residual is larger).
-   net == NULL detaches. */
+   (opts.guess = 2 or 3 then starts each mode from its prolongated
+   prediction; see tbl_solve_opts.guess).  The network must stay alive while
+   attached: destroy the context or detach (net == NULL) first. */
 int tbl_guess_attach(tbl_ctx *ctx, tbl_net *net, int level);
 int tbl_guess_shape(tbl_ctx *ctx, int *H, int *W);
 /* Training pairs from solved modes: f and v device arrays (user layout);

@@ -58,10 +58,14 @@ def test_wild_guess_falls_back_to_warm_start(torch_mod, precision):
     assert np.array_equal(got, ref) and np.array_equal(gc, rc)
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [1])
 def test_forced_guess_keeps_boundary_data(torch_mod, precision):
     """guess=3 uses the prediction for every mode; the warm start's boundary
-    columns and wall row are kept and the solve reaches the tolerance."""
+    columns and wall row are kept and the solve reaches the tolerance.
+    (Mixed precision only: the reference's FAS cycle adds the coarse grid's
+    boundary-column values themselves as a correction (intpol,
+    poisson_par.f:877-942), so with nonzero data on the x boundaries it
+    stalls -- the oracle does the same.)"""
     torch = torch_mod
     g = wl.Grid(129, 33, 4, 3, 0.03, 1e-3, 1.03, 0.0, 0.6)
     rng = np.random.default_rng(3)

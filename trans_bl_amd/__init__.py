@@ -189,6 +189,7 @@ class Context:
         if getattr(self, "h", None):
             lib().tbl_destroy(self.h)
             self.h = None
+        self._guess_net = None
 
     def __del__(self):
         try:

@@ -1,9 +1,10 @@
 // Learned initial guess for the v-Poisson solve.
 //
 // The U-Net (tbl_net.cpp) works on one coarse multigrid level lg: its input
-// is the source g = kb*wx - dwzdx sampled at that level's points (scaled by
-// the mode's max |g|), kz and the normalised height y/ytop; its output is v
-// at those points scaled by sv = max|g| / (kz^2 + kappa^2).  The prediction
+// is the source S = B g (function_source) fully weighted down to that level
+// (scaled by the mode's max |S| there), kz and the normalised height y/ytop;
+// its output is v at that level's points scaled by
+// sv = max|S| / (kz^2 + kappa^2).  The prediction
 // is prolongated to the fine grid with the multigrid's own bilinear
 // interpolation (poisson_par.f:877-920) onto the interior of the fine grid;
 // the boundary columns and the wall row keep the warm start's values.  The
@@ -37,24 +38,33 @@ struct Guess {
 
 namespace {
 
-// max |g| per mode over the fine grid (internal layout)
+// max |g| per mode over the fine grid (internal layout): grid (blocks, K),
+// per-block maxima combined with an atomic max on the bit pattern (out must
+// be cleared first); a mode whose g is zero gets 1 (finish_absmax_kernel)
 __global__ void mode_absmax_kernel(Level F, int K, const cx<double> *G, double *out)
 {
-    const int k = blockIdx.x;
+    const int k = blockIdx.y;
     double m = 0.0;
     const cx<double> *g = G + static_cast<size_t>(k) * F.plane;
-    for (size_t e = threadIdx.x; e < F.plane; e += blockDim.x) {
+    for (size_t e = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; e < F.plane;
+         e += static_cast<size_t>(gridDim.x) * 256) {
         const int j = static_cast<int>(e % F.ld);
         if (j < F.ny) m = fmax(m, hypot(g[e].re, g[e].im));
     }
-    __shared__ double red[256];
-    red[threadIdx.x] = m;
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        atomicMax(reinterpret_cast<unsigned long long *>(out + k), __double_as_longlong(m));
     }
-    if (threadIdx.x == 0) out[k] = red[0] > 0 ? red[0] : 1.0;
+}
+
+__global__ void finish_absmax_kernel(int K, double *out)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < K && !(out[k] > 0)) out[k] = 1.0;
 }
 
 // features (K, H, W, 4) at level lg from the fine g (stride s = 2^(lg-1))
@@ -192,14 +202,31 @@ static int guess_setup(tbl_ctx *c, tbl_net *net, int lvl)
     return TBL_OK;
 }
 
+// W[0] = V[0]; for l < L: W[l+1] = injection of W[l], S[l+1] = full weighting
+// of S[l] (R[l+1] takes the injection's unused copy).  Needs S[0].
+static void restrict_chain(tbl_ctx *c, int L, hipStream_t st)
+{
+    const Level &F = c->lv[0];
+    convert<double, double>(F.plane * c->K, c->V[0], c->W[0], 0, st);
+    for (int l = 0; l < L; ++l)
+        restrict_level<double>(c->lv[l], c->lv[l + 1], c->K, c->W[l], c->S[l], c->W[l + 1],
+                               c->R[l + 1], c->S[l + 1], 1, nullptr, st);
+}
+
+// features from the fully weighted source on the network's level (S[L] after
+// restrict_chain): point samples of the fine source would alias its
+// short-wave content on the coarse levels
 static void build_features(tbl_ctx *c, hipStream_t st)
 {
     Guess *g = c->guess;
-    const Level &F = c->lv[0], &Cl = c->lv[g->lvl - 1];
-    hipLaunchKernelGGL(mode_absmax_kernel, dim3(c->K), dim3(256), 0, st, F, c->K, c->R[0], g->sf);
+    const int L = g->lvl - 1;
+    const Level &Cl = c->lv[L];
+    fill(g->sf, c->K, 0.0, st);
+    hipLaunchKernelGGL(mode_absmax_kernel, dim3(16, c->K), dim3(256), 0, st, Cl, c->K, c->S[L], g->sf);
+    hipLaunchKernelGGL(finish_absmax_kernel, dim3((c->K + 255) / 256), dim3(256), 0, st, c->K, g->sf);
     const long n = static_cast<long>(c->K) * g->H * g->W;
-    hipLaunchKernelGGL(features_kernel, dim3(nblk(n)), dim3(256), 0, st, F, Cl, c->K, g->H, g->W,
-                       1 << (g->lvl - 1), c->R[0], g->sf, g->ycoarse, g->ytop, c->g.beta,
+    hipLaunchKernelGGL(features_kernel, dim3(nblk(n)), dim3(256), 0, st, Cl, Cl, c->K, g->H, g->W, 1,
+                       c->S[L], g->sf, g->ycoarse, g->ytop, c->g.beta,
                        static_cast<float>(std::sqrt(1.0 / g->kappa2)), g->feat);
 }
 
@@ -212,19 +239,15 @@ int guess_apply(tbl_ctx *c, hipStream_t st, bool accept_all, int *fallback,
     if (!g || !g->net) return TBL_ERR_STATE;
     const Level &F = c->lv[0];
     const int K = c->K;
+    const int L = g->lvl - 1;
+    const Level &Cl = c->lv[L];
+    // warm start (V[0] after gather) kept in W[0] and injected down to level
+    // L (W[L]); the source fully weighted alongside (S[L])
+    restrict_chain(c, L, st);
     build_features(c, st);
     TBL_HIP(hipStreamSynchronize(st));
     int rc = tbl_net_forward(g->net, g->feat, g->pred, K);
     if (rc != TBL_OK) return rc;
-    const int L = g->lvl - 1;
-    const Level &Cl = c->lv[L];
-    // warm start (V[0] after gather) kept in W[0]; injected down to level L
-    // (W[l]) with the source fully weighted alongside (S[l]); R[l] takes
-    // the injection's unused copy
-    convert<double, double>(F.plane * K, c->V[0], c->W[0], 0, st);
-    for (int l = 0; l < L; ++l)
-        restrict_level<double>(c->lv[l], c->lv[l + 1], K, c->W[l], l == 0 ? c->S[0] : c->S[l],
-                               c->W[l + 1], c->R[l + 1], c->S[l + 1], 1, nullptr, st);
     hipLaunchKernelGGL(place_kernel, dim3(nblk(Cl.plane * K)), dim3(256), 0, st, Cl, K, g->H, g->W,
                        g->pred, g->sf, c->g.beta, g->kappa2, c->V[L]);
     // residuals on level L after kGuessSweeps line sweeps of each candidate
@@ -291,6 +314,8 @@ extern "C" int tbl_guess_pairs(tbl_ctx *c, const double *d_f, const double *d_v,
     hipStream_t st = c->stream;
     const Level &F = c->lv[0], &Cl = c->lv[g->lvl - 1];
     gather(c->g.nx, c->g.ny, c->K, F, nullptr, d_f, d_v, c->g.beta, c->R[0], c->V[0], 1, st);
+    source(c->la_d[0], c->R[0], c->S[0], st);
+    restrict_chain(c, g->lvl - 1, st);
     build_features(c, st);
     const long n = static_cast<long>(c->K) * g->H * g->W;
     hipLaunchKernelGGL(targets_kernel, dim3(nblk(n)), dim3(256), 0, st, F, Cl, c->K, g->H, g->W,

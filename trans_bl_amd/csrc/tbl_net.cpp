@@ -33,6 +33,7 @@ namespace {
 struct Conv {
     int cin, cout, ks, lvl, relu;
     long woff, boff;   // offsets into the flat parameter vector
+    long wpoff;        // offset of the packed 3x3 weights (direct convolution)
 };
 
 enum {
@@ -59,6 +60,7 @@ struct tbl_net {
     // parameters: fp32 master, bf16 copy, gradient, Adam moments
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     bf16 *wb = nullptr;
+    bf16 *wp = nullptr;          // 3x3 weights packed for the direct convolution
     long step = 0;
     // activations (bf16, NHWC) and fp32 pre/post-activation scratch
     bf16 *x_in = nullptr;
@@ -109,6 +111,12 @@ int net_alloc(tbl_net *n)
     int rc;
 #define A_(p, cnt) if ((rc = dmalloc(&(p), (cnt))) != TBL_OK) return rc
     A_(n->w, np); A_(n->g, np); A_(n->m, np); A_(n->v, np); A_(n->wb, np);
+    long npk = 0;
+    for (auto &c : n->L) {
+        c.wpoff = npk;
+        if (c.ks == 3) npk += 9L * c.cout * conv3x3_cinp(c.cin);
+    }
+    A_(n->wp, npk);
     const long N = n->nmax;
     long maxz = 0, maxcol = 0;
     for (auto &c : n->L) {
@@ -164,11 +172,11 @@ void conv_fwd(tbl_net *n, int id, const bf16 *x, int N, hipStream_t st)
     const int h = n->H >> c.lvl, w = n->W >> c.lvl;
     const long P = static_cast<long>(N) * h * w;
     const int kk = c.ks * c.ks * c.cin;
-    const bf16 *A = x;
     if (c.ks == 3) {
-        im2col(N, h, w, c.cin, x, n->col, st);
-        A = n->col;
+        conv3x3(N, h, w, c.cin, c.cout, x, n->wp + c.wpoff, n->w + c.boff, n->act[id], c.relu, st);
+        return;
     }
+    const bf16 *A = x;
     float *Z = (id == OUT) ? n->out : n->z;
     gemm(static_cast<int>(P), c.cout, kk, A, kk, 1, n->wb + c.woff, c.cout, 1, Z, c.cout, 1.f,
          0.f, st);
@@ -210,6 +218,9 @@ void conv_bwd(tbl_net *n, int id, const bf16 *x, const float *dy, float *dx, int
 void forward(tbl_net *n, int N, hipStream_t st)
 {
     const int B = n->base;
+    // the bf16 weights change with every Adam step and set_params: repack
+    for (auto &c : n->L)
+        if (c.ks == 3) pack_conv3x3(c.cin, c.cout, n->wb + c.woff, n->wp + c.wpoff, st);
     conv_fwd(n, E0A, n->x_in, N, st);
     conv_fwd(n, E0B, n->act[E0A], N, st);
     pool(N, n->H, n->W, B, n->act[E0B], n->pool_out[0], st);
@@ -304,7 +315,7 @@ extern "C" int tbl_net_destroy(tbl_net *n)
 {
     if (!n) return TBL_ERR_ARG;
     auto fr = [](void *p) { if (p) (void)hipFree(p); };
-    fr(n->w); fr(n->g); fr(n->m); fr(n->v); fr(n->wb); fr(n->x_in); fr(n->d_xin);
+    fr(n->w); fr(n->g); fr(n->m); fr(n->v); fr(n->wb); fr(n->wp); fr(n->x_in); fr(n->d_xin);
     for (int i = 0; i < NCONV; ++i) { fr(n->act[i]); fr(n->d_act[i]); }
     for (int l = 0; l < 3; ++l) { fr(n->pool_out[l]); fr(n->d_pool[l]); fr(n->cat[l]); fr(n->d_cat[l]); }
     fr(n->z); fr(n->out); fr(n->d_out); fr(n->col); fr(n->dcol); fr(n->dzb); fr(n->d_loss);

@@ -30,6 +30,12 @@ void mse(long n, const float *y, const float *t, float *grad, float *loss, hipSt
 void adam(long n, float *w, bf16 *wb, const float *g, float *m, float *v, float lr, float b1,
           float b2, float eps, float c1, float c2, hipStream_t st);
 void zero(long n, float *a, hipStream_t st);
+// direct 3x3 convolution + bias (+ ReLU) -> bf16 NHWC; weights packed by
+// pack_conv3x3 into [tap][cout][cin rounded up to 32]; cout a multiple of 32
+int conv3x3_cinp(int cin);
+void pack_conv3x3(int cin, int cout, const bf16 *w, bf16 *wp, hipStream_t st);
+void conv3x3(int n, int h, int w, int cin, int cout, const bf16 *x, const bf16 *wp,
+             const float *bias, bf16 *y, int relu, hipStream_t st);
 
 }  // namespace nn
 }  // namespace tbl

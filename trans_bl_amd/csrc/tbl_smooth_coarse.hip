@@ -243,29 +243,70 @@ __global__ void __launch_bounds__(256) smooth_coarsest_2s_kernel(LevelArgs<T> L,
         slu[e] = glu[static_cast<size_t>(e / NY) * ld + e % NY];
     __syncthreads();
     const T qx = L.qx;
+    // stage-1 lane roles for the colour phases: RW lanes per row set, NG line
+    // groups; each lane's row kind is fixed, so its stencil weights are set
+    // once: (a, b, c) applied as (a X(j-1) + b X(j)) + c X(j+1), which equals
+    // the reference's row forms exactly (zero terms add nothing, and the
+    // two-term sum of row 2 only commutes)
+    constexpr int RW = NY <= 32 ? 32 : 64;
+    constexpr int NG = 256 / RW;
+    const int jr = threadIdx.x % RW, cs = threadIdx.x / RW;
+    T ca = T(0), cb = T(1), cc = T(0);
+    if (jr == 1) {
+        cb = L.sp[1];
+        cc = L.sp[2];
+    } else if (jr >= 2 && jr <= NY - 3) {
+        ca = L.cp[0];
+        cb = L.cp[1];
+        cc = L.cp[2];
+    }
+    const bool grouped = jr >= NY - 2;
     for (int it = 0; it < sweeps; ++it) {
         for (int phase = 0; phase < 4; ++phase) {
             const int nl = phase_lines(phase, nx);
-            // stage 1: right-hand sides (rhs_border1/2, rhs_center)
-            for (int e = threadIdx.x; e < nl * NY; e += blockDim.x) {
-                const int c = e / NY, j = e % NY;
-                const int i = phase_column(phase, c, nx);
-                const int side = (phase == 0) ? (c == 0 ? -1 : 1) : 0;
-                const cx<T> *col = sv + static_cast<size_t>(i) * LS;
-                auto X = [&](int jj) {
-                    return xoff<T>(col, LS, jj, side, side == 0 && jj >= NY - 2);
-                };
-                cx<T> rhs;
-                if (j == 0) {
-                    rhs = col[0];
-                } else {
-                    const cx<T> sj = ss[static_cast<size_t>(i) * LS + j];
-                    if (j == 1) rhs = sj - qx * (L.sp[2] * X(2) + L.sp[1] * X(1));
-                    else if (j <= NY - 3)
-                        rhs = sj - qx * (L.cp[0] * X(j - 1) + L.cp[1] * X(j) + L.cp[2] * X(j + 1));
-                    else rhs = sj - qx * X(j);
+            if (phase == 0) {
+                for (int e = threadIdx.x; e < nl * NY; e += blockDim.x) {
+                    const int c = e / NY, j = e % NY;
+                    const int i = phase_column(0, c, nx);
+                    const int side = c == 0 ? -1 : 1;
+                    const cx<T> *col = sv + static_cast<size_t>(i) * LS;
+                    auto X = [&](int jj) { return xoff<T>(col, LS, jj, side, false); };
+                    cx<T> rhs;
+                    if (j == 0) {
+                        rhs = col[0];
+                    } else {
+                        const cx<T> sj = ss[static_cast<size_t>(i) * LS + j];
+                        if (j == 1) rhs = sj - qx * (L.sp[2] * X(2) + L.sp[1] * X(1));
+                        else if (j <= NY - 3)
+                            rhs = sj - qx * (L.cp[0] * X(j - 1) + L.cp[1] * X(j) + L.cp[2] * X(j + 1));
+                        else rhs = sj - qx * X(j);
+                    }
+                    rbuf[c * LS + j] = rhs;
                 }
-                rbuf[c * LS + j] = rhs;
+            } else {
+                // colour phases: lane (cs, jr) forms row jr of lines cs, cs+NG, ...;
+                // the row's own x term from four loads, its neighbours' by shuffles
+                const int ncg = (nl + NG - 1) / NG;
+                for (int q = 0; q < ncg; ++q) {
+                    const int c = cs + q * NG;
+                    const int i = 2 + (phase - 1) + 3 * min(c, nl - 1);
+                    const cx<T> *col = sv + static_cast<size_t>(i) * LS;
+                    cx<T> x = {T(0), T(0)};
+                    if (jr < NY) {
+                        const cx<T> m2 = col[jr - 2 * LS], m1 = col[jr - LS];
+                        const cx<T> p1 = col[jr + LS], p2 = col[jr + 2 * LS];
+                        x = grouped ? -(p2 + m2) + T(16) * (p1 + m1)
+                                    : T(-1) * m2 + T(16) * m1 + T(16) * p1 - T(1) * p2;
+                    }
+                    const cx<T> xm = shfl_up(x, 1, RW), xp = shfl_down(x, 1, RW);
+                    if (c < nl && jr < NY) {
+                        const cx<T> rhs =
+                            jr == 0 ? col[0]
+                                    : ss[static_cast<size_t>(i) * LS + jr] -
+                                          qx * ((ca * xm + cb * x) + cc * xp);
+                        rbuf[c * LS + jr] = rhs;
+                    }
+                }
             }
             __syncthreads();
             // stage 2: two threads (real, imaginary part) per line

@@ -626,7 +626,11 @@ extern "C" int tbl_time_smoother(tbl_ctx *c, int prec, int level, int sweeps, do
     const int l = level - 1;
     hipStream_t st = c->stream;
     TBL_HIP(hipEventRecord(c->ev0, st));
-    if (prec)
+    if (level == c->L)
+        // the coarsest level's solve (FP64 in both precisions), one launch
+        TRY(smooth_coarsest<double>(c->la_d[l], c->V[l], c->S[l], c->lu_d[l], sweeps, 1.0,
+                                    nullptr, c->G64[l], st));
+    else if (prec)
         TRY(smooth_sweeps<float>(c->la_f[l], c->E32[l], c->S32[l], c->lu_f[l], sweeps, 1.2f,
                                  nullptr, c->G32[l], c->H32[l], st));
     else
@@ -638,7 +642,11 @@ extern "C" int tbl_time_smoother(tbl_ctx *c, int prec, int level, int sweeps, do
     float t = 0.f;
     TBL_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
     *ms = t;
-    if (launches) *launches = 4 * sweeps;
+    if (launches) {
+        if (level == c->L) *launches = 1;
+        else if (((prec ? c->G32 : c->G64)[l]) > kStripShape) *launches = 2 * sweeps;
+        else *launches = 4 * sweeps;
+    }
     return TBL_OK;
 }
 

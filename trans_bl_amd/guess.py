@@ -2,10 +2,10 @@
 training on (RHS, v) pairs that the solver itself generates, and the solve
 that starts from the network's prediction.
 
-The network runs on one coarse multigrid level (default: the second
-coarsest); its prediction is prolongated to the fine grid and multigrid runs
-to the same tolerance as without it.  Modes whose predicted start has a larger
-residual than the zero start fall back to zero.
+The network runs on one coarse multigrid level (default: the coarsest); its
+prediction is prolongated to the fine grid and multigrid runs to the same
+tolerance as without it (guess=3), or, with guess=2, only modes whose
+prediction passes a residual test on that level start from it.
 """
 from __future__ import annotations
 
@@ -121,12 +121,19 @@ def mfma_gemm(a_ptr, b_ptr, c_ptr, M, N, K, transpose_a=0, transpose_b=0, splitk
 
 
 def default_level(ctx: Context) -> int:
-    return max(1, ctx.grid.levels - 1)
+    """The coarsest multigrid level: the smooth part of v is what the
+    network must supply (the V-cycles remove the rest), and the image there
+    is 4**(levels-1) times smaller than the fine grid."""
+    return ctx.grid.levels
 
 
 def attach(ctx: Context, net: Net, level: int) -> None:
+    """Attach `net` to `ctx` on multigrid `level`.  The context keeps a
+    reference: the library uses the network's buffers in every guessed solve,
+    so the network must outlive the attachment."""
     L = _setup()
     _check(L.tbl_guess_attach(ctx.h, net.h, level), "tbl_guess_attach")
+    ctx._guess_net = net
 
 
 def image_shape(ctx: Context, level: int):
@@ -151,10 +158,12 @@ class Guess:
         self.net, self.level = net, level
         attach(ctx, net, level)
 
-    def solve(self, ctx: Context, f_dev, u_dev, tol, **kw):
-        """Solve L v = B f starting from the network's prediction."""
+    def solve(self, ctx: Context, f_dev, u_dev, tol, test=False, **kw):
+        """Solve L v = B f starting from the network's prediction (every
+        mode; with test=True only modes whose prediction passes the
+        residual test, guess=2)."""
         kw = dict(kw)
-        kw["guess"] = 2
+        kw["guess"] = 2 if test else 3
         return ctx.solve_rhs_dev(f_dev.data_ptr(), u_dev.data_ptr(), tol, **kw)
 
 

@@ -12,6 +12,13 @@ Runs, on the 1025x193x64 v-Poisson workload:
      plain FP64 solve, each from zero to 1e-9.
 Prints one JSON line per stage so the counter rows can be matched to stages
 by dispatch order.
+
+    python3 -m trans_bl_amd.prof_kernels --summarize FETCH_DIR WRITE_DIR OUT.json
+
+reads the two counter runs' CSV output (rocprofv3 --output-format csv) and
+writes, per kernel, the average FETCH_SIZE and WRITE_SIZE per dispatch, each
+scaled so that the 256 MiB copy (the copy kernel's first dispatch) reads and
+writes exactly 256 MiB, as HBM bytes per launch.
 """
 from __future__ import annotations
 
@@ -57,5 +64,53 @@ def main():
     ctx.close()
 
 
+def _counter_rows(directory, counter):
+    import csv
+    import glob
+    rows = []
+    for path in glob.glob(os.path.join(directory, "*counter_collection.csv")):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] == counter:
+                    rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    return rows
+
+
+def summarize(fetch_dir, write_dir, out):
+    """Per-kernel HBM bytes per dispatch from the two counter runs."""
+    fetch = _counter_rows(fetch_dir, "FETCH_SIZE")
+    write = _counter_rows(write_dir, "WRITE_SIZE")
+    copy_kib = COPY_BYTES / 1024.0
+
+    def calib(rows):
+        for _, name, v in rows:
+            if "copyBuffer" in name or "copy" in name.lower() and "kernel" in name.lower():
+                return copy_kib / v
+        raise SystemExit("no copy dispatch found for the calibration")
+
+    fs, ws = calib(fetch), calib(write)
+    table = {}
+    for rows, key, scale in ((fetch, "fetch", fs), (write, "write", ws)):
+        for _, name, v in rows:
+            e = table.setdefault(name, {"fetch_kib": [], "write_kib": []})
+            e[key + "_kib"].append(v * scale)
+    res = {"fetch_scale": fs, "write_scale": ws, "kernels": {}}
+    for name, e in table.items():
+        if not e["fetch_kib"] or not e["write_kib"]:
+            continue
+        f = sum(e["fetch_kib"]) / len(e["fetch_kib"])
+        w = sum(e["write_kib"]) / len(e["write_kib"])
+        res["kernels"][name] = {"dispatches": len(e["fetch_kib"]),
+                                "fetch_bytes": round(f * 1024), "write_bytes": round(w * 1024),
+                                "hbm_bytes_per_launch": round((f + w) * 1024)}
+    with open(out, "w") as fo:
+        json.dump(res, fo, indent=1, sort_keys=True)
+    print(json.dumps({"kernels": len(res["kernels"]), "fetch_scale": fs, "write_scale": ws}))
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) == 5 and sys.argv[1] == "--summarize":
+        summarize(sys.argv[2], sys.argv[3], sys.argv[4])
+    else:
+        main()
