@@ -70,7 +70,8 @@ typedef struct tbl_solve_opts {
     int max_cycles;   /* 0 -> 11, the reference cap (poisson_par.f:33) */
     int nu1;          /* pre-smoothing sweeps, 0 -> 3 (itera, poisson_par.f:24) */
     int nu2;          /* post-smoothing sweeps, 0 -> 1 (poisson_par.f:49) */
-    int ncoarse;      /* coarsest-level sweeps, 0 -> 60 (20*itera, :45) */
+    int ncoarse;      /* coarsest-level sweeps, 0 -> 60 (20*itera, :45) for
+                         precision 0, 10 for the FP32 correction cycle */
     int use_graph;    /* 1: replay the V-cycle as a captured hipGraph */
     int guess;        /* 0: start from uy (the reference's warm start,
                             poisson_par.f:157-161); 1: start from zero;

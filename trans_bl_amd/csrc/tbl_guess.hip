@@ -22,7 +22,10 @@
 #include "tbl_ctx.h"
 
 struct tbl_net;
-extern "C" int tbl_net_forward(tbl_net *n, const float *d_x, float *d_y, int N);
+
+namespace tbl {
+int net_forward_async(tbl_net *n, const float *d_x, float *d_y, int N, hipStream_t st);
+}
 
 namespace tbl {
 
@@ -241,33 +244,55 @@ int guess_apply(tbl_ctx *c, hipStream_t st, bool accept_all, int *fallback,
     const int K = c->K;
     const int L = g->lvl - 1;
     const Level &Cl = c->lv[L];
+    if (accept_all && L > 0) {
+        // every mode starts from the prediction: restrict the source only,
+        // and assign the interpolant to the fine interior (the warm start's
+        // boundary columns and wall row stay)
+        for (int l = 0; l < L; ++l)
+            restrict_level<double>(c->lv[l], c->lv[l + 1], K, c->V[l], c->S[l], c->W[l + 1],
+                                   nullptr, c->S[l + 1], 0, nullptr, st);
+        build_features(c, st);
+        int rc = net_forward_async(g->net, g->feat, g->pred, K, st);
+        if (rc != TBL_OK) return rc;
+        hipLaunchKernelGGL(place_kernel, dim3(nblk(Cl.plane * K)), dim3(256), 0, st, Cl, K, g->H,
+                           g->W, g->pred, g->sf, c->g.beta, g->kappa2, c->V[L]);
+        for (int l = L; l >= 1; --l) {
+            if (l - 1 > 0)
+                fill(reinterpret_cast<double *>(c->V[l - 1]), c->lv[l - 1].plane * K * 2, 0.0, st);
+            prolong_level<double>(c->lv[l - 1], c->lv[l], K, c->V[l - 1], c->V[l], nullptr,
+                                  l - 1 > 0 ? 0 : 2, nullptr, st);
+        }
+        TBL_HIP(hipGetLastError());
+        *fallback = 0;
+        *guess_res_max = 0.0;
+        return TBL_OK;
+    }
     // warm start (V[0] after gather) kept in W[0] and injected down to level
     // L (W[L]); the source fully weighted alongside (S[L])
     restrict_chain(c, L, st);
     build_features(c, st);
-    TBL_HIP(hipStreamSynchronize(st));
-    int rc = tbl_net_forward(g->net, g->feat, g->pred, K);
+    int rc = net_forward_async(g->net, g->feat, g->pred, K, st);
     if (rc != TBL_OK) return rc;
     hipLaunchKernelGGL(place_kernel, dim3(nblk(Cl.plane * K)), dim3(256), 0, st, Cl, K, g->H, g->W,
                        g->pred, g->sf, c->g.beta, g->kappa2, c->V[L]);
     // residuals on level L after kGuessSweeps line sweeps of each candidate
     // (the prediction's pixel-scale noise is what the operator amplifies most;
     // the sweeps remove it, and the smoothed prediction is the one used)
+    // (without the test only the prediction is swept)
     if (L > 0) {
         for (cx<double> *cand : {c->V[L], c->W[L]}) {
+            if (accept_all && cand == c->W[L]) continue;
             rc = smooth_sweeps<double>(c->la_d[L], cand, c->S[L], c->lu_d[L], kGuessSweeps, 1.2,
                                        nullptr, c->G64[L], c->H64[L], st);
             if (rc != TBL_OK) return rc;
         }
     }
-    fill(g->resg, K, 0.0, st);
-    fill(g->res0, K, 0.0, st);
-    if (L > 0) {
-        defect<double, double>(c->la_d[L], c->V[L], c->S[L], c->R[L], -1.0, nullptr, g->resg, st);
-        defect<double, double>(c->la_d[L], c->W[L], c->S[L], c->R[L], -1.0, nullptr, g->res0, st);
-    } else {
-        defect<double, double>(c->la_d[0], c->V[0], c->S[0], c->R[0], -1.0, nullptr, g->resg, st);
-        defect<double, double>(c->la_d[0], c->W[0], c->S[0], c->R[0], -1.0, nullptr, g->res0, st);
+    if (!accept_all) {
+        fill(g->resg, K, 0.0, st);
+        fill(g->res0, K, 0.0, st);
+        const int l = L > 0 ? L : 0;
+        defect<double, double>(c->la_d[l], c->V[l], c->S[l], c->R[l], -1.0, nullptr, g->resg, st);
+        defect<double, double>(c->la_d[l], c->W[l], c->S[l], c->R[l], -1.0, nullptr, g->res0, st);
     }
     for (int l = L; l >= 1; --l) {
         fill(reinterpret_cast<double *>(c->V[l - 1]), c->lv[l - 1].plane * K * 2, 0.0, st);
@@ -277,6 +302,11 @@ int guess_apply(tbl_ctx *c, hipStream_t st, bool accept_all, int *fallback,
     hipLaunchKernelGGL(fallback_kernel, dim3(nblk(F.plane * K)), dim3(256), 0, st, F, K, g->res0,
                        g->resg, c->W[0], c->V[0], accept_all ? 1 : 0, g->fallback);
     TBL_HIP(hipGetLastError());
+    if (accept_all) {
+        *fallback = 0;
+        *guess_res_max = 0.0;
+        return TBL_OK;
+    }
     std::vector<double> rg(K);
     TBL_HIP(hipMemcpyAsync(fallback, g->fallback, sizeof(int), hipMemcpyDeviceToHost, st));
     TBL_HIP(hipMemcpyAsync(rg.data(), g->resg, K * sizeof(double), hipMemcpyDeviceToHost, st));

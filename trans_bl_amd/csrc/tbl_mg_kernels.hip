@@ -21,12 +21,13 @@ __device__ __forceinline__ void atomic_max_nonneg(double *addr, double v)
 // rows of one strip of columns of one mode.  Its 64 lanes hold rows
 // j0-2 .. j0+61 (two halo rows each side); it walks the strip in x keeping
 // the five columns i-2..i+2 of its rows in registers (one new column load per
-// step) and takes the y-neighbours from lane shuffles, so every value of v is
+// step, issued two steps ahead) and takes the y-neighbours from lane shuffles, so every value of v is
 // loaded about once.  Row forms: Fortran row 2 (sp_poi_coef), interior rows
 // (cp_poi_coef), row ny-1 (pp_poi_coef) and row ny (lp_poi_coef, with the
 // top boundary term); x part: the border-column one-sided stencils on
 // columns 2 and nx-1, the centred fourth-order stencil elsewhere (grouped
-// on the two top rows).
+// on the two top rows).  The divisions by 12 dx^2 fac_y and fac_y are
+// multiplications by their reciprocals (last-bit rounding differs).
 constexpr int kDefRows = 60;
 
 __host__ __device__ inline int defect_strip(int nx) { return nx >= 512 ? 32 : 16; }
@@ -57,13 +58,22 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
     const T k2b2 = -kk * kk * L.beta * L.beta;
     const T alpha2 = sqrt((L.alpha * L.alpha - k2b2) / L.fac_y);
     const T scx = T(12) * L.dx2 * L.fac_y;
+    // reciprocals of the two per-level divisors (b/scx, d/fac_y below): the
+    // per-point divisions were most of the kernel's VALU work
+    const T iscx = T(1) / scx, ifac = T(1) / L.fac_y;
     T q = T(0);
     if (own) q = (j == 1) ? L.qdy2[0] : (j == ny - 2) ? L.qdy2[j - 1] : L.qdy2[j - 2];
     cx<T> w0 = vload(ia - 2), w1 = vload(ia - 1), w2 = vload(ia), w3 = vload(ia + 1),
           w4 = vload(ia + 2);
+    // two further columns in flight (prefetch distance 2)
+    cx<T> p0 = vload(ia + 3), p1 = vload(ia + 4);
+    const cx<T> *sk = S + static_cast<size_t>(k) * L.plane;
+    cx<T> s_cur = own ? sk[static_cast<size_t>(ia) * ld + j] : cx<T>{T(0), T(0)};
     double amax = 0.0;
     for (int i = ia; i < ib; ++i) {
-        const cx<T> nxt = vload(i + 3);
+        const cx<T> pn = vload(i + 5);
+        const cx<T> s_nxt = (own && i + 1 < ib) ? sk[static_cast<size_t>(i + 1) * ld + j]
+                                             : cx<T>{T(0), T(0)};
         cx<T> xs, xo;   // x part at row j: shared (ungrouped) and own
         if (i == 1) {
             xs = T(10) * w1 - T(15) * w2 - T(4) * w3 + T(14) * w4 - T(6) * vload(4) + vload(5);
@@ -85,28 +95,32 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
                 const cx<T> b = L.sp[2] * xp + L.sp[1] * xs;
                 const cx<T> c = L.sp[3] * vm1 + L.sp[4] * w2 + L.sp[5] * vp1 + L.sp[6] * vp2;
                 const cx<T> d = k2b2 * (L.sp[1] * w2 + L.sp[2] * vp1);
-                lv = b / scx + c * q + d / L.fac_y;
+                lv = b * iscx + c * q + d * ifac;
             } else if (j <= ny - 3) {
                 const cx<T> b = L.cp[0] * xm + L.cp[1] * xs + L.cp[2] * xp;
                 const cx<T> c = L.cp[3] * vm2 + L.cp[4] * vm1 + L.cp[5] * w2 + L.cp[6] * vp1
                                 + L.cp[7] * vp2;
                 const cx<T> d = k2b2 * (L.cp[0] * vm1 + L.cp[1] * w2 + L.cp[2] * vp1);
-                lv = b / scx + c * q + d / L.fac_y;
+                lv = b * iscx + c * q + d * ifac;
             } else if (j == ny - 2) {
                 const cx<T> c = L.pp[3] * vp1 + L.pp[2] * w2 + L.pp[1] * vm1;
                 const cx<T> d = k2b2 * w2;
-                lv = xo / scx + c * q + d / L.fac_y;
+                lv = xo * iscx + c * q + d * ifac;
             } else {
                 const cx<T> c = L.lp[2] * w2 + L.lp[3] * vm1 + L.lp[4] * vm2;
                 const cx<T> d = k2b2 * w2;
-                lv = xo / scx + c * q + d / L.fac_y - L.lp[1] * alpha2 * w2 * L.sqq_top;
+                lv = xo * iscx + c * q + d * ifac - L.lp[1] * alpha2 * w2 * L.sqq_top;
             }
             const size_t o = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * ld + j;
-            const cx<T> r = S[o] + si * lv;
+            const cx<T> r = s_cur + si * lv;
             out[o] = cx<TO>{static_cast<TO>(r.re), static_cast<TO>(r.im)};
-            if (resmax) amax = fmax(amax, hypot(static_cast<double>(r.re), static_cast<double>(r.im)));
+            if (resmax) {
+                const double rr = static_cast<double>(r.re), ri = static_cast<double>(r.im);
+                amax = fmax(amax, sqrt(rr * rr + ri * ri));
+            }
         }
-        w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = nxt;
+        w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = p0; p0 = p1; p1 = pn;
+        s_cur = s_nxt;
     }
     if (resmax) {
         for (int o = 32; o > 0; o >>= 1) amax = fmax(amax, __shfl_down(amax, o, 64));
@@ -159,7 +173,9 @@ __global__ void __launch_bounds__(256) restrict_kernel(Level F, Level Cl, int K,
 // correction + intpol (poisson_par.f:877-942).  The coarse correction is
 // v_c - w_c on interior columns and v_c itself on the two boundary columns
 // (the reference interpolates those unchanged values as well).  For the
-// correction scheme (fas == 0) it is v_c.
+// correction scheme (fas == 0) it is v_c.  fas == 2 assigns the bilinear
+// interpolant of v_c to the fine interior (rows 2..ny, Fortran) instead of
+// adding it.
 template <typename T>
 __global__ void __launch_bounds__(256) prolong_kernel(Level F, Level Cl, int K, cx<T> *Vf,
                                                       const cx<T> *Vc, const cx<T> *Wc,
@@ -176,7 +192,7 @@ __global__ void __launch_bounds__(256) prolong_kernel(Level F, Level Cl, int K, 
     const cx<T> *wc = Wc + static_cast<size_t>(k) * Cl.plane;
     auto E = [&](int ci, int cj) {
         const size_t o = static_cast<size_t>(ci) * Cl.ld + cj;
-        if (fas && ci >= 1 && ci <= Cl.nx - 2) return vc[o] - wc[o];
+        if (fas == 1 && ci >= 1 && ci <= Cl.nx - 2) return vc[o] - wc[o];
         return vc[o];
     };
     auto row = [&](int cj) {   // value on an even (0-based) fine row
@@ -187,7 +203,11 @@ __global__ void __launch_bounds__(256) prolong_kernel(Level F, Level Cl, int K, 
     if ((j & 1) == 0) t = row(j >> 1);
     else t = T(0.5) * (row((j >> 1) + 1) + row(j >> 1));
     const size_t o = static_cast<size_t>(k) * F.plane + static_cast<size_t>(i) * F.ld + j;
-    Vf[o] = Vf[o] + t;
+    if (fas == 2) {
+        if (j >= 1) Vf[o] = t;   // assign the interpolant, wall row untouched
+    } else {
+        Vf[o] = Vf[o] + t;
+    }
 }
 
 // ---------------------------------------------------------------------------

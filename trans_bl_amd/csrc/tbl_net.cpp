@@ -61,6 +61,7 @@ struct tbl_net {
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     bf16 *wb = nullptr;
     bf16 *wp = nullptr;          // 3x3 weights packed for the direct convolution
+    bool wp_stale = true;        // wb changed since the last packing
     long step = 0;
     // activations (bf16, NHWC) and fp32 pre/post-activation scratch
     bf16 *x_in = nullptr;
@@ -219,8 +220,11 @@ void forward(tbl_net *n, int N, hipStream_t st)
 {
     const int B = n->base;
     // the bf16 weights change with every Adam step and set_params: repack
-    for (auto &c : n->L)
-        if (c.ks == 3) pack_conv3x3(c.cin, c.cout, n->wb + c.woff, n->wp + c.wpoff, st);
+    if (n->wp_stale) {
+        for (auto &c : n->L)
+            if (c.ks == 3) pack_conv3x3(c.cin, c.cout, n->wb + c.woff, n->wp + c.wpoff, st);
+        n->wp_stale = false;
+    }
     conv_fwd(n, E0A, n->x_in, N, st);
     conv_fwd(n, E0B, n->act[E0A], N, st);
     pool(N, n->H, n->W, B, n->act[E0B], n->pool_out[0], st);
@@ -300,6 +304,7 @@ extern "C" int tbl_net_create(int H, int W, int max_batch, unsigned seed, tbl_ne
         n->stream = n->own;
         if (rc == TBL_OK) {
             to_bf16(n->nparam, n->w, n->wb, n->stream);
+            n->wp_stale = true;
             if (hipStreamSynchronize(n->stream) != hipSuccess) rc = TBL_ERR_HIP;
         }
     }
@@ -344,15 +349,25 @@ extern "C" int tbl_net_set_params(tbl_net *n, const float *host)
 }
 
 // x: device fp32 NHWC (N, H, W, 4); y: device fp32 (N, H, W, 2)
-extern "C" int tbl_net_forward(tbl_net *n, const float *d_x, float *d_y, int N)
+namespace tbl {
+// forward pass queued on `st` (no host synchronisation); the solver's guess
+// path runs it on the solve's stream
+int net_forward_async(tbl_net *n, const float *d_x, float *d_y, int N, hipStream_t st)
 {
     if (check_n(n, N) || !d_x || !d_y) return TBL_ERR_ARG;
-    hipStream_t st = n->stream;
     to_bf16(static_cast<long>(N) * n->hw(0) * 4, d_x, n->x_in, st);
     forward(n, N, st);
     TBL_HIP(hipMemcpyAsync(d_y, n->out, sizeof(float) * N * n->hw(0) * 2, hipMemcpyDeviceToDevice, st));
     TBL_HIP(hipGetLastError());
-    TBL_HIP(hipStreamSynchronize(st));
+    return TBL_OK;
+}
+}  // namespace tbl
+
+extern "C" int tbl_net_forward(tbl_net *n, const float *d_x, float *d_y, int N)
+{
+    const int rc = tbl::net_forward_async(n, d_x, d_y, N, n ? n->stream : nullptr);
+    if (rc != TBL_OK) return rc;
+    TBL_HIP(hipStreamSynchronize(n->stream));
     return TBL_OK;
 }
 
@@ -396,6 +411,7 @@ extern "C" int tbl_net_train_step(tbl_net *n, const float *d_x, const float *d_t
     const float c1 = 1.f - std::pow(b1, static_cast<float>(n->step));
     const float c2 = 1.f - std::pow(b2, static_cast<float>(n->step));
     adam(n->nparam, n->w, n->wb, n->g, n->m, n->v, lr, b1, b2, 1e-8f, c1, c2, st);
+    n->wp_stale = true;
     TBL_HIP(hipGetLastError());
     if (loss) {
         TBL_HIP(hipMemcpyAsync(loss, n->d_loss, sizeof(float), hipMemcpyDeviceToHost, st));

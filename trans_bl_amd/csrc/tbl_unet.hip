@@ -131,11 +131,14 @@ void gemm(int M, int N, int K, const bf16 *A, long sam, long sak, const bf16 *B,
 // im2col + GEMM (K index = tap * cin + c, taps row-major), without the
 // im2col matrix or the fp32 pre-activation in HBM.
 //
-// Workgroup: 4 wavefronts; output tile = one image row, PX consecutive
-// pixels x CT output channels (CT = 64: 2x2 wavefronts of 32x32; CT = 32: 4
-// wavefronts of 32 pixels).  Per 32-channel input chunk the workgroup stages
-// the 3 x (PX+2) input patch and the chunk's packed weights [tap][cout][c] in
-// LDS, then each wavefront runs 9 taps x 4 MFMAs (v_mfma_f32_32x32x8_bf16).
+// Workgroup: 4 wavefronts; output tile = PX consecutive pixels of the
+// flattened (image, row, column) index x CT output channels (CT = 64: 2x2
+// wavefronts of 32x32; CT = 32: 4 wavefronts of 32 pixels), so narrow
+// images fill whole tiles.  Per 32-channel input chunk the workgroup stages
+// the chunk's packed weights [tap][cout][c] in LDS; each lane fetches its
+// pixel's shifted input (4 channels, 8 bytes) straight from global memory
+// (cached: neighbouring lanes and taps reuse the lines), and each wavefront
+// runs 9 taps x 4 MFMAs (v_mfma_f32_32x32x8_bf16) per chunk.
 // ---------------------------------------------------------------------------
 constexpr int CK = 32;          // input channels per LDS chunk
 constexpr int CKP = CK + 4;     // padded LDS row (bank spread)
@@ -153,41 +156,28 @@ __global__ void pack_conv3x3_kernel(int cin, int cinp, int cout, const bf16 *w, 
 }
 
 template <int CT>
-__global__ void __launch_bounds__(256) conv3x3_kernel(int H, int W, int cin, int cinp, int cout,
-                                                      const bf16 *__restrict__ X,
+__global__ void __launch_bounds__(256) conv3x3_kernel(long NP, int H, int W, int cin, int cinp,
+                                                      int cout, const bf16 *__restrict__ X,
                                                       const bf16 *__restrict__ wp,
                                                       const float *__restrict__ bias,
                                                       bf16 *__restrict__ Y, int relu)
 {
     constexpr int PX = (CT == 64) ? 64 : 128;
-    __shared__ bf16 Ps[3][PX + 2][CKP];
     __shared__ bf16 Ws[9][CT][CKP];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int w0 = blockIdx.x * PX, h = blockIdx.y;
-    const int nct = cout / CT;
-    const int n = blockIdx.z / nct, o0 = (blockIdx.z % nct) * CT;
+    const long p0 = static_cast<long>(blockIdx.x) * PX;
+    const int o0 = blockIdx.y * CT;
     const int wm = (CT == 64) ? (wave >> 1) * 32 : wave * 32;
     const int wn = (CT == 64) ? (wave & 1) * 32 : 0;
+    // this lane's A-operand pixel (row of the 32x8 fragment)
+    const long pa = p0 + wm + (lane & 31);
+    const bool pa_ok = pa < NP;
+    const int wa = static_cast<int>(pa % W), ha = static_cast<int>((pa / W) % H);
+    const long na = pa / (static_cast<long>(W) * H);
     float16v acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.f;
     for (int c0 = 0; c0 < cinp; c0 += CK) {
-        // input patch: rows h-1..h+1, columns w0-1..w0+PX, channels c0..c0+CK
-        for (int e = tid; e < 3 * (PX + 2) * (CK / 4); e += 256) {
-            const int q = e % (CK / 4), col = (e / (CK / 4)) % (PX + 2), r = e / ((CK / 4) * (PX + 2));
-            const int hh = h + r - 1, ww = w0 + col - 1, c = c0 + 4 * q;
-            short4v v = {0, 0, 0, 0};
-            if (hh >= 0 && hh < H && ww >= 0 && ww < W) {
-                const bf16 *src = X + ((static_cast<long>(n) * H + hh) * W + ww) * cin + c;
-                if (c + 3 < cin) {
-                    v = *reinterpret_cast<const short4v *>(src);
-                } else {
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) v[t] = (c + t < cin) ? static_cast<short>(src[t]) : 0;
-                }
-            }
-            *reinterpret_cast<short4v *>(&Ps[r][col][4 * q]) = v;
-        }
         // weights of this chunk: [tap][cout o0..o0+CT][c0..c0+CK]
         for (int e = tid; e < 9 * CT * (CK / 4); e += 256) {
             const int q = e % (CK / 4), o = (e / (CK / 4)) % CT, t = e / ((CK / 4) * CT);
@@ -197,11 +187,15 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(int H, int W, int cin, int
         __syncthreads();
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
-            const int dh = t / 3, dw = t % 3;
+            const int hh = ha + t / 3 - 1, ww = wa + t % 3 - 1;
+            const bool ok = pa_ok && hh >= 0 && hh < H && ww >= 0 && ww < W;
+            const bf16 *src = X + ((na * H + hh) * W + ww) * cin;
 #pragma unroll
             for (int kk = 0; kk < CK; kk += 8) {
                 const int kb = kk + (lane >> 5) * 4;
-                const short4v a = *reinterpret_cast<const short4v *>(&Ps[dh][wm + (lane & 31) + dw][kb]);
+                // cin is a multiple of 4: the 4 channels are all in range or none
+                short4v a = {0, 0, 0, 0};
+                if (ok && c0 + kb < cin) a = *reinterpret_cast<const short4v *>(src + c0 + kb);
                 const short4v b = *reinterpret_cast<const short4v *>(&Ws[t][wn + (lane & 31)][kb]);
                 acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
             }
@@ -209,16 +203,16 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(int H, int W, int cin, int
         __syncthreads();
     }
     // D layout: lane holds output channel (lane & 31); register i holds pixel
-    // 8*(i/4) + 4*(lane/32) + (i%4)
+    // 8*(i/4) + 4*(lane/32) + (i%4) of the wavefront's 32
     const int o = o0 + wn + (lane & 31);
     const float bo = bias[o];
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
-        const int w = w0 + wm + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-        if (w < W) {
+        const long p = p0 + wm + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+        if (p < NP) {
             float v = acc[i] + bo;
             if (relu) v = fmaxf(v, 0.f);
-            Y[((static_cast<long>(n) * H + h) * W + w) * cout + o] = f2bf(v);
+            Y[p * cout + o] = f2bf(v);
         }
     }
 }
@@ -511,14 +505,15 @@ void conv3x3(int n, int h, int w, int cin, int cout, const bf16 *x, const bf16 *
              const float *bias, bf16 *y, int relu, hipStream_t st)
 {
     const int cinp = conv3x3_cinp(cin);
+    const long np = static_cast<long>(n) * h * w;
     if (cout % 64 == 0) {
-        dim3 grid((w + 63) / 64, h, n * (cout / 64));
-        hipLaunchKernelGGL(conv3x3_kernel<64>, grid, dim3(256), 0, st, h, w, cin, cinp, cout, x, wp,
-                           bias, y, relu);
+        dim3 grid(static_cast<unsigned>((np + 63) / 64), cout / 64);
+        hipLaunchKernelGGL(conv3x3_kernel<64>, grid, dim3(256), 0, st, np, h, w, cin, cinp, cout, x,
+                           wp, bias, y, relu);
     } else {
-        dim3 grid((w + 127) / 128, h, n * (cout / 32));
-        hipLaunchKernelGGL(conv3x3_kernel<32>, grid, dim3(256), 0, st, h, w, cin, cinp, cout, x, wp,
-                           bias, y, relu);
+        dim3 grid(static_cast<unsigned>((np + 127) / 128), cout / 32);
+        hipLaunchKernelGGL(conv3x3_kernel<32>, grid, dim3(256), 0, st, np, h, w, cin, cinp, cout, x,
+                           wp, bias, y, relu);
     }
 }
 
