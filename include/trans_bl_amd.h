@@ -68,10 +68,11 @@ typedef struct tbl_solve_opts {
     int precision;    /* 0: FP64 FAS cycle (poisson_par.f:33-50);
                          1: FP32 line smoothing under FP64 defect correction */
     int max_cycles;   /* 0 -> 11, the reference cap (poisson_par.f:33) */
-    int nu1;          /* pre-smoothing sweeps, 0 -> 3 (itera, poisson_par.f:24) */
+    int nu1;          /* pre-smoothing sweeps, 0 -> 3 (itera, poisson_par.f:24)
+                         for precision 0, 2 for the FP32 correction cycle */
     int nu2;          /* post-smoothing sweeps, 0 -> 1 (poisson_par.f:49) */
     int ncoarse;      /* coarsest-level sweeps, 0 -> 60 (20*itera, :45) for
-                         precision 0, 10 for the FP32 correction cycle */
+                         precision 0, 5 for the FP32 correction cycle */
     int use_graph;    /* 1: replay the V-cycle as a captured hipGraph */
     int guess;        /* 0: start from uy (the reference's warm start,
                             poisson_par.f:157-161); 1: start from zero;
@@ -174,9 +175,11 @@ int tbl_residual_dev(tbl_ctx *ctx, const double *d_wx, const double *d_dwzdx,
 
 /* Line-solve shape of one multigrid level's smoother: `lanes` (16, 32 or
    64) lanes of a wavefront share one y-line; a negative value selects the
-   same shape with LDS-staged, coalesced loads and stores; 1000 + lanes
-   selects the fused-sweep strip smoother (a whole sweep per launch, one
-   workgroup per strip of columns in LDS; not on the coarsest level).
+   same shape with LDS-staged, coalesced loads and stores; 2000 + lanes
+   stages only the x-direction terms in LDS and reads v and the source
+   lane-contiguous; 1000 + lanes selects the fused-sweep strip smoother (a
+   whole sweep per launch, one workgroup per strip of columns in LDS; not
+   on the coarsest level).
    All forms give the same colour-ordered sweep.  prec 0 = FP64 smoother,
    1 = FP32 smoother.  Chosen by the autotuner; no reference counterpart
    (the reference solves each line sequentially, penta_par.f:367-404). */

@@ -181,18 +181,29 @@ def test_grid_rejection(gpu_ready):
         tb.Context(wl.Grid(129, 33, 2, 6))      # coarsest level too small
 
 
+def _lanes(code):
+    return code - 2000 if code > 2000 else code - 1000 if code > 1000 else abs(code)
+
+
+@pytest.mark.parametrize("grid", [SMALL, CFG1], ids=["small", "385x97x8"])
+@pytest.mark.parametrize("form", ["staged", "xstaged"])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_staged_smoother_matches_direct(torch_mod, precision):
-    """The LDS-staged (coalesced) smoother does the same arithmetic as the
-    direct one: identical bits."""
-    g = SMALL
+def test_staged_smoother_matches_direct(torch_mod, precision, form, grid):
+    """The LDS-staged (coalesced) smoothers -- x terms, v and S staged, or
+    only the x terms -- do the same arithmetic as the direct one: identical
+    bits."""
+    g = grid
     f = wl.wavepacket_rhs(g, tag="staged")
     z = np.zeros_like(f)
     with context(g) as ctx:
+        for lvl in range(1, g.levels + 1):
+            for prec in (0, 1):
+                ctx.set_line_shape(prec, lvl, _lanes(ctx.line_shape(prec, lvl)))
         a, ca, _, _ = ctx.poi_uy(None, -f, z, 1e-9, precision=precision, max_cycles=20)
         for lvl in range(1, g.levels + 1):
             for prec in (0, 1):
-                ctx.set_line_shape(prec, lvl, -abs(ctx.line_shape(prec, lvl)))
+                lanes = _lanes(ctx.line_shape(prec, lvl))
+                ctx.set_line_shape(prec, lvl, -lanes if form == "staged" else 2000 + lanes)
         b, cb, _, _ = ctx.poi_uy(None, -f, z, 1e-9, precision=precision, max_cycles=20)
     assert np.array_equal(a, b) and np.array_equal(ca, cb)
 

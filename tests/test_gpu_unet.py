@@ -89,6 +89,26 @@ def test_forward_and_gradients_match_oracle(torch_mod):
     assert np.linalg.norm(gxv - rdx) / np.linalg.norm(rdx) < 1e-2
 
 
+def test_forward_wide_image_matches_oracle(torch_mod):
+    """Forward at the width of the 1025x193 grid's coarsest level (image
+    32x136): the convolution kernels' LDS tiles then span more than one image
+    row and reach across rows at both ends."""
+    torch = torch_mod
+    from trans_bl_amd import guess
+    rng = np.random.default_rng(9)
+    x, _ = _problem(rng, n=1, H=32, W=136)
+    net = guess.Net(32, 136, 1, seed=13)
+    theta = U.to_bf16(net.get_params())
+    net.set_params(theta.astype(np.float32))
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.zeros((1, 32, 136, 2), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    net.forward(xd.data_ptr(), yd.data_ptr(), 1)
+    out = U.UNet(theta, bf16=True).forward(x)
+    y = yd.cpu().numpy()
+    assert np.linalg.norm(y - out) / np.linalg.norm(out) < 1e-2
+
+
 def test_training_reduces_loss(torch_mod):
     torch = torch_mod
     from trans_bl_amd import guess

@@ -15,13 +15,17 @@ import os
 
 from . import TblError
 
-# direct, LDS-staged (negative) and fused-sweep strip (1000 + lanes) forms
-CANDIDATES = (64, 32, 16, -64, -32, -16, 1064, 1032, 1016)
+# direct, LDS-staged (negative), x-terms-staged (2000 + lanes) and
+# fused-sweep strip (1000 + lanes) forms
+CANDIDATES = (64, 32, 16, -64, -32, -16, 2064, 2032, 2016, 1064, 1032, 1016)
 
 
 def size_class(ctx, level):
+    """Key of a level's shape: nx x ny x modes, with a trailing "c" on the
+    coarsest level (whose solve takes other shapes: no strip form)."""
     nx, ny = ctx.grid.level_shape(level)
-    return f"{nx}x{ny}x{ctx.grid.nmodes}"
+    tail = "c" if level == ctx.grid.levels else ""
+    return f"{nx}x{ny}x{ctx.grid.nmodes}{tail}"
 
 
 def tune(ctx, sweeps: int = 6, levels=None, precisions=(0, 1), verbose=False):

@@ -38,6 +38,19 @@ int line_rows_for(int G, int ny);
 // Fused-sweep strip smoother: line shape kStripShape + G selects it (G lanes
 // per line); halo of kStripHalo columns each side of a strip.
 constexpr int kStripShape = 1000;
+// Colour-phase smoother with only the x terms staged in LDS: line shape
+// kXStagedShape + G (G lanes per line).
+constexpr int kXStagedShape = 2000;
+
+// Line-shape codes: G, -G, kStripShape + G or kXStagedShape + G.
+inline bool shape_is_strip(int code) { return code > kStripShape && code < kXStagedShape; }
+inline int shape_lanes(int code)
+{
+    return code > kXStagedShape ? code - kXStagedShape
+           : code > kStripShape ? code - kStripShape
+           : code < 0           ? -code
+                                : code;
+}
 constexpr int kStripHalo = 6;
 struct StripGeom {
     int width, nstrips;   // strip width in interior columns, number of strips

@@ -20,15 +20,27 @@ __device__ __forceinline__ void atomic_max_nonneg(double *addr, double v)
 // Defect by column marching: one wavefront owns kDefRows consecutive output
 // rows of one strip of columns of one mode.  Its 64 lanes hold rows
 // j0-2 .. j0+61 (two halo rows each side); it walks the strip in x keeping
-// the five columns i-2..i+2 of its rows in registers (one new column load per
-// step, issued two steps ahead) and takes the y-neighbours from lane shuffles, so every value of v is
-// loaded about once.  Row forms: Fortran row 2 (sp_poi_coef), interior rows
+// the columns i-2..i+2 of its rows in a ring of eight registers and takes the
+// y-neighbours from lane shuffles, so every value of v is loaded about once.
+// The march goes in blocks of B columns: at a block's start the
+// wavefront issues the loads of the next block's v and S columns, then
+// computes and stores the block from registers, and only at its end takes
+// the loaded columns over.  (The output stores share the load counter, so
+// any wait for a load also waits for every store and load before it; one
+// such wait per block instead of per column keeps a block's loads in flight
+// behind its arithmetic.)  B trades that against registers: 8 for FP32, 2
+// for FP64, whose arithmetic needs more wavefronts in flight to keep the
+// vector units busy.  Loads go through buffer resources, zero where the
+// stencil has no point.  Row forms: Fortran row 2 (sp_poi_coef), interior rows
 // (cp_poi_coef), row ny-1 (pp_poi_coef) and row ny (lp_poi_coef, with the
 // top boundary term); x part: the border-column one-sided stencils on
 // columns 2 and nx-1, the centred fourth-order stencil elsewhere (grouped
 // on the two top rows).  The divisions by 12 dx^2 fac_y and fac_y are
 // multiplications by their reciprocals (last-bit rounding differs).
 constexpr int kDefRows = 60;
+template <typename T>
+constexpr int kDefBlock = sizeof(T) == 4 ? 8 : 2;
+
 
 __host__ __device__ inline int defect_strip(int nx) { return nx >= 512 ? 32 : 16; }
 
@@ -50,9 +62,16 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
     const bool jin = j >= 0 && j < ny;
     const bool own = lane >= 2 && lane < 2 + kDefRows && j >= 1 && j < ny;
     const int ia = 1 + sc * strip, ib = min(ia + strip, nx - 1);
-    const cx<T> *vk = V + static_cast<size_t>(k) * L.plane;
+    const size_t pbytes = L.plane * sizeof(cx<T>);
+    const __amdgpu_buffer_rsrc_t vr = buf_rsrc(V + static_cast<size_t>(k) * L.plane, pbytes);
+    const __amdgpu_buffer_rsrc_t sr = buf_rsrc(S + static_cast<size_t>(k) * L.plane, pbytes);
     auto vload = [&](int i) -> cx<T> {
-        return (jin && i >= 0 && i < nx) ? vk[static_cast<size_t>(i) * ld + j] : cx<T>{T(0), T(0)};
+        const bool ok = jin && i >= 0 && i < nx;
+        return buf_load(vr, ok ? static_cast<unsigned>((i * ld + j) * sizeof(cx<T>)) : kBufOff, T(0));
+    };
+    auto sload = [&](int i) -> cx<T> {
+        const bool ok = own && i < nx;
+        return buf_load(sr, ok ? static_cast<unsigned>((i * ld + j) * sizeof(cx<T>)) : kBufOff, T(0));
     };
     const T kk = static_cast<T>(k);
     const T k2b2 = -kk * kk * L.beta * L.beta;
@@ -63,68 +82,93 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
     const T iscx = T(1) / scx, ifac = T(1) / L.fac_y;
     T q = T(0);
     if (own) q = (j == 1) ? L.qdy2[0] : (j == ny - 2) ? L.qdy2[j - 1] : L.qdy2[j - 2];
-    cx<T> w0 = vload(ia - 2), w1 = vload(ia - 1), w2 = vload(ia), w3 = vload(ia + 1),
-          w4 = vload(ia + 2);
-    // two further columns in flight (prefetch distance 2)
-    cx<T> p0 = vload(ia + 3), p1 = vload(ia + 4);
-    const cx<T> *sk = S + static_cast<size_t>(k) * L.plane;
-    cx<T> s_cur = own ? sk[static_cast<size_t>(ia) * ld + j] : cx<T>{T(0), T(0)};
+    constexpr int B = kDefBlock<T>;
+    // win[t] = column i0 - 2 + t of the current block (i0 .. i0+B-1 computed),
+    // sq[t] = source column i0 + t; nb / ns: the next block's new columns
+    cx<T> win[B + 4], sq[B], nb[B], ns[B];
+#pragma unroll
+    for (int t = 0; t < B + 4; ++t) win[t] = vload(ia - 2 + t);
+#pragma unroll
+    for (int t = 0; t < B; ++t) sq[t] = sload(ia + t);
     double amax = 0.0;
-    for (int i = ia; i < ib; ++i) {
-        const cx<T> pn = vload(i + 5);
-        const cx<T> s_nxt = (own && i + 1 < ib) ? sk[static_cast<size_t>(i + 1) * ld + j]
-                                             : cx<T>{T(0), T(0)};
-        cx<T> xs, xo;   // x part at row j: shared (ungrouped) and own
-        if (i == 1) {
-            xs = T(10) * w1 - T(15) * w2 - T(4) * w3 + T(14) * w4 - T(6) * vload(4) + vload(5);
-            xo = xs;
-        } else if (i == nx - 2) {
-            xs = T(10) * w3 - T(15) * w2 - T(4) * w1 + T(14) * w0 - T(6) * vload(nx - 5)
-                 + vload(nx - 6);
-            xo = xs;
-        } else {
-            xs = T(-1) * w0 + T(16) * w1 - T(30) * w2 + T(16) * w3 - T(1) * w4;
-            xo = (j >= ny - 2) ? T(-1) * (w0 + w4) + T(16) * (w1 + w3) - T(30) * w2 : xs;
+    for (int i0 = ia; i0 < ib; i0 += B) {
+        if (i0 + B < ib) {
+#pragma unroll
+            for (int t = 0; t < B; ++t) {
+                nb[t] = vload(i0 + B + 2 + t);
+                ns[t] = sload(i0 + B + t);
+            }
         }
-        const cx<T> xm = shfl_up(xs, 1, 64), xp = shfl_down(xs, 1, 64);
-        const cx<T> vm1 = shfl_up(w2, 1, 64), vm2 = shfl_up(w2, 2, 64);
-        const cx<T> vp1 = shfl_down(w2, 1, 64), vp2 = shfl_down(w2, 2, 64);
-        if (own) {
-            cx<T> lv;
-            if (j == 1) {
-                const cx<T> b = L.sp[2] * xp + L.sp[1] * xs;
-                const cx<T> c = L.sp[3] * vm1 + L.sp[4] * w2 + L.sp[5] * vp1 + L.sp[6] * vp2;
-                const cx<T> d = k2b2 * (L.sp[1] * w2 + L.sp[2] * vp1);
-                lv = b * iscx + c * q + d * ifac;
-            } else if (j <= ny - 3) {
-                const cx<T> b = L.cp[0] * xm + L.cp[1] * xs + L.cp[2] * xp;
-                const cx<T> c = L.cp[3] * vm2 + L.cp[4] * vm1 + L.cp[5] * w2 + L.cp[6] * vp1
-                                + L.cp[7] * vp2;
-                const cx<T> d = k2b2 * (L.cp[0] * vm1 + L.cp[1] * w2 + L.cp[2] * vp1);
-                lv = b * iscx + c * q + d * ifac;
-            } else if (j == ny - 2) {
-                const cx<T> c = L.pp[3] * vp1 + L.pp[2] * w2 + L.pp[1] * vm1;
-                const cx<T> d = k2b2 * w2;
-                lv = xo * iscx + c * q + d * ifac;
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int i = i0 + u;
+            if (i >= ib) break;
+            const cx<T> w0 = win[u], w1 = win[u + 1], w2 = win[u + 2], w3 = win[u + 3],
+                        w4 = win[u + 4];
+            const cx<T> s_cur = sq[u];
+            // x part at row j, shared with the y-neighbours (ungrouped); the
+            // two top rows use their own grouped form (xo below)
+            cx<T> xs;
+            const bool border = i == 1 || i == nx - 2;
+            if (i == 1) {
+                xs = T(10) * w1 - T(15) * w2 - T(4) * w3 + T(14) * w4 - T(6) * vload(4) + vload(5);
+            } else if (i == nx - 2) {
+                xs = T(10) * w3 - T(15) * w2 - T(4) * w1 + T(14) * w0 - T(6) * vload(nx - 5)
+                     + vload(nx - 6);
             } else {
-                const cx<T> c = L.lp[2] * w2 + L.lp[3] * vm1 + L.lp[4] * vm2;
-                const cx<T> d = k2b2 * w2;
-                lv = xo * iscx + c * q + d * ifac - L.lp[1] * alpha2 * w2 * L.sqq_top;
+                xs = T(-1) * w0 + T(16) * w1 - T(30) * w2 + T(16) * w3 - T(1) * w4;
             }
-            const size_t o = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * ld + j;
-            const cx<T> r = s_cur + si * lv;
-            out[o] = cx<TO>{static_cast<TO>(r.re), static_cast<TO>(r.im)};
-            if (resmax) {
-                const double rr = static_cast<double>(r.re), ri = static_cast<double>(r.im);
-                amax = fmax(amax, sqrt(rr * rr + ri * ri));
+            auto own_x = [&]() {
+                return border ? xs : T(-1) * (w0 + w4) + T(16) * (w1 + w3) - T(30) * w2;
+            };
+            const cx<T> xm = shfl_up(xs, 1, 64), xp = shfl_down(xs, 1, 64);
+            const cx<T> vm1 = shfl_up(w2, 1, 64), vm2 = shfl_up(w2, 2, 64);
+            const cx<T> vp1 = shfl_down(w2, 1, 64), vp2 = shfl_down(w2, 2, 64);
+            if (own) {
+                cx<T> lv;
+                if (j == 1) {
+                    const cx<T> b = L.sp[2] * xp + L.sp[1] * xs;
+                    const cx<T> c = L.sp[3] * vm1 + L.sp[4] * w2 + L.sp[5] * vp1 + L.sp[6] * vp2;
+                    const cx<T> d = k2b2 * (L.sp[1] * w2 + L.sp[2] * vp1);
+                    lv = b * iscx + c * q + d * ifac;
+                } else if (j <= ny - 3) {
+                    const cx<T> b = L.cp[0] * xm + L.cp[1] * xs + L.cp[2] * xp;
+                    const cx<T> c = L.cp[3] * vm2 + L.cp[4] * vm1 + L.cp[5] * w2 + L.cp[6] * vp1
+                                    + L.cp[7] * vp2;
+                    const cx<T> d = k2b2 * (L.cp[0] * vm1 + L.cp[1] * w2 + L.cp[2] * vp1);
+                    lv = b * iscx + c * q + d * ifac;
+                } else if (j == ny - 2) {
+                    const cx<T> c = L.pp[3] * vp1 + L.pp[2] * w2 + L.pp[1] * vm1;
+                    const cx<T> d = k2b2 * w2;
+                    lv = own_x() * iscx + c * q + d * ifac;
+                } else {
+                    const cx<T> c = L.lp[2] * w2 + L.lp[3] * vm1 + L.lp[4] * vm2;
+                    const cx<T> d = k2b2 * w2;
+                    lv = own_x() * iscx + c * q + d * ifac - L.lp[1] * alpha2 * w2 * L.sqq_top;
+                }
+                const size_t o = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * ld + j;
+                const cx<T> r = s_cur + si * lv;
+                out[o] = cx<TO>{static_cast<TO>(r.re), static_cast<TO>(r.im)};
+                if (resmax) {
+                    // max of |r|^2; one square root per wavefront below
+                    // (sqrt is monotone and correctly rounded: same maximum)
+                    const double rr = static_cast<double>(r.re), ri = static_cast<double>(r.im);
+                    amax = fmax(amax, rr * rr + ri * ri);
+                }
             }
         }
-        w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = p0; p0 = p1; p1 = pn;
-        s_cur = s_nxt;
+        // slide: the block's last four columns lead the next block's window
+#pragma unroll
+        for (int t = 0; t < 4; ++t) win[t] = win[B + t];
+#pragma unroll
+        for (int t = 0; t < B; ++t) {
+            win[4 + t] = nb[t];
+            sq[t] = ns[t];
+        }
     }
     if (resmax) {
         for (int o = 32; o > 0; o >>= 1) amax = fmax(amax, __shfl_down(amax, o, 64));
-        if (lane == 0) atomic_max_nonneg<T>(resmax + k, amax);
+        if (lane == 0) atomic_max_nonneg<T>(resmax + k, sqrt(amax));
     }
 }
 
@@ -378,12 +422,14 @@ int line_rows_for(int G, int ny)
 
 // G > 0: lanes per line with direct (lane-contiguous) global access;
 // G < 0: |G| lanes per line with the LDS-staged, coalesced kernel;
-// G > kStripShape: fused-sweep strips with G - kStripShape lanes per line
+// kStripShape < G < kXStagedShape: fused-sweep strips with G - kStripShape
+// lanes per line; G > kXStagedShape: G - kXStagedShape lanes per line, only
+// the x terms staged in LDS
 template <typename T>
 int smooth_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
                   const int *active, int G, cx<T> *halo, hipStream_t st)
 {
-    if (G > kStripShape)
+    if (G > kStripShape && G < kXStagedShape)
         return strip_sweeps<T>(L, V, S, lu, sweeps, rf, active, G - kStripShape, halo, st);
     return phase_sweeps<T>(L, V, S, lu, sweeps, rf, active, G, st);
 }

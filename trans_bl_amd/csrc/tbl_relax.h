@@ -14,6 +14,7 @@
 // penta_par.f:367-404).
 #pragma once
 
+#include "tbl_buf.h"
 #include "tbl_kernels.h"
 
 namespace tbl {
@@ -51,6 +52,9 @@ __device__ __forceinline__ cx<T> xoff(const cx<T> *__restrict__ col, int ld, int
 // (forward multipliers), u1, u2, pivot (back substitution, as banbk5_poi
 // uses them) and -u1/p, -u2/p, 1/p (the scan's map composition).  Rows past
 // ny get the identity (zeros, unit pivot).
+template <typename T>
+constexpr bool kIsF32 = sizeof(T) == 4;
+
 template <typename T, int R>
 struct LineLU {
     T l1[R], l2[R], u1[R], u2[R], pv[R], m1[R], m2[R], rp[R];
@@ -73,6 +77,36 @@ __device__ __forceinline__ LineLU<T, R> load_line_lu(const T *__restrict__ lu, i
         f.m1[r] = ok ? lu[5 * ld + j] : T(0);
         f.m2[r] = ok ? lu[6 * ld + j] : T(0);
         f.rp[r] = ok ? lu[7 * ld + j] : T(1);
+    }
+    return f;
+}
+
+// Same factors through a buffer resource over the LU table: off0 = byte
+// offset of the line's table (field 0, row 0).  Rows past ny get the
+// identity; their loads may run past the table and then read zero.
+template <typename T, int R>
+__device__ __forceinline__ LineLU<T, R> load_line_lu_buf(__amdgpu_buffer_rsrc_t rs, unsigned off0,
+                                                        int ld, int j0, int ny)
+{
+    LineLU<T, R> f;
+    const unsigned fb = static_cast<unsigned>(ld) * sizeof(T);
+    const unsigned o = off0 + static_cast<unsigned>(j0) * sizeof(T);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const bool ok = j0 + r < ny;
+        const unsigned e = o + r * sizeof(T);
+        const T l1 = buf_scalar(rs, e, T(0)), l2 = buf_scalar(rs, e + fb, T(0));
+        const T u1 = buf_scalar(rs, e + 2 * fb, T(0)), u2 = buf_scalar(rs, e + 3 * fb, T(0));
+        const T pv = buf_scalar(rs, e + 4 * fb, T(0)), m1 = buf_scalar(rs, e + 5 * fb, T(0));
+        const T m2 = buf_scalar(rs, e + 6 * fb, T(0)), rp = buf_scalar(rs, e + 7 * fb, T(0));
+        f.l1[r] = ok ? l1 : T(0);
+        f.l2[r] = ok ? l2 : T(0);
+        f.u1[r] = ok ? u1 : T(0);
+        f.u2[r] = ok ? u2 : T(0);
+        f.pv[r] = ok ? pv : T(1);
+        f.m1[r] = ok ? m1 : T(0);
+        f.m2[r] = ok ? m2 : T(0);
+        f.rp[r] = ok ? rp : T(1);
     }
     return f;
 }
@@ -191,7 +225,11 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const LineLU<T
 #pragma unroll
         for (int r = R - 1; r >= 0; --r) {
             const T pj = f.pv[r], w1 = f.u1[r], w2 = f.u2[r];
-            const cx<T> xx = ((y[r] - w1 * a) - w2 * b) / pj;
+            // FP64 divides by the pivot as banbk5_poi does; the FP32
+            // smoother (a correction solve under FP64 defects) multiplies
+            // by the reciprocal instead of a ~10-instruction division
+            const cx<T> t = (y[r] - w1 * a) - w2 * b;
+            const cx<T> xx = kIsF32<T> ? t * f.rp[r] : t / pj;
             b = a; a = xx;
             y[r] = xx;
         }

@@ -382,10 +382,9 @@ int smooth_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu
         return TBL_OK;
     }
     if (coarse_lds_bytes<T>(L.nx, L.ny, false) > 160 * 1024)
-        return smooth_sweeps<T>(L, V, S, lu, sweeps, rf, active, G > kStripShape ? G - kStripShape : G,
-                                nullptr, st);
-    if (G > kStripShape) G -= kStripShape;
-    if (G < 0) G = -G;
+        return smooth_sweeps<T>(L, V, S, lu, sweeps, rf, active,
+                                shape_is_strip(G) ? G - kStripShape : G, nullptr, st);
+    G = shape_lanes(G);
     const int R = line_rows_for(G, L.ny);
     if (R < 0) return TBL_ERR_GRID;
     bool done = false;

@@ -27,10 +27,12 @@ constexpr int kStagedLines = 4;   // lines per workgroup of the staged smoother
 
 // Same relaxation with coalesced global traffic: the group first reads its
 // line lane-strided (row t, t+G, ...: contiguous across the wavefront),
-// forming the off-diagonal x terms, v and S in LDS; the scan then works on
-// lane-contiguous segments from LDS; the relaxed v goes back through LDS and
-// is stored lane-strided.  Arithmetic identical to smooth_phase_kernel.
-template <typename T, int G, int R>
+// forming the off-diagonal x terms in LDS -- with FULL also v and S, else v
+// and S are read and v written lane-contiguous from global memory (a third
+// of the LDS, so more lines in flight per CU); the scan then works on
+// lane-contiguous segments; with FULL the relaxed v goes back through LDS
+// and is stored lane-strided.  Arithmetic identical to smooth_phase_kernel.
+template <typename T, int G, int R, bool FULL>
 __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T> *V,
                                                             const cx<T> *S, const T *lu,
                                                             int phase, T rf, const int *active)
@@ -39,13 +41,21 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
     const int LDW = L.ld;                                // rows held per line
     cx<T> *base_s = reinterpret_cast<cx<T> *>(smem_raw);
     const int grp = threadIdx.x / G, lane = threadIdx.x % G;
-    cx<T> *Xs = base_s + static_cast<size_t>(grp) * 3 * LDW;
+    cx<T> *Xs = base_s + static_cast<size_t>(grp) * (FULL ? 3 : 1) * LDW;
     cx<T> *Vs = Xs + LDW, *Ss = Vs + LDW;
     const int nlines = phase_lines(phase, L.nx);
     const int gid = blockIdx.x * kStagedLines + grp;
     const int k = gid / nlines;
     const bool live = k < L.K && !(active && !active[k]);
     const int ny = L.ny, ld = L.ld;
+    // buffer resources over the (at most two) modes of this workgroup's
+    // lines: 32-bit offsets, the row steps of the loads in the instruction
+    const int g0 = blockIdx.x * kStagedLines;
+    const int kf = g0 / nlines;
+    const int kl = min((g0 + kStagedLines - 1) / nlines, L.K - 1);
+    const size_t span = static_cast<size_t>(kl - kf + 1) * L.plane * sizeof(cx<T>);
+    const __amdgpu_buffer_rsrc_t vr = buf_rsrc(V + static_cast<size_t>(kf) * L.plane, span);
+    const __amdgpu_buffer_rsrc_t sr = buf_rsrc(S + static_cast<size_t>(kf) * L.plane, span);
     int side = 0;
     size_t base = 0;
     if (live) {
@@ -53,33 +63,80 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
         const int i = phase_column(phase, c, L.nx);
         side = (phase == 0) ? (c == 0 ? -1 : 1) : 0;
         base = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * ld;
-        const cx<T> *col = V + base;
-        // G*R >= ny: a fixed trip count lets every load be issued up front
+        // Every load of the line is issued before any is used: the rows'
+        // x-neighbour columns (border: the one-sided stencil's five columns
+        // on the inner side; interior: i-2, i-1, i+1, i+2), v and S; rows
+        // past ny read padding or zero and are not used.  xoff's arithmetic
+        // follows.
+        const int sg = -side;
+        const int o0 = side ? -sg * ld : -2 * ld, o1 = side ? sg * ld : -ld;
+        const int o2 = side ? 2 * sg * ld : ld, o3 = side ? 3 * sg * ld : 2 * ld;
+        const int o4 = side ? 4 * sg * ld : 2 * ld;
+        constexpr unsigned ES = sizeof(cx<T>);
+        const unsigned lo = static_cast<unsigned>(((k - kf) * L.plane + static_cast<size_t>(i) * ld + lane) * ES);
+        const unsigned b0 = lo + o0 * ES, b1 = lo + o1 * ES, b2 = lo + o2 * ES, b3 = lo + o3 * ES,
+                       b4 = lo + o4 * ES;
+        cx<T> a0[R], a1[R], a2[R], a3[R], a4[R], vv[R], ss[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const unsigned rs = r * G * ES;
+            a0[r] = buf_load(vr, b0 + rs, T(0));
+            a1[r] = buf_load(vr, b1 + rs, T(0));
+            a2[r] = buf_load(vr, b2 + rs, T(0));
+            a3[r] = buf_load(vr, b3 + rs, T(0));
+            a4[r] = buf_load(vr, b4 + rs, T(0));
+            if (FULL) {
+                vv[r] = buf_load(vr, lo + rs, T(0));
+                ss[r] = buf_load(sr, lo + rs, T(0));
+            }
+        }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int j = lane + r * G;
             if (j < ny) {
-                Xs[j] = xoff<T>(col, ld, j, side, side == 0 && j >= ny - 2);
-                Vs[j] = col[j];
-                Ss[j] = S[base + j];
+                cx<T> x;
+                if (side)
+                    x = T(10) * a0[r] - T(4) * a1[r] + T(14) * a2[r] - T(6) * a3[r] + a4[r];
+                else if (j >= ny - 2)
+                    x = -(a3[r] + a0[r]) + T(16) * (a2[r] + a1[r]);
+                else
+                    x = T(-1) * a0[r] + T(16) * a1[r] + T(16) * a2[r] - T(1) * a3[r];
+                Xs[j] = x;
+                if (FULL) {
+                    Vs[j] = vv[r];
+                    Ss[j] = ss[r];
+                }
             }
         }
     }
     __syncthreads();
     if (live) {
-        const T *lut = lu + (static_cast<size_t>(k) * 2 + (side != 0)) * kLuFields * ld;
-        const LineLU<T, R> f = load_line_lu<T, R>(lut, ld, lane * R, ny);
-        relax_core<T, G, R>(L, f, rf, lane, [&](int j) { return Xs[j]; },
-                            [&](int j) { return Vs[j]; }, [&](int j, int) { return Ss[j]; },
-                            [&](int j, cx<T> v) { Vs[j] = v; });
+        const __amdgpu_buffer_rsrc_t lr =
+            buf_rsrc(lu, static_cast<size_t>(L.K) * 2 * kLuFields * ld * sizeof(T));
+        const unsigned lo = static_cast<unsigned>((static_cast<size_t>(k) * 2 + (side != 0)) *
+                                                  kLuFields * ld * sizeof(T));
+        const LineLU<T, R> f = load_line_lu_buf<T, R>(lr, lo, ld, lane * R, ny);
+        if (FULL) {
+            relax_core<T, G, R>(L, f, rf, lane, [&](int j) { return Xs[j]; },
+                                [&](int j) { return Vs[j]; }, [&](int j, int) { return Ss[j]; },
+                                [&](int j, cx<T> v) { Vs[j] = v; });
+        } else {
+            cx<T> *col = V + base;
+            const cx<T> *scol = S + base;
+            relax_core<T, G, R>(L, f, rf, lane, [&](int j) { return Xs[j]; },
+                                [&](int j) { return col[j]; }, [&](int j, int) { return scol[j]; },
+                                [&](int j, cx<T> v) { col[j] = v; });
+        }
     }
-    __syncthreads();
-    if (live) {
-        cx<T> *col = V + base;
+    if (FULL) {
+        __syncthreads();
+        if (live) {
+            cx<T> *col = V + base;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int j = lane + r * G;
-            if (j >= 1 && j < ny) col[j] = Vs[j];
+            for (int r = 0; r < R; ++r) {
+                const int j = lane + r * G;
+                if (j >= 1 && j < ny) col[j] = Vs[j];
+            }
         }
     }
 }
@@ -95,38 +152,47 @@ static void launch_phase(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const 
                        lu, phase, rf, active);
 }
 
-template <typename T, int G, int R>
+template <bool FULL>
+static size_t staged_lds(int ld, size_t esize)
+{
+    return static_cast<size_t>(kStagedLines) * (FULL ? 3 : 1) * ld * esize;
+}
+
+template <typename T, int G, int R, bool FULL>
 static void launch_staged(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int phase,
                           T rf, const int *active, hipStream_t st)
 {
     const size_t lines = static_cast<size_t>(phase_lines(phase, L.nx)) * L.K;
     if (!lines) return;
-    const size_t shm = static_cast<size_t>(kStagedLines) * 3 * L.ld * sizeof(cx<T>);
-    hipLaunchKernelGGL((smooth_staged_kernel<T, G, R>),
+    hipLaunchKernelGGL((smooth_staged_kernel<T, G, R, FULL>),
                        dim3(static_cast<unsigned>((lines + kStagedLines - 1) / kStagedLines)),
-                       dim3(kStagedLines * G), shm, st, L, V, S, lu, phase, rf, active);
+                       dim3(kStagedLines * G), staged_lds<FULL>(L.ld, sizeof(cx<T>)), st, L, V, S,
+                       lu, phase, rf, active);
 }
 
 // One colour phase per launch.  G > 0: lanes per line with direct
-// (lane-contiguous) global access; G < 0: |G| lanes per line, LDS-staged.
+// (lane-contiguous) global access; G < 0: |G| lanes per line, x terms, v
+// and S staged in LDS; G > kXStagedShape: G - kXStagedShape lanes per line,
+// only the x terms staged.
 template <typename T>
 int phase_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
                  const int *active, int G, hipStream_t st)
 {
-    const bool staged = G < 0;
-    if (staged) G = -G;
+    const int form = G < 0 ? 1 : G > kXStagedShape ? 2 : 0;
+    G = form == 1 ? -G : form == 2 ? G - kXStagedShape : G;
     const int R = line_rows_for(G, L.ny);
     if (R < 0) return TBL_ERR_GRID;
-    if (staged && static_cast<size_t>(kStagedLines) * 3 * L.ld * sizeof(cx<T>) > 160 * 1024)
-        return TBL_ERR_GRID;
+    if (form == 1 && staged_lds<true>(L.ld, sizeof(cx<T>)) > 160 * 1024) return TBL_ERR_GRID;
+    if (form == 2 && staged_lds<false>(L.ld, sizeof(cx<T>)) > 160 * 1024) return TBL_ERR_GRID;
     for (int it = 0; it < sweeps; ++it)
         for (int ph = 0; ph < 4; ++ph) {
             bool done = false;
-#define X(GG, RR)                                                             \
-    if (!done && G == GG && R == RR) {                                        \
-        if (staged) launch_staged<T, GG, RR>(L, V, S, lu, ph, rf, active, st); \
-        else launch_phase<T, GG, RR>(L, V, S, lu, ph, rf, active, st);        \
-        done = true;                                                          \
+#define X(GG, RR)                                                                       \
+    if (!done && G == GG && R == RR) {                                                  \
+        if (form == 1) launch_staged<T, GG, RR, true>(L, V, S, lu, ph, rf, active, st);  \
+        else if (form == 2) launch_staged<T, GG, RR, false>(L, V, S, lu, ph, rf, active, st); \
+        else launch_phase<T, GG, RR>(L, V, S, lu, ph, rf, active, st);                  \
+        done = true;                                                                    \
     }
             TBL_LINE_SHAPES(X)
 #undef X
@@ -138,11 +204,14 @@ int phase_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, i
 int phase_kernel_attributes()
 {
     hipError_t e = hipSuccess;
-#define X(GG, RR)                                                                          \
-    for (const void *f : {reinterpret_cast<const void *>(&smooth_staged_kernel<double, GG, RR>), \
-                          reinterpret_cast<const void *>(&smooth_staged_kernel<float, GG, RR>)}) { \
-        hipError_t r = allow_full_lds(f);                                                  \
-        if (r != hipSuccess) e = r;                                                        \
+#define X(GG, RR)                                                                                \
+    for (const void *f :                                                                         \
+         {reinterpret_cast<const void *>(&smooth_staged_kernel<double, GG, RR, true>),           \
+          reinterpret_cast<const void *>(&smooth_staged_kernel<float, GG, RR, true>),            \
+          reinterpret_cast<const void *>(&smooth_staged_kernel<double, GG, RR, false>),          \
+          reinterpret_cast<const void *>(&smooth_staged_kernel<float, GG, RR, false>)}) {        \
+        hipError_t r = allow_full_lds(f);                                                        \
+        if (r != hipSuccess) e = r;                                                              \
     }
     TBL_LINE_SHAPES(X)
 #undef X

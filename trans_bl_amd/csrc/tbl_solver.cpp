@@ -299,17 +299,22 @@ struct Sched {
     int maxc, nu1, nu2, nc;
 };
 
-// Coarsest-level sweeps of the FP32 correction cycle when the caller leaves
-// ncoarse at 0.  The FP64 cycle keeps the reference's 60 (20*itera,
-// poisson_par.f:45); the correction cycle only needs an approximate coarse
-// correction: 5 to 60 sweeps gave the same V-cycle counts on the 1025x193x64
-// and 385x97x8 wavepacket workloads, and the converged v does not depend on
-// it (it is the fixed point of the fine-level residual).
-constexpr int kMixedCoarseSweeps = 10;
+// Schedule of the FP32 correction cycle when the caller leaves nu1 and
+// ncoarse at 0.  The FP64 cycle keeps the reference's 3 pre-smoothing and 60
+// coarsest sweeps (itera and 20*itera, poisson_par.f:24, 45).  The
+// correction cycle only needs an approximate correction, and the converged v
+// does not depend on it (it is the fixed point of the fine-level residual):
+// on the 1025x193x64, 385x97x8 and 2049x257x128 wavepacket workloads 2
+// pre-sweeps and 5 coarsest sweeps took 0.5-0.7 more V-cycles than 3 and 10
+// but 7-26 % less time to the same 1e-9 residual; fewer coarsest sweeps
+// started to add cycles.
+constexpr int kMixedPreSweeps = 2;
+constexpr int kMixedCoarseSweeps = 5;
 
 Sched sched_of(const tbl_solve_opts *o)
 {
-    Sched s{11, 3, 1, (o && o->precision == 1) ? kMixedCoarseSweeps : 60};
+    const bool mixed = o && o->precision == 1;
+    Sched s{11, mixed ? kMixedPreSweeps : 3, 1, mixed ? kMixedCoarseSweeps : 60};
     if (o) {
         if (o->max_cycles > 0) s.maxc = o->max_cycles;
         if (o->nu1 > 0) s.nu1 = o->nu1;
@@ -606,8 +611,8 @@ extern "C" int tbl_residual_dev(tbl_ctx *c, const double *d_wx, const double *d_
 // smoother, 1: FP32 smoother.
 extern "C" int tbl_set_line_shape(tbl_ctx *c, int prec, int level, int lanes)
 {
-    const bool strip = lanes > kStripShape;
-    const int a = strip ? lanes - kStripShape : lanes < 0 ? -lanes : lanes;
+    const bool strip = shape_is_strip(lanes);
+    const int a = shape_lanes(lanes);
     if (!c || level < 1 || level > c->L || (a != 16 && a != 32 && a != 64))
         return TBL_ERR_ARG;
     const Level &lv = c->lv[level - 1];
@@ -652,7 +657,7 @@ extern "C" int tbl_time_smoother(tbl_ctx *c, int prec, int level, int sweeps, do
     *ms = t;
     if (launches) {
         if (level == c->L) *launches = 1;
-        else if (((prec ? c->G32 : c->G64)[l]) > kStripShape) *launches = 2 * sweeps;
+        else if (shape_is_strip((prec ? c->G32 : c->G64)[l])) *launches = 2 * sweeps;
         else *launches = 4 * sweeps;
     }
     return TBL_OK;

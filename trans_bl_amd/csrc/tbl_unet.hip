@@ -5,6 +5,7 @@
 // (v_mfma_f32_32x32x8_bf16: bf16 operands, fp32 accumulation); the
 // backward pass is two more GEMMs (weights: col^T x dZ, data: dZ x W^T)
 // and a col2im gather.  Weights keep an fp32 master copy for Adam.
+#include "tbl_buf.h"
 #include "tbl_nn.h"
 
 namespace tbl {
@@ -217,6 +218,153 @@ __global__ void __launch_bounds__(256) conv3x3_kernel(long NP, int H, int W, int
     }
 }
 
+// Same convolution with the input staged in LDS, on gfx950's K=16 matrix
+// instruction (v_mfma_f32_32x32x16_bf16): a workgroup owns PX consecutive
+// flattened output pixels (4 wavefronts x MT 32-pixel tiles) and CT = 32*NT
+// output channels.  Per 32-channel chunk it holds the packed weights and the
+// PX + 2W + 2 input pixels its 3x3 taps can reach in LDS (rows of CKQ bf16,
+// 16-byte aligned, so every A and B fragment is one 16-byte LDS read).  The
+// next chunk's global loads (NI 8-byte pieces of input and the weights per
+// thread) are issued before the current chunk's arithmetic and written to
+// LDS after it.
+constexpr int CKQ = CK + 8;     // LDS row of the K=16 kernel (80 bytes)
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <int MT, int NT, int NI>
+__global__ void __launch_bounds__(256) conv3x3s_kernel(long NP, int H, int W, int cin, int cinp,
+                                                       int cout, const bf16 *__restrict__ X,
+                                                       const bf16 *__restrict__ wp,
+                                                       const float *__restrict__ bias,
+                                                       bf16 *__restrict__ Y, int relu)
+{
+    constexpr int CT = 32 * NT, PX = 4 * 32 * MT, WPT = 9 * CT * (CK / 4) / 256;
+    static_assert(9 * CT * (CK / 4) % 256 == 0, "weight pieces per thread");
+    __shared__ __attribute__((aligned(16))) bf16 Ws[9][CT][CKQ];
+    extern __shared__ __attribute__((aligned(16))) unsigned char conv_smem[];
+    bf16 (*Xs)[CKQ] = reinterpret_cast<bf16 (*)[CKQ]>(conv_smem);
+    const int NQ = PX + 2 * W + 2;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const long p0 = static_cast<long>(blockIdx.x) * PX;
+    const long q0 = p0 - W - 1;                   // global pixel of Xs[0]
+    const int o0 = blockIdx.y * CT;
+    // this lane's A-operand pixels (row of each 32x16 fragment) and tap masks
+    int lq[MT];
+    unsigned tapok[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+        const int lp = wave * 32 * MT + m * 32 + (lane & 31);
+        const long pa = p0 + lp;
+        const int wa = static_cast<int>(pa % W), ha = static_cast<int>((pa / W) % H);
+        unsigned ok = 0;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int hh = ha + t / 3 - 1, ww = wa + t % 3 - 1;
+            if (pa < NP && hh >= 0 && hh < H && ww >= 0 && ww < W) ok |= 1u << t;
+        }
+        lq[m] = lp + W + 1;
+        tapok[m] = ok;
+    }
+    float16v acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[m][n][i] = 0.f;
+    // buffer resources: the weights, and the activations from the tile's
+    // first reachable pixel; pieces outside the image or past cin read zero
+    const long qb = q0 > 0 ? q0 : 0;
+    const __amdgpu_buffer_rsrc_t wres =
+        buf_rsrc_uniform(wp, static_cast<size_t>(9) * cout * cinp * sizeof(bf16));
+    const __amdgpu_buffer_rsrc_t xres =
+        buf_rsrc_uniform(X + qb * cin, static_cast<size_t>(NP - qb) * cin * sizeof(bf16));
+    short4v wr[WPT], xr[NI];
+    auto fetch = [&](int c0) {
+#pragma unroll
+        for (int s = 0; s < WPT; ++s) {
+            const int e = tid + 256 * s;
+            const int q = e % (CK / 4), o = (e / (CK / 4)) % CT, t = e / ((CK / 4) * CT);
+            const unsigned off = static_cast<unsigned>(((t * cout + o0 + o) * cinp + c0 + 4 * q) * 2);
+            wr[s] = __builtin_bit_cast(short4v, __builtin_amdgcn_raw_buffer_load_b64(wres, off, 0, 0));
+        }
+#pragma unroll
+        for (int s = 0; s < NI; ++s) {
+            const int e = tid + 256 * s;
+            const int q = e / (CK / 4), g = e % (CK / 4);
+            const long gp = q0 + q;
+            // cin is a multiple of 4: a 4-channel group is all in range or none
+            const bool ok = q < NQ && gp >= 0 && c0 + 4 * g < cin;
+            const unsigned off =
+                ok ? static_cast<unsigned>(((gp - qb) * cin + c0 + 4 * g) * 2) : kBufOff;
+            xr[s] = __builtin_bit_cast(short4v, __builtin_amdgcn_raw_buffer_load_b64(xres, off, 0, 0));
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int s = 0; s < WPT; ++s) {
+            const int e = tid + 256 * s;
+            const int q = e % (CK / 4), o = (e / (CK / 4)) % CT, t = e / ((CK / 4) * CT);
+            *reinterpret_cast<short4v *>(&Ws[t][o][4 * q]) = wr[s];
+        }
+#pragma unroll
+        for (int s = 0; s < NI; ++s) {
+            const int e = tid + 256 * s;
+            const int q = e / (CK / 4), g = e % (CK / 4);
+            if (q < NQ) *reinterpret_cast<short4v *>(&Xs[q][4 * g]) = xr[s];
+        }
+    };
+    fetch(0);
+    stash();
+    __syncthreads();
+    for (int c0 = 0; c0 < cinp; c0 += CK) {
+        const bool more = c0 + CK < cinp;
+        if (more) fetch(c0 + CK);
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int sh = (t / 3 - 1) * W + (t % 3 - 1);
+#pragma unroll
+            for (int kk = 0; kk < CK; kk += 16) {
+                const int kb = kk + (lane >> 5) * 8;
+                bf16x8 b[NT];
+#pragma unroll
+                for (int n = 0; n < NT; ++n)
+                    b[n] = *reinterpret_cast<const bf16x8 *>(&Ws[t][n * 32 + (lane & 31)][kb]);
+#pragma unroll
+                for (int m = 0; m < MT; ++m) {
+                    bf16x8 a = {};
+                    if (tapok[m] >> t & 1u) a = *reinterpret_cast<const bf16x8 *>(&Xs[lq[m] + sh][kb]);
+#pragma unroll
+                    for (int n = 0; n < NT; ++n)
+                        acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[n], acc[m][n], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+        if (more) {
+            stash();
+            __syncthreads();
+        }
+    }
+    // D layout: lane holds output channel (lane & 31); register i holds pixel
+    // 8*(i/4) + 4*(lane/32) + (i%4) of the 32-pixel tile
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+        const int o = o0 + n * 32 + (lane & 31);
+        const float bo = bias[o];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const long p = p0 + wave * 32 * MT + m * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                if (p < NP) {
+                    float v = acc[m][n][i] + bo;
+                    if (relu) v = fmaxf(v, 0.f);
+                    Y[p * cout + o] = f2bf(v);
+                }
+            }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // element-wise and layout kernels
 // ---------------------------------------------------------------------------
@@ -330,21 +478,24 @@ __global__ void pool_back_kernel(int Nn, int H, int W, int Cc, const float *dY, 
 }
 
 // Y[:, :, :, 0:C1] = up2(A) (nearest), Y[:, :, :, C1:C1+C2] = Bskip
+// four channels (8 bytes) per thread; C1 and C2 are multiples of 4 and the
+// pixel count fits 32 bits
 __global__ void up_concat_kernel(int Nn, int H, int W, int C1, int C2, const bf16 *A, const bf16 *Bk,
                                  bf16 *Y)
 {
-    const int Ct = C1 + C2;
-    const long total = static_cast<long>(Nn) * H * W * Ct;
-    const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+    const int Ct = C1 + C2, G = Ct / 4;
+    const unsigned total = static_cast<unsigned>(Nn) * H * W * G;
+    const unsigned e = blockIdx.x * 256u + threadIdx.x;
     if (e >= total) return;
-    const int c = static_cast<int>(e % Ct);
-    long p = e / Ct;
+    const int c = static_cast<int>(e % G) * 4;
+    const unsigned p = e / G;
     const int w = static_cast<int>(p % W), h = static_cast<int>((p / W) % H);
-    const int n = static_cast<int>(p / (static_cast<long>(W) * H));
-    if (c < C1)
-        Y[e] = A[((static_cast<long>(n) * (H / 2) + h / 2) * (W / 2) + w / 2) * C1 + c];
-    else
-        Y[e] = Bk[p * C2 + (c - C1)];
+    const int n = static_cast<int>(p / (static_cast<unsigned>(W) * H));
+    const bf16 *src = (c < C1)
+        ? A + (static_cast<size_t>((n * (H / 2) + h / 2) * (W / 2) + w / 2) * C1 + c)
+        : Bk + (static_cast<size_t>(p) * C2 + (c - C1));
+    *reinterpret_cast<short4v *>(Y + static_cast<size_t>(p) * Ct + c) =
+        *reinterpret_cast<const short4v *>(src);
 }
 
 // split the gradient of up_concat: dA (low res, sum of 4) and dB (skip)
@@ -463,7 +614,7 @@ void pool_back(int n, int h, int w, int c, const float *dy, float *dx, int acc, 
 void up_concat(int n, int h, int w, int c1, int c2, const bf16 *a, const bf16 *b, bf16 *y,
                hipStream_t st)
 {
-    const long t = static_cast<long>(n) * h * w * (c1 + c2);
+    const long t = static_cast<long>(n) * h * w * ((c1 + c2) / 4);
     hipLaunchKernelGGL(up_concat_kernel, dim3(nb(t)), dim3(256), 0, st, n, h, w, c1, c2, a, b, y);
 }
 void up_concat_back(int n, int h, int w, int c1, int c2, const float *dy, float *da, float *db,
@@ -495,6 +646,18 @@ void zero(long n, float *a, hipStream_t st)
     if (n > 0) hipLaunchKernelGGL(zero_kernel, dim3(nb(n)), dim3(256), 0, st, n, a);
 }
 int conv3x3_cinp(int cin) { return (cin + CK - 1) / CK * CK; }
+// conv3x3s_kernel: dynamic LDS of the input stage for PX pixels of width w
+// (the weight stage, at most 9 * 64 * CKQ bf16 = 46 KB, is static), and the
+// 8-byte input pieces per thread it holds in registers
+static size_t conv3x3s_lds(int px, int w) { return static_cast<size_t>(px + 2 * w + 2) * CKQ * sizeof(bf16); }
+constexpr size_t kConvStagedLds = 64 * 1024;
+// input pieces per thread of the two tilings (reach of widths up to 143)
+constexpr int kConvPieces128 = 13, kConvPieces256 = 17;
+static bool conv3x3s_fits(int px, int w, int pieces)
+{
+    return conv3x3s_lds(px, w) <= kConvStagedLds &&
+           static_cast<long>(px + 2 * w + 2) * (CK / 4) <= 256L * pieces;
+}
 void pack_conv3x3(int cin, int cout, const bf16 *w, bf16 *wp, hipStream_t st)
 {
     const int cinp = conv3x3_cinp(cin);
@@ -506,6 +669,25 @@ void conv3x3(int n, int h, int w, int cin, int cout, const bf16 *x, const bf16 *
 {
     const int cinp = conv3x3_cinp(cin);
     const long np = static_cast<long>(n) * h * w;
+    // dynamic LDS above the default limit is opted into once per process
+    static const bool staged_ok =
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&conv3x3s_kernel<1, 2, kConvPieces128>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kConvStagedLds) == hipSuccess &&
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&conv3x3s_kernel<2, 1, kConvPieces256>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kConvStagedLds) == hipSuccess;
+    // LDS-staged form when the reach of a tile fits beside the weights
+    if (staged_ok && cout % 64 == 0 && conv3x3s_fits(128, w, kConvPieces128)) {
+        dim3 grid(static_cast<unsigned>((np + 127) / 128), cout / 64);
+        hipLaunchKernelGGL((conv3x3s_kernel<1, 2, kConvPieces128>), grid, dim3(256),
+                           conv3x3s_lds(128, w), st, np, h, w, cin, cinp, cout, x, wp, bias, y, relu);
+        return;
+    }
+    if (staged_ok && cout % 32 == 0 && conv3x3s_fits(256, w, kConvPieces256)) {
+        dim3 grid(static_cast<unsigned>((np + 255) / 256), cout / 32);
+        hipLaunchKernelGGL((conv3x3s_kernel<2, 1, kConvPieces256>), grid, dim3(256),
+                           conv3x3s_lds(256, w), st, np, h, w, cin, cinp, cout, x, wp, bias, y, relu);
+        return;
+    }
     if (cout % 64 == 0) {
         dim3 grid(static_cast<unsigned>((np + 63) / 64), cout / 64);
         hipLaunchKernelGGL(conv3x3_kernel<64>, grid, dim3(256), 0, st, np, h, w, cin, cinp, cout, x,
