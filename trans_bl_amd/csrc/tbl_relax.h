@@ -111,6 +111,122 @@ __device__ __forceinline__ LineLU<T, R> load_line_lu_buf(__amdgpu_buffer_rsrc_t 
     return f;
 }
 
+// ---------------------------------------------------------------------------
+// the partitioned scan across the G lanes of a line
+// ---------------------------------------------------------------------------
+// Affine map of a run of rows: state' = P state + c (P 2x2 real, c two
+// complex values).
+template <typename T>
+struct Aff {
+    T p00, p01, p10, p11;
+    cx<T> c0, c1;
+};
+template <typename T>
+__device__ __forceinline__ Aff<T> aff_identity()
+{
+    return {T(1), T(0), T(0), T(1), {T(0), T(0)}, {T(0), T(0)}};
+}
+// a <- a after q (q's run feeds a's): c = P_a c_q + c_a, P = P_a P_q.  With
+// q the identity the result is a again, bit for bit.
+template <typename T>
+__device__ __forceinline__ void aff_fold(Aff<T> &a, const Aff<T> &q)
+{
+    const cx<T> nc0 = a.p00 * q.c0 + a.p01 * q.c1 + a.c0;
+    const cx<T> nc1 = a.p10 * q.c0 + a.p11 * q.c1 + a.c1;
+    a.c0 = nc0; a.c1 = nc1;
+    const T m00 = a.p00 * q.p00 + a.p01 * q.p10, m01 = a.p00 * q.p01 + a.p01 * q.p11;
+    const T m10 = a.p10 * q.p00 + a.p11 * q.p10, m11 = a.p10 * q.p01 + a.p11 * q.p11;
+    a.p00 = m00; a.p01 = m01; a.p10 = m10; a.p11 = m11;
+}
+
+// DPP lane moves (a VALU operand modifier: no LDS round trip).  CTRL: row
+// shift right by d (from lane - d of the same 16-lane row) = 0x110 + d, row
+// shift left (from lane + d) = 0x100 + d, wavefront shift right / left by
+// one = 0x138 / 0x130.  Lanes without a source keep `id`.
+template <int CTRL>
+__device__ __forceinline__ float dpp(float id, float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, id),
+                                                                 __builtin_bit_cast(int, x), CTRL,
+                                                                 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double id, double x)
+{
+    const unsigned long long i = __builtin_bit_cast(unsigned long long, id);
+    const unsigned long long v = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_update_dpp(
+        static_cast<int>(static_cast<unsigned>(i)), static_cast<int>(static_cast<unsigned>(v)), CTRL,
+        0xF, 0xF, false));
+    const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_update_dpp(
+        static_cast<int>(static_cast<unsigned>(i >> 32)),
+        static_cast<int>(static_cast<unsigned>(v >> 32)), CTRL, 0xF, 0xF, false));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+template <int CTRL, typename T>
+__device__ __forceinline__ cx<T> dpp(cx<T> id, cx<T> x)
+{
+    return {dpp<CTRL>(id.re, x.re), dpp<CTRL>(id.im, x.im)};
+}
+template <int CTRL, typename T>
+__device__ __forceinline__ Aff<T> dpp(const Aff<T> &id, const Aff<T> &x)
+{
+    return {dpp<CTRL>(id.p00, x.p00), dpp<CTRL>(id.p01, x.p01), dpp<CTRL>(id.p10, x.p10),
+            dpp<CTRL>(id.p11, x.p11), dpp<CTRL>(id.c0, x.c0),   dpp<CTRL>(id.c1, x.c1)};
+}
+template <typename T>
+__device__ __forceinline__ Aff<T> aff_shfl(const Aff<T> &x, int src, int width)
+{
+    return {__shfl(x.p00, src, width), __shfl(x.p01, src, width), __shfl(x.p10, src, width),
+            __shfl(x.p11, src, width), {__shfl(x.c0.re, src, width), __shfl(x.c0.im, src, width)},
+            {__shfl(x.c1.re, src, width), __shfl(x.c1.im, src, width)}};
+}
+// value of lane - 1 / lane + 1 (the caller replaces it on a group's first /
+// last lane)
+template <typename T>
+__device__ __forceinline__ cx<T> lane_prev(cx<T> x) { return dpp<0x138>(cx<T>{T(0), T(0)}, x); }
+template <typename T>
+__device__ __forceinline__ cx<T> lane_next(cx<T> x) { return dpp<0x130>(cx<T>{T(0), T(0)}, x); }
+
+// Inclusive scan of the lanes' maps in lane order (lane t ends with the map
+// of lanes 0..t): four DPP steps inside each 16-lane row, then the rows are
+// joined through lane shuffles (one step for 32 lanes, two for 64).
+template <typename T, int G>
+__device__ __forceinline__ void scan_up(Aff<T> &m, int lane)
+{
+    const Aff<T> id = aff_identity<T>();
+    aff_fold(m, dpp<0x111>(id, m));
+    aff_fold(m, dpp<0x112>(id, m));
+    aff_fold(m, dpp<0x114>(id, m));
+    aff_fold(m, dpp<0x118>(id, m));
+    if (G >= 32) {
+        const Aff<T> q = aff_shfl(m, (lane & 32) + 15, G);   // end of the row below
+        if (lane & 16) aff_fold(m, q);
+    }
+    if (G == 64) {
+        const Aff<T> q = aff_shfl(m, 31, G);
+        if (lane & 32) aff_fold(m, q);
+    }
+}
+// Same in reverse lane order (lane t ends with the map of lanes t..G-1).
+template <typename T, int G>
+__device__ __forceinline__ void scan_down(Aff<T> &m, int lane)
+{
+    const Aff<T> id = aff_identity<T>();
+    aff_fold(m, dpp<0x101>(id, m));
+    aff_fold(m, dpp<0x102>(id, m));
+    aff_fold(m, dpp<0x104>(id, m));
+    aff_fold(m, dpp<0x108>(id, m));
+    if (G >= 32) {
+        const Aff<T> q = aff_shfl(m, (lane & 32) + 16, G);   // start of the row above
+        if (!(lane & 16)) aff_fold(m, q);
+    }
+    if (G == 64) {
+        const Aff<T> q = aff_shfl(m, 32, G);
+        if (!(lane & 32)) aff_fold(m, q);
+    }
+}
+
 // The line's data is reached through accessors: getX(j) = off-diagonal x
 // term of row j, getV(j) = current v, getS(j, r) = source of row j = lane*R + r,
 // putV(j, v) = store.
@@ -155,32 +271,19 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const LineLU<T
 
     // ---- forward substitution: y_j = (r_j - l2_j y_{j-2}) - l1_j y_{j-1}
     // state (a, b) = (y_{j-2}, y_{j-1}); map M_j = [[0,1],[-l2,-l1]]
-    T p00 = 1, p01 = 0, p10 = 0, p11 = 1;
-    cx<T> c0 = {0, 0}, c1 = {0, 0};
+    Aff<T> m = aff_identity<T>();
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const T a1 = f.l1[r], a2 = f.l2[r];
-        const T n10 = -a2 * p00 - a1 * p10, n11 = -a2 * p01 - a1 * p11;
-        p00 = p10; p01 = p11; p10 = n10; p11 = n11;
-        const cx<T> nc1 = (y[r] - a2 * c0) - a1 * c1;
-        c0 = c1; c1 = nc1;
+        const T n10 = -a2 * m.p00 - a1 * m.p10, n11 = -a2 * m.p01 - a1 * m.p11;
+        m.p00 = m.p10; m.p01 = m.p11; m.p10 = n10; m.p11 = n11;
+        const cx<T> nc1 = (y[r] - a2 * m.c0) - a1 * m.c1;
+        m.c0 = m.c1; m.c1 = nc1;
     }
-#pragma unroll
-    for (int d = 1; d < G; d <<= 1) {
-        const T q00 = __shfl_up(p00, d, G), q01 = __shfl_up(p01, d, G);
-        const T q10 = __shfl_up(p10, d, G), q11 = __shfl_up(p11, d, G);
-        const cx<T> e0 = shfl_up(c0, d, G), e1 = shfl_up(c1, d, G);
-        if (lane >= d) {
-            const cx<T> nc0 = p00 * e0 + p01 * e1 + c0;
-            const cx<T> nc1 = p10 * e0 + p11 * e1 + c1;
-            c0 = nc0; c1 = nc1;
-            const T m00 = p00 * q00 + p01 * q10, m01 = p00 * q01 + p01 * q11;
-            const T m10 = p10 * q00 + p11 * q10, m11 = p10 * q01 + p11 * q11;
-            p00 = m00; p01 = m01; p10 = m10; p11 = m11;
-        }
-    }
+    scan_up<T, G>(m, lane);
     {
-        cx<T> a = shfl_up(c0, 1, G), b = shfl_up(c1, 1, G);
+        // state entering this lane's rows: the previous lane's inclusive map
+        cx<T> a = lane_prev(m.c0), b = lane_prev(m.c1);
         if (lane == 0) { a = {0, 0}; b = {0, 0}; }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -193,34 +296,20 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const LineLU<T
 
     // ---- back substitution: x_j = ((y_j - u1_j x_{j+1}) - u2_j x_{j+2}) / piv_j
     // state (a, b) = (x_{j+1}, x_{j+2}); map [[-u1/p, -u2/p],[1, 0]]
-    p00 = 1; p01 = 0; p10 = 0; p11 = 1;
-    c0 = {0, 0}; c1 = {0, 0};
+    m = aff_identity<T>();
 #pragma unroll
     for (int r = R - 1; r >= 0; --r) {
         // composition only: precomputed -u1/p, -u2/p, 1/p (the replay below
         // divides by the pivot like the reference)
         const T m0 = f.m1[r], m1 = f.m2[r], rp = f.rp[r];
-        const T n00 = m0 * p00 + m1 * p10, n01 = m0 * p01 + m1 * p11;
-        p10 = p00; p11 = p01; p00 = n00; p01 = n01;
-        const cx<T> nc0 = rp * y[r] + m0 * c0 + m1 * c1;
-        c1 = c0; c0 = nc0;
+        const T n00 = m0 * m.p00 + m1 * m.p10, n01 = m0 * m.p01 + m1 * m.p11;
+        m.p10 = m.p00; m.p11 = m.p01; m.p00 = n00; m.p01 = n01;
+        const cx<T> nc0 = rp * y[r] + m0 * m.c0 + m1 * m.c1;
+        m.c1 = m.c0; m.c0 = nc0;
     }
-#pragma unroll
-    for (int d = 1; d < G; d <<= 1) {
-        const T q00 = __shfl_down(p00, d, G), q01 = __shfl_down(p01, d, G);
-        const T q10 = __shfl_down(p10, d, G), q11 = __shfl_down(p11, d, G);
-        const cx<T> e0 = shfl_down(c0, d, G), e1 = shfl_down(c1, d, G);
-        if (lane + d < G) {
-            const cx<T> nc0 = p00 * e0 + p01 * e1 + c0;
-            const cx<T> nc1 = p10 * e0 + p11 * e1 + c1;
-            c0 = nc0; c1 = nc1;
-            const T m00 = p00 * q00 + p01 * q10, m01 = p00 * q01 + p01 * q11;
-            const T m10 = p10 * q00 + p11 * q10, m11 = p10 * q01 + p11 * q11;
-            p00 = m00; p01 = m01; p10 = m10; p11 = m11;
-        }
-    }
+    scan_down<T, G>(m, lane);
     {
-        cx<T> a = shfl_down(c0, 1, G), b = shfl_down(c1, 1, G);
+        cx<T> a = lane_next(m.c0), b = lane_next(m.c1);
         if (lane == G - 1) { a = {0, 0}; b = {0, 0}; }
 #pragma unroll
         for (int r = R - 1; r >= 0; --r) {
