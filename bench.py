@@ -41,12 +41,20 @@ ROWS = (1, 2, 3, 4, 5, 7, 9, 13, 17)
 
 
 def smoother_kernel(prec, lanes, ny):
-    """Name of the smoother kernel a line shape runs (as rocprofv3 prints it)."""
-    kind = "strip" if lanes > 1000 else "staged" if lanes < 0 else "phase"
-    g = lanes - 1000 if lanes > 1000 else abs(lanes)
+    """Name of the smoother kernel a line shape runs (as rocprofv3 prints it):
+    G direct, -G staged (x terms, v, S in LDS), 2000+G staged (x terms only),
+    1000+G fused-sweep strip."""
+    g = lanes - 2000 if lanes > 2000 else lanes - 1000 if lanes > 1000 else abs(lanes)
     need = -(-ny // g)
     r = next(x for x in ROWS if x >= need)
-    return f"smooth_{kind}_kernel<{'float' if prec else 'double'}, {g}, {r}>"
+    t = "float" if prec else "double"
+    if lanes > 2000:
+        return f"smooth_staged_kernel<{t}, {g}, {r}, false>"
+    if lanes > 1000:
+        return f"smooth_strip_kernel<{t}, {g}, {r}>"
+    if lanes < 0:
+        return f"smooth_staged_kernel<{t}, {g}, {r}, true>"
+    return f"smooth_phase_kernel<{t}, {g}, {r}>"
 
 
 def measured_traffic(kernel):

@@ -183,14 +183,11 @@ __global__ void __launch_bounds__(256) restrict_kernel(Level F, Level Cl, int K,
                                                        cx<T> *Vc, cx<T> *Wc, cx<T> *Sc,
                                                        int fas, const int *active)
 {
-    const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
-    const size_t per = Cl.plane;
-    const int k = static_cast<int>(idx / per);
-    if (k >= K) return;
+    // grid (rows / 32, columns / 8, modes), block 32 x 8: no index division
+    const int k = blockIdx.z;
     if (active && !active[k]) return;
-    const size_t rem = idx - static_cast<size_t>(k) * per;
-    const int i = static_cast<int>(rem / Cl.ld), j = static_cast<int>(rem % Cl.ld);
-    if (j >= Cl.ny) return;
+    const int j = blockIdx.x * 32 + threadIdx.x, i = blockIdx.y * 8 + threadIdx.y;
+    if (j >= Cl.ny || i >= Cl.nx) return;
     const int fi = 2 * i, fj = 2 * j;
     const cx<T> *vf = Vf + static_cast<size_t>(k) * F.plane;
     const cx<T> *rf = Rf + static_cast<size_t>(k) * F.plane;
@@ -225,12 +222,10 @@ __global__ void __launch_bounds__(256) prolong_kernel(Level F, Level Cl, int K, 
                                                       const cx<T> *Vc, const cx<T> *Wc,
                                                       int fas, const int *active)
 {
-    const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
-    const int k = static_cast<int>(idx / F.plane);
-    if (k >= K) return;
+    // grid (rows / 32, columns / 8, modes), block 32 x 8: no index division
+    const int k = blockIdx.z;
     if (active && !active[k]) return;
-    const size_t rem = idx - static_cast<size_t>(k) * F.plane;
-    const int i = static_cast<int>(rem / F.ld), j = static_cast<int>(rem % F.ld);
+    const int j = blockIdx.x * 32 + threadIdx.x, i = blockIdx.y * 8 + threadIdx.y;
     if (j >= F.ny || i < 1 || i > F.nx - 2) return;
     const cx<T> *vc = Vc + static_cast<size_t>(k) * Cl.plane;
     const cx<T> *wc = Wc + static_cast<size_t>(k) * Cl.plane;
@@ -303,13 +298,13 @@ __global__ void __launch_bounds__(256) gather_kernel(int nx, int ny, size_t ld, 
 __global__ void __launch_bounds__(256) source_kernel(LevelArgs<double> L, const cx<double> *Gin,
                                                      cx<double> *S)
 {
-    const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
-    const int k = static_cast<int>(idx / L.plane);
-    if (k >= L.K) return;
-    const size_t rem = idx - static_cast<size_t>(k) * L.plane;
-    const int j = static_cast<int>(rem % L.ld);
-    if (j >= L.ny) return;
-    const cx<double> *g = Gin + (idx - j);
+    // grid (rows / 32, columns / 8, modes), block 32 x 8: no index division
+    const int k = blockIdx.z;
+    const int j = blockIdx.x * 32 + threadIdx.x, i = blockIdx.y * 8 + threadIdx.y;
+    if (j >= L.ny || i >= L.nx) return;
+    const size_t col = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * L.ld;
+    const size_t idx = col + j;
+    const cx<double> *g = Gin + col;
     cx<double> s = {0, 0};
     if (j == 1) s = L.sp[1] * g[1] + L.sp[2] * g[2];
     else if (j >= 2 && j <= L.ny - 3) s = L.cp[0] * g[j - 1] + L.cp[1] * g[j] + L.cp[2] * g[j + 1];
@@ -372,13 +367,12 @@ __global__ void fill_int_kernel(int *p, int n, int v)
 __global__ void __launch_bounds__(256) update_kernel(Level F, int K, cx<double> *V,
                                                      const cx<float> *E, const int *active)
 {
-    const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
-    const int k = static_cast<int>(idx / F.plane);
-    if (k >= K) return;
+    // grid (rows / 32, columns / 8, modes), block 32 x 8: no index division
+    const int k = blockIdx.z;
     if (active && !active[k]) return;
-    const size_t rem = idx - static_cast<size_t>(k) * F.plane;
-    const int i = static_cast<int>(rem / F.ld), j = static_cast<int>(rem % F.ld);
+    const int j = blockIdx.x * 32 + threadIdx.x, i = blockIdx.y * 8 + threadIdx.y;
     if (i < 1 || i > F.nx - 2 || j < 1 || j >= F.ny) return;
+    const size_t idx = static_cast<size_t>(k) * F.plane + static_cast<size_t>(i) * F.ld + j;
     const cx<float> e = E[idx];
     V[idx] = cx<double>{V[idx].re + static_cast<double>(e.re), V[idx].im + static_cast<double>(e.im)};
 }
@@ -398,6 +392,13 @@ __global__ void __launch_bounds__(256) convert_kernel(size_t n, const cx<TA> *a,
 // ---------------------------------------------------------------------------
 
 static const int kRs[] = {1, 2, 3, 4, 5, 7, 9, 13, 17};
+
+// grid of the point-wise level kernels: 32 rows x 8 columns per block
+static dim3 grid_2d(const Level &l, int K)
+{
+    return dim3(static_cast<unsigned>((l.ny + 31) / 32), static_cast<unsigned>((l.nx + 7) / 8),
+                static_cast<unsigned>(K));
+}
 
 // Raise the dynamic-LDS limit of every LDS-resident smoother instantiation
 // once, before any stream capture can happen.
@@ -450,18 +451,16 @@ template <typename T>
 void restrict_level(const Level &F, const Level &Cl, int K, const cx<T> *Vf, const cx<T> *Rf,
                     cx<T> *Vc, cx<T> *Wc, cx<T> *Sc, int fas, const int *active, hipStream_t st)
 {
-    const size_t n = Cl.plane * K;
-    hipLaunchKernelGGL((restrict_kernel<T>), dim3(nblk(n)), dim3(256), 0, st, F, Cl, K, Vf, Rf, Vc,
-                       Wc, Sc, fas, active);
+    hipLaunchKernelGGL((restrict_kernel<T>), grid_2d(Cl, K), dim3(32, 8), 0, st, F, Cl, K, Vf, Rf,
+                       Vc, Wc, Sc, fas, active);
 }
 
 template <typename T>
 void prolong_level(const Level &F, const Level &Cl, int K, cx<T> *Vf, const cx<T> *Vc,
                    const cx<T> *Wc, int fas, const int *active, hipStream_t st)
 {
-    const size_t n = F.plane * K;
-    hipLaunchKernelGGL((prolong_kernel<T>), dim3(nblk(n)), dim3(256), 0, st, F, Cl, K, Vf, Vc, Wc,
-                       fas, active);
+    hipLaunchKernelGGL((prolong_kernel<T>), grid_2d(F, K), dim3(32, 8), 0, st, F, Cl, K, Vf, Vc,
+                       Wc, fas, active);
 }
 
 void gather(int nx, int ny, int K, const Level &F, const double *wx, const double *dz,
@@ -477,7 +476,8 @@ void gather(int nx, int ny, int K, const Level &F, const double *wx, const doubl
 
 void source(const LevelArgs<double> &L, const cx<double> *G, cx<double> *S, hipStream_t st)
 {
-    hipLaunchKernelGGL(source_kernel, dim3(nblk(L.plane * L.K)), dim3(256), 0, st, L, G, S);
+    hipLaunchKernelGGL(source_kernel, dim3((L.ny + 31) / 32, (L.nx + 7) / 8, L.K), dim3(32, 8), 0,
+                       st, L, G, S);
 }
 
 void scatter(int nx, int ny, int K, const Level &F, const cx<double> *V, double *uy,
@@ -508,7 +508,7 @@ void fill_int(int *p, int n, int v, hipStream_t st)
 void update(const Level &F, int K, cx<double> *V, const cx<float> *E, const int *active,
             hipStream_t st)
 {
-    hipLaunchKernelGGL(update_kernel, dim3(nblk(F.plane * K)), dim3(256), 0, st, F, K, V, E, active);
+    hipLaunchKernelGGL(update_kernel, grid_2d(F, K), dim3(32, 8), 0, st, F, K, V, E, active);
 }
 
 template <typename TA, typename TB>
