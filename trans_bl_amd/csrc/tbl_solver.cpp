@@ -77,6 +77,14 @@ LevelArgs<T> level_args(const tbl_ctx &c, int l, const T *qdy2)
     return A;
 }
 
+// Key of the line shapes of all levels (a captured V-cycle bakes them in).
+int shapes_key(const std::vector<int> &G)
+{
+    unsigned h = 2166136261u;
+    for (int g : G) h = (h ^ static_cast<unsigned>(g + 4096)) * 16777619u;
+    return static_cast<int>(h & 0x7fffffff);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -304,12 +312,13 @@ struct Sched {
 // coarsest sweeps (itera and 20*itera, poisson_par.f:24, 45).  The
 // correction cycle only needs an approximate correction, and the converged v
 // does not depend on it (it is the fixed point of the fine-level residual):
-// on the 1025x193x64, 385x97x8 and 2049x257x128 wavepacket workloads 2
-// pre-sweeps and 5 coarsest sweeps took 0.5-0.7 more V-cycles than 3 and 10
-// but 7-26 % less time to the same 1e-9 residual; fewer coarsest sweeps
-// started to add cycles.
+// on wavepacket workloads from 257x65 to 2049x257, 2 pre-sweeps and 10
+// coarsest sweeps took up to one more V-cycle than 3 and 10 but less time
+// to the same 1e-9 residual on all but the smallest grids.  5 coarsest
+// sweeps were faster again where ny <= 193 but needed up to 15 cycles on
+// grids with ny = 257.
 constexpr int kMixedPreSweeps = 2;
-constexpr int kMixedCoarseSweeps = 5;
+constexpr int kMixedCoarseSweeps = 10;
 
 Sched sched_of(const tbl_solve_opts *o)
 {
@@ -425,7 +434,7 @@ int mixed_cycle(tbl_ctx *c, const Sched &s, double tol)
 int run_cycle(tbl_ctx *c, int prec, const Sched &s, double tol, bool use_graph)
 {
     if (!use_graph) return prec ? mixed_cycle(c, s, tol) : fas_cycle(c, s, tol);
-    const int key[5] = {s.nu1, s.nu2, s.nc, c->G64[0], c->G32[0]};
+    const int key[5] = {s.nu1, s.nu2, s.nc, shapes_key(c->G64), shapes_key(c->G32)};
     if (!c->graph[prec] || std::memcmp(key, c->graph_key[prec], sizeof(key)) != 0 ||
         c->graph_tol[prec] != tol) {
         if (c->graph[prec]) {
