@@ -20,8 +20,8 @@ __device__ __forceinline__ void atomic_max_nonneg(double *addr, double v)
 // Defect by column marching: one wavefront owns kDefRows consecutive output
 // rows of one strip of columns of one mode.  Its 64 lanes hold rows
 // j0-2 .. j0+61 (two halo rows each side); it walks the strip in x keeping
-// the columns i-2..i+2 of its rows in a ring of eight registers and takes the
-// y-neighbours from lane shuffles, so every value of v is loaded about once.
+// the columns i-2..i+2 of its rows in registers and takes the y-neighbours
+// from DPP lane shifts, so every value of v is loaded about once.
 // The march goes in blocks of B columns: at a block's start the
 // wavefront issues the loads of the next block's v and S columns, then
 // computes and stores the block from registers, and only at its end takes
@@ -121,9 +121,13 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
             auto own_x = [&]() {
                 return border ? xs : T(-1) * (w0 + w4) + T(16) * (w1 + w3) - T(30) * w2;
             };
-            const cx<T> xm = shfl_up(xs, 1, 64), xp = shfl_down(xs, 1, 64);
-            const cx<T> vm1 = shfl_up(w2, 1, 64), vm2 = shfl_up(w2, 2, 64);
-            const cx<T> vp1 = shfl_down(w2, 1, 64), vp2 = shfl_down(w2, 2, 64);
+            // y-neighbours from the adjacent lanes by DPP wavefront shifts
+            // (no LDS round trip); the edge lanes 0, 1, 62, 63 are halo rows
+            // whose results are not used
+            const cx<T> z = {T(0), T(0)};
+            const cx<T> xm = dpp<0x138>(z, xs), xp = dpp<0x130>(z, xs);
+            const cx<T> vm1 = dpp<0x138>(z, w2), vm2 = dpp<0x138>(z, vm1);
+            const cx<T> vp1 = dpp<0x130>(z, w2), vp2 = dpp<0x130>(z, vp1);
             if (own) {
                 cx<T> lv;
                 if (j == 1) {
