@@ -196,16 +196,37 @@ def test_staged_smoother_matches_direct(torch_mod, precision, form, grid):
     f = wl.wavepacket_rhs(g, tag="staged")
     z = np.zeros_like(f)
     with context(g) as ctx:
-        for lvl in range(1, g.levels + 1):
+        # (the coarsest level keeps its own kernel)
+        for lvl in range(1, g.levels):
             for prec in (0, 1):
                 ctx.set_line_shape(prec, lvl, _lanes(ctx.line_shape(prec, lvl)))
         a, ca, _, _ = ctx.poi_uy(None, -f, z, 1e-9, precision=precision, max_cycles=20)
-        for lvl in range(1, g.levels + 1):
+        for lvl in range(1, g.levels):
             for prec in (0, 1):
                 lanes = _lanes(ctx.line_shape(prec, lvl))
                 ctx.set_line_shape(prec, lvl, -lanes if form == "staged" else 2000 + lanes)
         b, cb, _, _ = ctx.poi_uy(None, -f, z, 1e-9, precision=precision, max_cycles=20)
     assert np.array_equal(a, b) and np.array_equal(ca, cb)
+
+
+@pytest.mark.parametrize("grid", [SMALL, CFG1], ids=["small", "385x97x8"])
+@pytest.mark.parametrize("lanes", [16, 32])
+def test_coarsest_scan_matches_sequential(torch_mod, grid, lanes):
+    """The coarsest level's lane-scan kernel (shape G) and its sequential
+    kernels (shape 1) solve the same lines: same converged solution, iterates
+    within rounding, FP64."""
+    f = wl.wavepacket_rhs(grid, tag="coarse")
+    z = np.zeros_like(f)
+    with context(grid) as ctx:
+        assert ctx.line_shape(0, grid.levels) == 1
+        a, ca, _, _ = ctx.poi_uy(None, -f, z, 1e-9, max_cycles=20)
+        a2, _, _, _ = ctx.poi_uy(None, -f, z, 0.0, max_cycles=2)
+        ctx.set_line_shape(0, grid.levels, lanes)
+        b, cb, _, _ = ctx.poi_uy(None, -f, z, 1e-9, max_cycles=20)
+        b2, _, _, _ = ctx.poi_uy(None, -f, z, 0.0, max_cycles=2)
+    assert rel(b2, a2) < 1e-12, rel(b2, a2)
+    assert np.array_equal(ca, cb)
+    assert rel(b, a) < 1e-12, rel(b, a)
 
 
 def _strip_shapes(ctx, g):

@@ -16,8 +16,10 @@ import os
 from . import TblError
 
 # direct, LDS-staged (negative), x-terms-staged (2000 + lanes) and
-# fused-sweep strip (1000 + lanes) forms
-CANDIDATES = (64, 32, 16, -64, -32, -16, 2064, 2032, 2016, 1064, 1032, 1016)
+# fused-sweep strip (1000 + lanes) forms; 1 = the coarsest level's sequential
+# kernels (rejected on other levels; any lane form there selects the
+# lane-scan coarsest kernel)
+CANDIDATES = (1, 64, 32, 16, -64, -32, -16, 2064, 2032, 2016, 1064, 1032, 1016)
 
 
 def size_class(ctx, level):
@@ -36,6 +38,11 @@ def tune(ctx, sweeps: int = 6, levels=None, precisions=(0, 1), verbose=False):
     for lvl in levels:
         entry = {"level": lvl, "us_per_sweep": {}}
         for prec in precisions:
+            if prec == 1 and lvl == ctx.grid.levels and entry.get("fp64") is not None:
+                # the coarsest solve runs in FP64 in both cycles
+                ctx.set_line_shape(1, lvl, entry["fp64"])
+                entry["fp32"] = entry["fp64"]
+                continue
             best, best_t = None, None
             orig = ctx.line_shape(prec, lvl)
             for lanes in CANDIDATES:

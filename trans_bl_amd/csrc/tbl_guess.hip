@@ -283,7 +283,7 @@ int guess_apply(tbl_ctx *c, hipStream_t st, bool accept_all, int *fallback,
         for (cx<double> *cand : {c->V[L], c->W[L]}) {
             if (accept_all && cand == c->W[L]) continue;
             rc = smooth_sweeps<double>(c->la_d[L], cand, c->S[L], c->lu_d[L], kGuessSweeps, 1.2,
-                                       nullptr, c->G64[L], c->H64[L], st);
+                                       nullptr, sweep_shape(c->G64[L]), c->H64[L], st);
             if (rc != TBL_OK) return rc;
         }
     }

@@ -42,6 +42,14 @@ constexpr int kStripShape = 1000;
 // kXStagedShape + G (G lanes per line).
 constexpr int kXStagedShape = 2000;
 
+// Coarsest level only: shape kCoarseSeqShape selects the sequential kernels
+// (one or two threads per line, smooth_coarsest_2s/_reg); any G form there
+// selects the G-lane scan kernel (smooth_coarsest_kernel).
+constexpr int kCoarseSeqShape = 1;
+// lanes for a colour-phase sweep on a level whose shape code may be the
+// sequential coarsest form
+inline int sweep_shape(int code) { return code == kCoarseSeqShape ? 16 : code; }
+
 // Line-shape codes: G, -G, kStripShape + G or kXStagedShape + G.
 inline bool shape_is_strip(int code) { return code > kStripShape && code < kXStagedShape; }
 inline int shape_lanes(int code)

@@ -347,6 +347,10 @@ int smooth_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu
 #define X(NY) reg_ny = reg_ny || L.ny == NY;
     TBL_COARSE_NY(X)
 #undef X
+    // shape kCoarseSeqShape: the sequential kernels where they apply; any other
+    // shape, or a level they do not fit, the G-lane scan kernel
+    if (G != kCoarseSeqShape) reg_ny = false;
+    G = sweep_shape(G);
     if (reg_ny && coarse_2s_lds_bytes<T>(L.nx, L.ny) <= 160 * 1024 &&
         2 * phase_lines(1, L.nx) <= 256) {
         const size_t shm = coarse_2s_lds_bytes<T>(L.nx, L.ny);

@@ -166,6 +166,8 @@ static int create_impl(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx *c)
             if (h32) TRY(dalloc(&c->H32[l], h32));
         }
     }
+    // the coarsest level starts on its sequential kernels
+    c->G64[L - 1] = c->G32[L - 1] = kCoarseSeqShape;
     c->la_d.resize(L);
     c->la_f.resize(L);
     for (int l = 0; l < L; ++l) {
@@ -620,6 +622,10 @@ extern "C" int tbl_residual_dev(tbl_ctx *c, const double *d_wx, const double *d_
 // smoother, 1: FP32 smoother.
 extern "C" int tbl_set_line_shape(tbl_ctx *c, int prec, int level, int lanes)
 {
+    if (c && level == c->L && lanes == kCoarseSeqShape) {
+        (prec ? c->G32 : c->G64)[level - 1] = lanes;
+        return TBL_OK;
+    }
     const bool strip = shape_is_strip(lanes);
     const int a = shape_lanes(lanes);
     if (!c || level < 1 || level > c->L || (a != 16 && a != 32 && a != 64))
