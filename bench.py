@@ -22,6 +22,7 @@ reference-ordered FP64 solve on a bounded sample of modes, on the host cores).
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -160,6 +161,9 @@ def main():
             guess = None
 
     def run(prec, learned, steps, warm):
+        # no garbage-collector pass inside a timed loop
+        gc.collect()
+        gc.disable()
         stats = []
         for it in range(warm + steps):
             if it == warm:
@@ -175,6 +179,7 @@ def main():
                 stats.append((cyc.copy(), res.copy(), st))
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        gc.enable()
         return wall, stats
 
     wall64, st64 = run(0, False, a.steps, a.warmup)
