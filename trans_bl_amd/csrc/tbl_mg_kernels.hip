@@ -221,16 +221,26 @@ __global__ void __launch_bounds__(256) restrict_kernel(Level F, Level Cl, int K,
 // correction scheme (fas == 0) it is v_c.  fas == 2 assigns the bilinear
 // interpolant of v_c to the fine interior (rows 2..ny, Fortran) instead of
 // adding it.
+// two adjacent complex values (an even row and the next), aligned as a pair
+template <typename T>
+struct alignas(4 * sizeof(T)) cx_pair {
+    cx<T> a, b;
+};
+
+// A thread handles the even fine row j0 = 2m and the odd row j0 + 1 of one
+// column: both interpolate from coarse rows m and m + 1, and the pair is one
+// aligned load and store (ld is a multiple of 8, so j0 starts a pair).
 template <typename T>
 __global__ void __launch_bounds__(256) prolong_kernel(Level F, Level Cl, int K, cx<T> *Vf,
                                                       const cx<T> *Vc, const cx<T> *Wc,
                                                       int fas, const int *active)
 {
-    // grid (rows / 32, columns / 8, modes), block 32 x 8: no index division
+    // grid (row pairs / 32, columns / 8, modes), block 32 x 8: no index division
     const int k = blockIdx.z;
     if (active && !active[k]) return;
-    const int j = blockIdx.x * 32 + threadIdx.x, i = blockIdx.y * 8 + threadIdx.y;
-    if (j >= F.ny || i < 1 || i > F.nx - 2) return;
+    const int m = blockIdx.x * 32 + threadIdx.x, i = blockIdx.y * 8 + threadIdx.y;
+    const int j0 = 2 * m;
+    if (j0 >= F.ny || i < 1 || i > F.nx - 2) return;
     const cx<T> *vc = Vc + static_cast<size_t>(k) * Cl.plane;
     const cx<T> *wc = Wc + static_cast<size_t>(k) * Cl.plane;
     auto E = [&](int ci, int cj) {
@@ -242,14 +252,26 @@ __global__ void __launch_bounds__(256) prolong_kernel(Level F, Level Cl, int K, 
         if ((i & 1) == 0) return E(i >> 1, cj);
         return T(0.5) * (E(i >> 1, cj) + E((i >> 1) + 1, cj));
     };
-    cx<T> t;
-    if ((j & 1) == 0) t = row(j >> 1);
-    else t = T(0.5) * (row((j >> 1) + 1) + row(j >> 1));
-    const size_t o = static_cast<size_t>(k) * F.plane + static_cast<size_t>(i) * F.ld + j;
+    const bool has1 = j0 + 1 < F.ny;
+    const cx<T> t0 = row(m);
+    const size_t o = static_cast<size_t>(k) * F.plane + static_cast<size_t>(i) * F.ld + j0;
+    if (!has1) {             // last row of an odd ny: even, alone
+        if (fas == 2) {
+            if (j0 >= 1) Vf[o] = t0;
+        } else {
+            Vf[o] = Vf[o] + t0;
+        }
+        return;
+    }
+    const cx<T> t1 = T(0.5) * (row(m + 1) + t0);
+    cx_pair<T> *pv = reinterpret_cast<cx_pair<T> *>(Vf + o);
     if (fas == 2) {
-        if (j >= 1) Vf[o] = t;   // assign the interpolant, wall row untouched
+        // assign the interpolant, wall row (j = 0) untouched
+        if (j0 >= 1) *pv = cx_pair<T>{t0, t1};
+        else Vf[o + 1] = t1;
     } else {
-        Vf[o] = Vf[o] + t;
+        const cx_pair<T> v = *pv;
+        *pv = cx_pair<T>{v.a + t0, v.b + t1};
     }
 }
 
@@ -372,6 +394,7 @@ __global__ void __launch_bounds__(256) update_kernel(Level F, int K, cx<double> 
                                                      const cx<float> *E, const int *active)
 {
     // grid (rows / 32, columns / 8, modes), block 32 x 8: no index division
+    // (one point per thread: measured faster here than row pairs)
     const int k = blockIdx.z;
     if (active && !active[k]) return;
     const int j = blockIdx.x * 32 + threadIdx.x, i = blockIdx.y * 8 + threadIdx.y;
@@ -463,8 +486,10 @@ template <typename T>
 void prolong_level(const Level &F, const Level &Cl, int K, cx<T> *Vf, const cx<T> *Vc,
                    const cx<T> *Wc, int fas, const int *active, hipStream_t st)
 {
-    hipLaunchKernelGGL((prolong_kernel<T>), grid_2d(F, K), dim3(32, 8), 0, st, F, Cl, K, Vf, Vc,
-                       Wc, fas, active);
+    const dim3 grid(static_cast<unsigned>((F.ny + 63) / 64), static_cast<unsigned>((F.nx + 7) / 8),
+                    static_cast<unsigned>(K));
+    hipLaunchKernelGGL((prolong_kernel<T>), grid, dim3(32, 8), 0, st, F, Cl, K, Vf, Vc, Wc, fas,
+                       active);
 }
 
 void gather(int nx, int ny, int K, const Level &F, const double *wx, const double *dz,
