@@ -179,7 +179,9 @@ int tbl_residual_dev(tbl_ctx *ctx, const double *d_wx, const double *d_dwzdx,
    stages only the x-direction terms in LDS and reads v and the source
    lane-contiguous; 1000 + lanes selects the fused-sweep strip smoother (a
    whole sweep per launch, one workgroup per strip of columns in LDS; not
-   on the coarsest level).
+   on the coarsest level).  On the coarsest level lanes = 1 (the default)
+   selects the sequential one-launch kernels and any lane form the G-lane
+   scan kernel; its FP32 shape is unused (that solve is FP64).
    All forms give the same colour-ordered sweep.  prec 0 = FP64 smoother,
    1 = FP32 smoother.  Chosen by the autotuner; no reference counterpart
    (the reference solves each line sequentially, penta_par.f:367-404). */

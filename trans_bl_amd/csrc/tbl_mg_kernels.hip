@@ -36,7 +36,9 @@ __device__ __forceinline__ void atomic_max_nonneg(double *addr, double v)
 // top boundary term); x part: the border-column one-sided stencils on
 // columns 2 and nx-1, the centred fourth-order stencil elsewhere (grouped
 // on the two top rows).  The divisions by 12 dx^2 fac_y and fac_y are
-// multiplications by their reciprocals (last-bit rounding differs).
+// multiplications by their reciprocals, and on interior rows the row's
+// coefficients are folded into eight per-lane factors before the march
+// (last-bit rounding differs from the reference's expression order).
 constexpr int kDefRows = 60;
 template <typename T>
 constexpr int kDefBlock = sizeof(T) == 4 ? 8 : 2;
@@ -82,6 +84,14 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
     const T iscx = T(1) / scx, ifac = T(1) / L.fac_y;
     T q = T(0);
     if (own) q = (j == 1) ? L.qdy2[0] : (j == ny - 2) ? L.qdy2[j - 1] : L.qdy2[j - 2];
+    // interior rows: the row's coefficients folded once --
+    // lv = (cp0 xm + cp1 xs + cp2 xp) / scx + (cp3..cp7 . v) q
+    //      + k2b2 (cp0 vm1 + cp1 w2 + cp2 vp1) / fac_y
+    //    = ax . (xm, xs, xp) + cv . (vm2, vm1, w2, vp1, vp2)
+    const T kf = k2b2 * ifac;
+    const T ax0 = L.cp[0] * iscx, ax1 = L.cp[1] * iscx, ax2 = L.cp[2] * iscx;
+    const T cv0 = L.cp[3] * q, cv1 = L.cp[4] * q + L.cp[0] * kf, cv2 = L.cp[5] * q + L.cp[1] * kf;
+    const T cv3 = L.cp[6] * q + L.cp[2] * kf, cv4 = L.cp[7] * q;
     constexpr int B = kDefBlock<T>;
     // win[t] = column i0 - 2 + t of the current block (i0 .. i0+B-1 computed),
     // sq[t] = source column i0 + t; nb / ns: the next block's new columns
@@ -136,11 +146,8 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
                     const cx<T> d = k2b2 * (L.sp[1] * w2 + L.sp[2] * vp1);
                     lv = b * iscx + c * q + d * ifac;
                 } else if (j <= ny - 3) {
-                    const cx<T> b = L.cp[0] * xm + L.cp[1] * xs + L.cp[2] * xp;
-                    const cx<T> c = L.cp[3] * vm2 + L.cp[4] * vm1 + L.cp[5] * w2 + L.cp[6] * vp1
-                                    + L.cp[7] * vp2;
-                    const cx<T> d = k2b2 * (L.cp[0] * vm1 + L.cp[1] * w2 + L.cp[2] * vp1);
-                    lv = b * iscx + c * q + d * ifac;
+                    lv = ax0 * xm + ax1 * xs + ax2 * xp + cv0 * vm2 + cv1 * vm1 + cv2 * w2
+                         + cv3 * vp1 + cv4 * vp2;
                 } else if (j == ny - 2) {
                     const cx<T> c = L.pp[3] * vp1 + L.pp[2] * w2 + L.pp[1] * vm1;
                     const cx<T> d = k2b2 * w2;
