@@ -47,16 +47,19 @@ __device__ __forceinline__ cx<float> buf_load(__amdgpu_buffer_rsrc_t r, unsigned
     __builtin_memcpy(&x, &v, sizeof(x));
     return x;
 }
-__device__ __forceinline__ double buf_scalar(__amdgpu_buffer_rsrc_t r, unsigned off, double)
+// scalar loads; soff is a wavefront-uniform byte offset (a scalar register)
+__device__ __forceinline__ double buf_scalar(__amdgpu_buffer_rsrc_t r, unsigned off, double,
+                                             int soff = 0)
 {
-    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0);
     double x;
     __builtin_memcpy(&x, &v, sizeof(x));
     return x;
 }
-__device__ __forceinline__ float buf_scalar(__amdgpu_buffer_rsrc_t r, unsigned off, float)
+__device__ __forceinline__ float buf_scalar(__amdgpu_buffer_rsrc_t r, unsigned off, float,
+                                            int soff = 0)
 {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
 }
 
 }  // namespace tbl

@@ -14,6 +14,7 @@ constexpr int kLuFields = 8;
 template <typename T>
 struct LevelArgs {
     int nx, ny, ld, K;
+    int lus;                      // row stride of the line-LU tables (lu_stride)
     size_t plane;                 // nx * ld
     T sp[7], cp[8], pp[4], lp[5]; // sp_poi_coef(:,l) ... lp_poi_coef(:,l)
     T qx;                         // 1 / (12 dx_l^2 fac_y)
@@ -34,6 +35,11 @@ inline Level make_level(int nx, int ny, double dx2)
 }
 
 int line_rows_for(int G, int ny);
+
+// Row stride of a level's line-LU tables: at least ny and every line shape's
+// G*R, so a lane's R rows are always inside the table; rows past ny hold the
+// identity (zero factors, unit pivot and 1/pivot).
+int lu_stride(int ny);
 
 // Fused-sweep strip smoother: line shape kStripShape + G selects it (G lanes
 // per line); halo of kStripHalo columns each side of a strip.

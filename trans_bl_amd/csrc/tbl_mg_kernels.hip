@@ -455,6 +455,16 @@ int line_rows_for(int G, int ny)
     return -1;
 }
 
+int lu_stride(int ny)
+{
+    int s = ny;
+    for (int G : {16, 32, 64}) {
+        const int R = line_rows_for(G, ny);
+        if (R > 0 && G * R > s) s = G * R;
+    }
+    return (s + 7) / 8 * 8;
+}
+
 // G > 0: lanes per line with direct (lane-contiguous) global access;
 // G < 0: |G| lanes per line with the LDS-staged, coalesced kernel;
 // kStripShape < G < kXStagedShape: fused-sweep strips with G - kStripShape

@@ -81,32 +81,28 @@ __device__ __forceinline__ LineLU<T, R> load_line_lu(const T *__restrict__ lu, i
     return f;
 }
 
-// Same factors through a buffer resource over the LU table: off0 = byte
-// offset of the line's table (field 0, row 0).  Rows past ny get the
-// identity; their loads may run past the table and then read zero.
+// Same factors through a buffer resource over the LU tables (row stride
+// lus = lu_stride(ny)): off0 = byte offset of the line's table (field 0,
+// row 0).  The tables hold the identity past ny, so every load is
+// unconditional; the field offsets are wavefront-uniform.
 template <typename T, int R>
 __device__ __forceinline__ LineLU<T, R> load_line_lu_buf(__amdgpu_buffer_rsrc_t rs, unsigned off0,
-                                                        int ld, int j0, int ny)
+                                                        int lus, int j0)
 {
     LineLU<T, R> f;
-    const unsigned fb = static_cast<unsigned>(ld) * sizeof(T);
+    const int fb = lus * static_cast<int>(sizeof(T));
     const unsigned o = off0 + static_cast<unsigned>(j0) * sizeof(T);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const bool ok = j0 + r < ny;
         const unsigned e = o + r * sizeof(T);
-        const T l1 = buf_scalar(rs, e, T(0)), l2 = buf_scalar(rs, e + fb, T(0));
-        const T u1 = buf_scalar(rs, e + 2 * fb, T(0)), u2 = buf_scalar(rs, e + 3 * fb, T(0));
-        const T pv = buf_scalar(rs, e + 4 * fb, T(0)), m1 = buf_scalar(rs, e + 5 * fb, T(0));
-        const T m2 = buf_scalar(rs, e + 6 * fb, T(0)), rp = buf_scalar(rs, e + 7 * fb, T(0));
-        f.l1[r] = ok ? l1 : T(0);
-        f.l2[r] = ok ? l2 : T(0);
-        f.u1[r] = ok ? u1 : T(0);
-        f.u2[r] = ok ? u2 : T(0);
-        f.pv[r] = ok ? pv : T(1);
-        f.m1[r] = ok ? m1 : T(0);
-        f.m2[r] = ok ? m2 : T(0);
-        f.rp[r] = ok ? rp : T(1);
+        f.l1[r] = buf_scalar(rs, e, T(0), 0);
+        f.l2[r] = buf_scalar(rs, e, T(0), fb);
+        f.u1[r] = buf_scalar(rs, e, T(0), 2 * fb);
+        f.u2[r] = buf_scalar(rs, e, T(0), 3 * fb);
+        f.pv[r] = buf_scalar(rs, e, T(0), 4 * fb);
+        f.m1[r] = buf_scalar(rs, e, T(0), 5 * fb);
+        f.m2[r] = buf_scalar(rs, e, T(0), 6 * fb);
+        f.rp[r] = buf_scalar(rs, e, T(0), 7 * fb);
     }
     return f;
 }
@@ -338,7 +334,7 @@ __device__ __forceinline__ void relax_line(const LevelArgs<T> &L, cx<T> *col,
                                            int side, T rf, int lane)
 {
     const int ny = L.ny, ld = L.ld;
-    const LineLU<T, R> f = load_line_lu<T, R>(lu, ld, lane * R, ny);
+    const LineLU<T, R> f = load_line_lu<T, R>(lu, L.lus, lane * R, ny);
     relax_core<T, G, R>(
         L, f, rf, lane,
         [&](int j) { return xoff<T>(col, ld, j, side, side == 0 && j >= ny - 2); },

@@ -32,13 +32,13 @@ __global__ void __launch_bounds__(512) smooth_coarsest_kernel(LevelArgs<T> L, cx
     T *slu = reinterpret_cast<T *>(sv + static_cast<size_t>(ls) * nx);
     cx<T> *gv = V + static_cast<size_t>(k) * L.plane;
     const cx<T> *gs = S + static_cast<size_t>(k) * L.plane;
-    const T *glu = lu + (static_cast<size_t>(k) * 2) * kLuFields * ld;
+    const T *glu = lu + (static_cast<size_t>(k) * 2) * kLuFields * L.lus;
     for (int e = threadIdx.x; e < nx * ny; e += blockDim.x) {
         const int i = e / ny, j = e % ny;
         sv[i * ls + j] = gv[static_cast<size_t>(i) * ld + j];
     }
     for (int e = threadIdx.x; e < 2 * kLuFields * ny; e += blockDim.x)
-        slu[e] = glu[static_cast<size_t>(e / ny) * ld + e % ny];
+        slu[e] = glu[static_cast<size_t>(e / ny) * L.lus + e % ny];
     __syncthreads();
     // the line solve indexes the factors with stride L.ld: give it ny here
     LevelArgs<T> Ls = L;
@@ -141,14 +141,14 @@ __global__ void __launch_bounds__(256) smooth_coarsest_reg_kernel(LevelArgs<T> L
     cx<T> *gv = V + static_cast<size_t>(k) * L.plane;
     const cx<T> *gs = S + static_cast<size_t>(k) * L.plane;
     T *slu = reinterpret_cast<T *>(SL ? ss + static_cast<size_t>(LS) * nx : ss);
-    const T *glu = lu + (static_cast<size_t>(k) * 2) * kLuFields * ld;
+    const T *glu = lu + (static_cast<size_t>(k) * 2) * kLuFields * L.lus;
     for (int e = threadIdx.x; e < nx * NY; e += blockDim.x) {
         const int i = e / NY, j = e % NY;
         sv[i * LS + j] = gv[static_cast<size_t>(i) * ld + j];
         if (SL) ss[i * LS + j] = gs[static_cast<size_t>(i) * ld + j];
     }
     for (int e = threadIdx.x; e < 2 * kLuFields * NY; e += blockDim.x)
-        slu[e] = glu[static_cast<size_t>(e / NY) * ld + e % NY];
+        slu[e] = glu[static_cast<size_t>(e / NY) * L.lus + e % NY];
     __syncthreads();
     const T *lu_centre = slu;
     const T *lu_border = slu + kLuFields * NY;
@@ -233,14 +233,14 @@ __global__ void __launch_bounds__(256) smooth_coarsest_2s_kernel(LevelArgs<T> L,
     T *slu = reinterpret_cast<T *>(rbuf + static_cast<size_t>(LS) * phase_lines(1, nx));
     cx<T> *gv = V + static_cast<size_t>(k) * L.plane;
     const cx<T> *gs = S + static_cast<size_t>(k) * L.plane;
-    const T *glu = lu + (static_cast<size_t>(k) * 2) * kLuFields * ld;
+    const T *glu = lu + (static_cast<size_t>(k) * 2) * kLuFields * L.lus;
     for (int e = threadIdx.x; e < nx * NY; e += blockDim.x) {
         const int i = e / NY, j = e % NY;
         sv[i * LS + j] = gv[static_cast<size_t>(i) * ld + j];
         ss[i * LS + j] = gs[static_cast<size_t>(i) * ld + j];
     }
     for (int e = threadIdx.x; e < 2 * kLuFields * NY; e += blockDim.x)
-        slu[e] = glu[static_cast<size_t>(e / NY) * ld + e % NY];
+        slu[e] = glu[static_cast<size_t>(e / NY) * L.lus + e % NY];
     __syncthreads();
     const T qx = L.qx;
     // stage-1 lane roles for the colour phases: RW lanes per row set, NG line

@@ -19,7 +19,7 @@ __global__ void __launch_bounds__(256) smooth_phase_kernel(LevelArgs<T> L, cx<T>
     const int i = phase_column(phase, c, L.nx);
     const int side = (phase == 0) ? (c == 0 ? -1 : 1) : 0;
     const size_t base = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * L.ld;
-    const T *lut = lu + (static_cast<size_t>(k) * 2 + (side != 0)) * kLuFields * L.ld;
+    const T *lut = lu + (static_cast<size_t>(k) * 2 + (side != 0)) * kLuFields * L.lus;
     relax_line<T, G, R>(L, V + base, S + base, lut, side, rf, lane);
 }
 
@@ -112,10 +112,10 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
     __syncthreads();
     if (live) {
         const __amdgpu_buffer_rsrc_t lr =
-            buf_rsrc(lu, static_cast<size_t>(L.K) * 2 * kLuFields * ld * sizeof(T));
+            buf_rsrc(lu, static_cast<size_t>(L.K) * 2 * kLuFields * L.lus * sizeof(T));
         const unsigned lo = static_cast<unsigned>((static_cast<size_t>(k) * 2 + (side != 0)) *
-                                                  kLuFields * ld * sizeof(T));
-        const LineLU<T, R> f = load_line_lu_buf<T, R>(lr, lo, ld, lane * R, ny);
+                                                  kLuFields * L.lus * sizeof(T));
+        const LineLU<T, R> f = load_line_lu_buf<T, R>(lr, lo, L.lus, lane * R);
         if (FULL) {
             relax_core<T, G, R>(L, f, rf, lane, [&](int j) { return Xs[j]; },
                                 [&](int j) { return Vs[j]; }, [&](int j, int) { return Ss[j]; },

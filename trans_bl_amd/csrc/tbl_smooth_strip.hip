@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(512) smooth_strip_kernel(LevelArgs<T> L, cx<T>
             out[r] = (j0 + r < ny) ? scol[j0 + r] : cx<T>{T(0), T(0)};
     };
     const LineLU<T, R> fc = load_line_lu<T, R>(
-        lu + static_cast<size_t>(k) * 2 * kLuFields * ld, ld, j0, ny);
+        lu + static_cast<size_t>(k) * 2 * kLuFields * L.lus, L.lus, j0, ny);
     int ph = 0, c = grp;
     if (c >= cnts[0]) {
         c -= ngrp;
@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(512) smooth_strip_kernel(LevelArgs<T> L, cx<T>
                 run(fc);
             } else {
                 run(load_line_lu<T, R>(
-                    lu + (static_cast<size_t>(k) * 2 + 1) * kLuFields * ld, ld, j0, ny));
+                    lu + (static_cast<size_t>(k) * 2 + 1) * kLuFields * L.lus, L.lus, j0, ny));
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) Scur[r] = Snext[r];

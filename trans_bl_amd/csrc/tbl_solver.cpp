@@ -61,6 +61,7 @@ LevelArgs<T> level_args(const tbl_ctx &c, int l, const T *qdy2)
     A.nx = lv.nx;
     A.ny = lv.ny;
     A.ld = lv.ld;
+    A.lus = lu_stride(lv.ny);
     A.K = c.K;
     A.plane = lv.plane;
     for (int i = 0; i < 7; ++i) A.sp[i] = static_cast<T>(c.c.sp_poi[l][i]);
@@ -135,13 +136,14 @@ static int create_impl(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx *c)
         TBL_HIP(hipMemcpy(c->qdy2_d[l], q.data(), sizeof(double) * lv.ld, hipMemcpyHostToDevice));
         TBL_HIP(hipMemcpy(c->qdy2_f[l], qf.data(), sizeof(float) * lv.ld, hipMemcpyHostToDevice));
         // line LU factors per (mode, centre/border)
-        const size_t per = static_cast<size_t>(kLuFields) * lv.ld;
+        // (row stride lu_stride(ny); rows past ny hold the identity)
+        const int d = lu_stride(lv.ny);
+        const size_t per = static_cast<size_t>(kLuFields) * d;
         std::vector<double> lu(per * 2 * K, 0.0);
         std::vector<float> luf(lu.size(), 0.0f);
         for (int k = 0; k < K; ++k)
             for (int t = 0; t < 2; ++t) {
                 double *b = &lu[(static_cast<size_t>(k) * 2 + t) * per];
-                const int d = lv.ld;
                 line_lu(c->g, c->c, c->m, l + 1, k, t == 1, b, b + d, b + 2 * d, b + 3 * d,
                         b + 4 * d);
                 for (int j = 0; j < lv.ny; ++j) {
@@ -149,6 +151,11 @@ static int create_impl(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx *c)
                     b[5 * d + j] = -b[2 * d + j] / p;
                     b[6 * d + j] = -b[3 * d + j] / p;
                     b[7 * d + j] = 1.0 / p;
+                }
+                for (int j = lv.ny; j < d; ++j) {
+                    for (int f = 0; f < kLuFields; ++f) b[f * d + j] = 0.0;
+                    b[4 * d + j] = 1.0;
+                    b[7 * d + j] = 1.0;
                 }
             }
         for (size_t e = 0; e < lu.size(); ++e) luf[e] = static_cast<float>(lu[e]);
