@@ -107,6 +107,26 @@ __device__ __forceinline__ LineLU<T, R> load_line_lu_buf(__amdgpu_buffer_rsrc_t 
     return f;
 }
 
+// Same from a plain pointer to the line's padded table (stride lus).
+template <typename T, int R>
+__device__ __forceinline__ LineLU<T, R> load_line_lu_tbl(const T *__restrict__ lu, int lus, int j0)
+{
+    LineLU<T, R> f;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        f.l1[r] = lu[j];
+        f.l2[r] = lu[lus + j];
+        f.u1[r] = lu[2 * lus + j];
+        f.u2[r] = lu[3 * lus + j];
+        f.pv[r] = lu[4 * lus + j];
+        f.m1[r] = lu[5 * lus + j];
+        f.m2[r] = lu[6 * lus + j];
+        f.rp[r] = lu[7 * lus + j];
+    }
+    return f;
+}
+
 // ---------------------------------------------------------------------------
 // the partitioned scan across the G lanes of a line
 // ---------------------------------------------------------------------------
@@ -334,7 +354,7 @@ __device__ __forceinline__ void relax_line(const LevelArgs<T> &L, cx<T> *col,
                                            int side, T rf, int lane)
 {
     const int ny = L.ny, ld = L.ld;
-    const LineLU<T, R> f = load_line_lu<T, R>(lu, L.lus, lane * R, ny);
+    const LineLU<T, R> f = load_line_lu_tbl<T, R>(lu, L.lus, lane * R);
     relax_core<T, G, R>(
         L, f, rf, lane,
         [&](int j) { return xoff<T>(col, ld, j, side, side == 0 && j >= ny - 2); },

@@ -101,8 +101,10 @@ __global__ void __launch_bounds__(512) smooth_strip_kernel(LevelArgs<T> L, cx<T>
         for (int r = 0; r < R; ++r)
             out[r] = (j0 + r < ny) ? scol[j0 + r] : cx<T>{T(0), T(0)};
     };
-    const LineLU<T, R> fc = load_line_lu<T, R>(
-        lu + static_cast<size_t>(k) * 2 * kLuFields * L.lus, L.lus, j0, ny);
+    const __amdgpu_buffer_rsrc_t lr =
+        buf_rsrc(lu, static_cast<size_t>(L.K) * 2 * kLuFields * L.lus * sizeof(T));
+    const unsigned tbl = static_cast<unsigned>(kLuFields * L.lus * sizeof(T));
+    const LineLU<T, R> fc = load_line_lu_buf<T, R>(lr, (2 * k) * tbl, L.lus, j0);
     int ph = 0, c = grp;
     if (c >= cnts[0]) {
         c -= ngrp;
@@ -129,8 +131,7 @@ __global__ void __launch_bounds__(512) smooth_strip_kernel(LevelArgs<T> L, cx<T>
             if (side == 0) {
                 run(fc);
             } else {
-                run(load_line_lu<T, R>(
-                    lu + (static_cast<size_t>(k) * 2 + 1) * kLuFields * L.lus, L.lus, j0, ny));
+                run(load_line_lu_buf<T, R>(lr, (2 * k + 1) * tbl, L.lus, j0));
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) Scur[r] = Snext[r];
