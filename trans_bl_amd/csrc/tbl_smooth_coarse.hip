@@ -216,8 +216,10 @@ __device__ __forceinline__ void coarse_chain(const T *rb, T *col, const T *f, T 
     for (int j = 1; j < NY; ++j) col[2 * j] = fma(rf, Y[j] - col[2 * j], col[2 * j]);
 }
 
+constexpr int kCoarse2sThreads = 512;
+
 template <typename T, int NY>
-__global__ void __launch_bounds__(256) smooth_coarsest_2s_kernel(LevelArgs<T> L, cx<T> *V,
+__global__ void __launch_bounds__(kCoarse2sThreads) smooth_coarsest_2s_kernel(LevelArgs<T> L, cx<T> *V,
                                                                   const cx<T> *S, const T *lu,
                                                                   int sweeps, T rf,
                                                                   const int *active)
@@ -249,7 +251,7 @@ __global__ void __launch_bounds__(256) smooth_coarsest_2s_kernel(LevelArgs<T> L,
     // the reference's row forms exactly (zero terms add nothing, and the
     // two-term sum of row 2 only commutes)
     constexpr int RW = NY <= 32 ? 32 : 64;
-    constexpr int NG = 256 / RW;
+    constexpr int NG = kCoarse2sThreads / RW;
     const int jr = threadIdx.x % RW, cs = threadIdx.x / RW;
     T ca = T(0), cb = T(1), cc = T(0);
     if (jr == 1) {
@@ -352,12 +354,13 @@ int smooth_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu
     if (G != kCoarseSeqShape) reg_ny = false;
     G = sweep_shape(G);
     if (reg_ny && coarse_2s_lds_bytes<T>(L.nx, L.ny) <= 160 * 1024 &&
-        2 * phase_lines(1, L.nx) <= 256) {
+        2 * phase_lines(1, L.nx) <= kCoarse2sThreads) {
         const size_t shm = coarse_2s_lds_bytes<T>(L.nx, L.ny);
         bool done = false;
 #define X(NY)                                                                                 \
     if (!done && L.ny == NY) {                                                                \
-        hipLaunchKernelGGL((smooth_coarsest_2s_kernel<T, NY>), dim3(L.K), dim3(256), shm, st, \
+        hipLaunchKernelGGL((smooth_coarsest_2s_kernel<T, NY>), dim3(L.K), dim3(kCoarse2sThreads), \
+                           shm, st,                                                           \
                            L, V, S, lu, sweeps, rf, active);                                  \
         done = true;                                                                          \
     }
