@@ -180,21 +180,24 @@ __global__ void __launch_bounds__(256) smooth_coarsest_reg_kernel(LevelArgs<T> L
 // factors being real -- run the two substitutions and the relaxation.  The
 // substitutions are contracted to fused multiply-adds (one dependent
 // operation per row and direction); the right-hand sides keep the
-// reference's operation order.  v, the source, the right-hand sides and
-// the factors are all in LDS.
+// reference's operation order.  v, the source and the right-hand sides are
+// in LDS; the factors, the same for every line of a phase, are read from
+// global memory at wavefront-uniform addresses (scalar loads, no LDS
+// traffic in the chains).
 template <typename T>
 __host__ __device__ inline size_t coarse_2s_lds_bytes(int nx, int ny)
 {
     const size_t ls = static_cast<size_t>(coarse_stride(ny));
-    return (2 * ls * nx + ls * phase_lines(1, nx)) * sizeof(cx<T>) +
-           static_cast<size_t>(2 * kLuFields) * ny * sizeof(T);
+    return (2 * ls * nx + ls * phase_lines(1, nx)) * sizeof(cx<T>);
 }
 
-// one part (real or imaginary; element stride 2) of one line
+// one part (real or imaginary; element stride 2) of one line; f: the
+// factors' fields, fs apart
 template <typename T, int NY>
-__device__ __forceinline__ void coarse_chain(const T *rb, T *col, const T *f, T rf)
+__device__ __forceinline__ void coarse_chain(const T *rb, T *col, const T *__restrict__ f, int fs,
+                                             T rf)
 {
-    const T *l1 = f, *l2 = f + NY, *m1 = f + 5 * NY, *m2 = f + 6 * NY, *rp = f + 7 * NY;
+    const T *l1 = f, *l2 = f + fs, *m1 = f + 5 * fs, *m2 = f + 6 * fs, *rp = f + 7 * fs;
     T Y[NY];
     T y1 = T(0), y2 = T(0);
     // y_j = (r_j - l2_j y_{j-2}) - l1_j y_{j-1}
@@ -219,8 +222,9 @@ __device__ __forceinline__ void coarse_chain(const T *rb, T *col, const T *f, T 
 constexpr int kCoarse2sThreads = 512;
 
 template <typename T, int NY>
-__global__ void __launch_bounds__(kCoarse2sThreads) smooth_coarsest_2s_kernel(LevelArgs<T> L, cx<T> *V,
-                                                                  const cx<T> *S, const T *lu,
+__global__ void __launch_bounds__(kCoarse2sThreads) smooth_coarsest_2s_kernel(LevelArgs<T> L, cx<T> *__restrict__ V,
+                                                                  const cx<T> *__restrict__ S,
+                                                                  const T *__restrict__ lu,
                                                                   int sweeps, T rf,
                                                                   const int *active)
 {
@@ -232,7 +236,6 @@ __global__ void __launch_bounds__(kCoarse2sThreads) smooth_coarsest_2s_kernel(Le
     cx<T> *sv = reinterpret_cast<cx<T> *>(smem_raw);
     cx<T> *ss = sv + static_cast<size_t>(LS) * nx;
     cx<T> *rbuf = ss + static_cast<size_t>(LS) * nx;
-    T *slu = reinterpret_cast<T *>(rbuf + static_cast<size_t>(LS) * phase_lines(1, nx));
     cx<T> *gv = V + static_cast<size_t>(k) * L.plane;
     const cx<T> *gs = S + static_cast<size_t>(k) * L.plane;
     const T *glu = lu + (static_cast<size_t>(k) * 2) * kLuFields * L.lus;
@@ -241,8 +244,6 @@ __global__ void __launch_bounds__(kCoarse2sThreads) smooth_coarsest_2s_kernel(Le
         sv[i * LS + j] = gv[static_cast<size_t>(i) * ld + j];
         ss[i * LS + j] = gs[static_cast<size_t>(i) * ld + j];
     }
-    for (int e = threadIdx.x; e < 2 * kLuFields * NY; e += blockDim.x)
-        slu[e] = glu[static_cast<size_t>(e / NY) * L.lus + e % NY];
     __syncthreads();
     const T qx = L.qx;
     // stage-1 lane roles for the colour phases: RW lanes per row set, NG line
@@ -317,7 +318,7 @@ __global__ void __launch_bounds__(kCoarse2sThreads) smooth_coarsest_2s_kernel(Le
                 const int i = phase_column(phase, c, nx);
                 coarse_chain<T, NY>(reinterpret_cast<const T *>(rbuf + c * LS) + part,
                                     reinterpret_cast<T *>(sv + static_cast<size_t>(i) * LS) + part,
-                                    slu + (phase == 0 ? kLuFields * NY : 0), rf);
+                                    glu + (phase == 0 ? kLuFields * L.lus : 0), L.lus, rf);
             }
             __syncthreads();
         }
