@@ -126,10 +126,12 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
                 xs = T(10) * w3 - T(15) * w2 - T(4) * w1 + T(14) * w0 - T(6) * vload(nx - 5)
                      + vload(nx - 6);
             } else {
-                xs = T(-1) * w0 + T(16) * w1 - T(30) * w2 + T(16) * w3 - T(1) * w4;
+                // -w0 + 16 w1 - 30 w2 + 16 w3 - w4 (multiply-adds in FP32)
+                xs = mac(T(-1), w4, mac(T(16), w3, mac(T(-30), w2, mac(T(16), w1, T(-1) * w0))));
             }
             auto own_x = [&]() {
-                return border ? xs : T(-1) * (w0 + w4) + T(16) * (w1 + w3) - T(30) * w2;
+                // -(w0 + w4) + 16 (w1 + w3) - 30 w2
+                return border ? xs : mac(T(-30), w2, dot2(T(-1), w0 + w4, T(16), w1 + w3));
             };
             // y-neighbours from the adjacent lanes by DPP wavefront shifts
             // (no LDS round trip); the edge lanes 0, 1, 62, 63 are halo rows
@@ -146,8 +148,9 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
                     const cx<T> d = k2b2 * (L.sp[1] * w2 + L.sp[2] * vp1);
                     lv = b * iscx + c * q + d * ifac;
                 } else if (j <= ny - 3) {
-                    lv = ax0 * xm + ax1 * xs + ax2 * xp + cv0 * vm2 + cv1 * vm1 + cv2 * w2
-                         + cv3 * vp1 + cv4 * vp2;
+                    // ax0 xm + ax1 xs + ax2 xp + cv0 vm2 + ... + cv4 vp2, summed left to right
+                    lv = mac(cv4, vp2, mac(cv3, vp1, mac(cv2, w2, mac(cv1, vm1, mac(cv0, vm2,
+                             mac(ax2, xp, dot2(ax0, xm, ax1, xs)))))));
                 } else if (j == ny - 2) {
                     const cx<T> c = L.pp[3] * vp1 + L.pp[2] * w2 + L.pp[1] * vm1;
                     const cx<T> d = k2b2 * w2;
@@ -158,7 +161,7 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
                     lv = own_x() * iscx + c * q + d * ifac - L.lp[1] * alpha2 * w2 * L.sqq_top;
                 }
                 const size_t o = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * ld + j;
-                const cx<T> r = s_cur + si * lv;
+                const cx<T> r = mac(si, lv, s_cur);   // s + si lv
                 out[o] = cx<TO>{static_cast<TO>(r.re), static_cast<TO>(r.im)};
                 if (resmax) {
                     // max of |r|^2; one square root per wavefront below

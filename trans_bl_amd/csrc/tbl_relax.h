@@ -19,6 +19,51 @@
 
 namespace tbl {
 
+template <typename T>
+constexpr bool kIsF32 = sizeof(T) == 4;
+
+// Multiply-add forms with a precision-dependent rounding.  FP64 rounds every
+// product and sum in the order the reference writes them (the results below
+// are bit-identical to the plain expressions in the comments); FP32 -- the
+// correction solve under FP64 defects -- fuses them into multiply-adds (one
+// rounding each, and about half the vector instructions).
+//   mac(a, x, c)           = a x + c
+//   dot2(a, x, b, y)       = a x + b y
+//   dot2c(a, x, b, y, c)   = (a x + b y) + c
+template <typename T>
+__device__ __forceinline__ T mac(T a, T x, T c)
+{
+    if constexpr (kIsF32<T>) return __builtin_fmaf(a, x, c);
+    else return a * x + c;
+}
+template <typename T>
+__device__ __forceinline__ cx<T> mac(T a, cx<T> x, cx<T> c)
+{
+    return {mac(a, x.re, c.re), mac(a, x.im, c.im)};
+}
+template <typename T>
+__device__ __forceinline__ T dot2(T a, T x, T b, T y)
+{
+    if constexpr (kIsF32<T>) return __builtin_fmaf(b, y, a * x);
+    else return a * x + b * y;
+}
+template <typename T>
+__device__ __forceinline__ cx<T> dot2(T a, cx<T> x, T b, cx<T> y)
+{
+    return {dot2(a, x.re, b, y.re), dot2(a, x.im, b, y.im)};
+}
+template <typename T>
+__device__ __forceinline__ T dot2c(T a, T x, T b, T y, T c)
+{
+    if constexpr (kIsF32<T>) return __builtin_fmaf(a, x, __builtin_fmaf(b, y, c));
+    else return (a * x + b * y) + c;
+}
+template <typename T>
+__device__ __forceinline__ cx<T> dot2c(T a, cx<T> x, T b, cx<T> y, cx<T> c)
+{
+    return {dot2c(a, x.re, b, y.re, c.re), dot2c(a, x.im, b, y.im, c.im)};
+}
+
 // ---------------------------------------------------------------------------
 // x-direction stencils (poisson_par.f:343-540 and 543-793)
 // ---------------------------------------------------------------------------
@@ -31,18 +76,18 @@ __device__ __forceinline__ cx<T> xoff(const cx<T> *__restrict__ col, int ld, int
                                       int side, bool grouped)
 {
     // col points at column i of the mode's plane; neighbours at +/- ld
-    if (side < 0) {
-        return T(10) * col[j - ld] - T(4) * col[j + ld] + T(14) * col[j + 2 * ld]
-               - T(6) * col[j + 3 * ld] + col[j + 4 * ld];
-    }
-    if (side > 0) {
-        return T(10) * col[j + ld] - T(4) * col[j - ld] + T(14) * col[j - 2 * ld]
-               - T(6) * col[j - 3 * ld] + col[j - 4 * ld];
+    // sums in the reference's order: 10 a - 4 b + 14 c - 6 d + e (border),
+    // -(p2 + m2) + 16 (p1 + m1) (grouped), -m2 + 16 m1 + 16 p1 - p2
+    if (side != 0) {
+        const int s = side < 0 ? ld : -ld;
+        return mac(T(-6), col[j + 3 * s],
+                   mac(T(14), col[j + 2 * s], mac(T(-4), col[j + s], T(10) * col[j - s])))
+               + col[j + 4 * s];
     }
     if (grouped)
-        return -(col[j + 2 * ld] + col[j - 2 * ld]) + T(16) * (col[j + ld] + col[j - ld]);
-    return T(-1) * col[j - 2 * ld] + T(16) * col[j - ld] + T(16) * col[j + ld]
-           - T(1) * col[j + 2 * ld];
+        return mac(T(16), col[j + ld] + col[j - ld], -(col[j + 2 * ld] + col[j - 2 * ld]));
+    return mac(T(-1), col[j + 2 * ld],
+               mac(T(16), col[j + ld], mac(T(16), col[j - ld], T(-1) * col[j - 2 * ld])));
 }
 
 // ---------------------------------------------------------------------------
@@ -52,8 +97,6 @@ __device__ __forceinline__ cx<T> xoff(const cx<T> *__restrict__ col, int ld, int
 // (forward multipliers), u1, u2, pivot (back substitution, as banbk5_poi
 // uses them) and -u1/p, -u2/p, 1/p (the scan's map composition).  Rows past
 // ny get the identity (zeros, unit pivot).
-template <typename T>
-constexpr bool kIsF32 = sizeof(T) == 4;
 
 template <typename T, int R>
 struct LineLU {
@@ -147,11 +190,12 @@ __device__ __forceinline__ Aff<T> aff_identity()
 template <typename T>
 __device__ __forceinline__ void aff_fold(Aff<T> &a, const Aff<T> &q)
 {
-    const cx<T> nc0 = a.p00 * q.c0 + a.p01 * q.c1 + a.c0;
-    const cx<T> nc1 = a.p10 * q.c0 + a.p11 * q.c1 + a.c1;
+    // c0 = P00 q.c0 + P01 q.c1 + c0, ..., P = P_a P_q
+    const cx<T> nc0 = dot2c(a.p00, q.c0, a.p01, q.c1, a.c0);
+    const cx<T> nc1 = dot2c(a.p10, q.c0, a.p11, q.c1, a.c1);
     a.c0 = nc0; a.c1 = nc1;
-    const T m00 = a.p00 * q.p00 + a.p01 * q.p10, m01 = a.p00 * q.p01 + a.p01 * q.p11;
-    const T m10 = a.p10 * q.p00 + a.p11 * q.p10, m11 = a.p10 * q.p01 + a.p11 * q.p11;
+    const T m00 = dot2(a.p00, q.p00, a.p01, q.p10), m01 = dot2(a.p00, q.p01, a.p01, q.p11);
+    const T m10 = dot2(a.p10, q.p00, a.p11, q.p10), m11 = dot2(a.p10, q.p01, a.p11, q.p11);
     a.p00 = m00; a.p01 = m01; a.p10 = m10; a.p11 = m11;
 }
 
@@ -274,11 +318,14 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const LineLU<T
             if (j == 0) {
                 rhs = v;
             } else if (j == 1) {
-                rhs = getS(j, r) - qx * (L.sp[2] * X[r + 2] + L.sp[1] * X[r + 1]);
+                // S - qx (sp2 X(j+1) + sp1 X(j))
+                rhs = mac(-qx, dot2(L.sp[2], X[r + 2], L.sp[1], X[r + 1]), getS(j, r));
             } else if (j <= ny - 3) {
-                rhs = getS(j, r) - qx * (L.cp[0] * X[r] + L.cp[1] * X[r + 1] + L.cp[2] * X[r + 2]);
+                // S - qx ((cp0 X(j-1) + cp1 X(j)) + cp2 X(j+1))
+                rhs = mac(-qx, mac(L.cp[2], X[r + 2], dot2(L.cp[0], X[r], L.cp[1], X[r + 1])),
+                          getS(j, r));
             } else {
-                rhs = getS(j, r) - qx * X[r + 1];
+                rhs = mac(-qx, X[r + 1], getS(j, r));
             }
         }
         vo[r] = v;
@@ -291,9 +338,10 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const LineLU<T
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const T a1 = f.l1[r], a2 = f.l2[r];
-        const T n10 = -a2 * m.p00 - a1 * m.p10, n11 = -a2 * m.p01 - a1 * m.p11;
+        // -a2 p0x - a1 p1x; (y - a2 c0) - a1 c1
+        const T n10 = dot2(-a2, m.p00, -a1, m.p10), n11 = dot2(-a2, m.p01, -a1, m.p11);
         m.p00 = m.p10; m.p01 = m.p11; m.p10 = n10; m.p11 = n11;
-        const cx<T> nc1 = (y[r] - a2 * m.c0) - a1 * m.c1;
+        const cx<T> nc1 = mac(-a1, m.c1, mac(-a2, m.c0, y[r]));
         m.c0 = m.c1; m.c1 = nc1;
     }
     scan_up<T, G>(m, lane);
@@ -304,7 +352,7 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const LineLU<T
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const T a1 = f.l1[r], a2 = f.l2[r];
-            const cx<T> yy = (y[r] - a2 * a) - a1 * b;
+            const cx<T> yy = mac(-a1, b, mac(-a2, a, y[r]));   // (y - a2 a) - a1 b
             a = b; b = yy;
             y[r] = yy;
         }
@@ -318,9 +366,10 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const LineLU<T
         // composition only: precomputed -u1/p, -u2/p, 1/p (the replay below
         // divides by the pivot like the reference)
         const T m0 = f.m1[r], m1 = f.m2[r], rp = f.rp[r];
-        const T n00 = m0 * m.p00 + m1 * m.p10, n01 = m0 * m.p01 + m1 * m.p11;
+        const T n00 = dot2(m0, m.p00, m1, m.p10), n01 = dot2(m0, m.p01, m1, m.p11);
         m.p10 = m.p00; m.p11 = m.p01; m.p00 = n00; m.p01 = n01;
-        const cx<T> nc0 = rp * y[r] + m0 * m.c0 + m1 * m.c1;
+        // (rp y + m0 c0) + m1 c1
+        const cx<T> nc0 = mac(m1, m.c1, dot2(rp, y[r], m0, m.c0));
         m.c1 = m.c0; m.c0 = nc0;
     }
     scan_down<T, G>(m, lane);
@@ -333,7 +382,7 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const LineLU<T
             // FP64 divides by the pivot as banbk5_poi does; the FP32
             // smoother (a correction solve under FP64 defects) multiplies
             // by the reciprocal instead of a ~10-instruction division
-            const cx<T> t = (y[r] - w1 * a) - w2 * b;
+            const cx<T> t = mac(-w2, b, mac(-w1, a, y[r]));   // (y - w1 a) - w2 b
             const cx<T> xx = kIsF32<T> ? t * f.rp[r] : t / pj;
             b = a; a = xx;
             y[r] = xx;
@@ -343,7 +392,7 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const LineLU<T
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int j = j0 + r;
-        if (j >= 1 && j < ny) putV(j, vo[r] + rf * (y[r] - vo[r]));
+        if (j >= 1 && j < ny) putV(j, mac(rf, y[r] - vo[r], vo[r]));
     }
 }
 

@@ -95,12 +95,14 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
             const int j = lane + r * G;
             if (j < ny) {
                 cx<T> x;
+                // the sums of xoff in its order (multiply-adds in FP32)
                 if (side)
-                    x = T(10) * a0[r] - T(4) * a1[r] + T(14) * a2[r] - T(6) * a3[r] + a4[r];
+                    x = mac(T(-6), a3[r], mac(T(14), a2[r], mac(T(-4), a1[r], T(10) * a0[r])))
+                        + a4[r];
                 else if (j >= ny - 2)
-                    x = -(a3[r] + a0[r]) + T(16) * (a2[r] + a1[r]);
+                    x = mac(T(16), a2[r] + a1[r], -(a3[r] + a0[r]));
                 else
-                    x = T(-1) * a0[r] + T(16) * a1[r] + T(16) * a2[r] - T(1) * a3[r];
+                    x = mac(T(-1), a3[r], mac(T(16), a2[r], mac(T(16), a1[r], T(-1) * a0[r])));
                 Xs[j] = x;
                 if (FULL) {
                     Vs[j] = vv[r];
