@@ -193,7 +193,8 @@ int tbl_get_line_shape(tbl_ctx *ctx, int prec, int level);
    by hipEvents on the context's stream; *ms receives the elapsed time and
    *launches the number of smoother kernel launches it made.  prec 0 times
    the FP64 smoother, 1 the FP32 one.  On the coarsest level it times that
-   level's solve as the V-cycle runs it (all sweeps in one launch, FP64). */
+   level's solve as the V-cycle of that precision runs it (all sweeps in one
+   launch). */
 int tbl_time_smoother(tbl_ctx *ctx, int prec, int level, int sweeps, double *ms,
                       int *launches);
 

@@ -211,22 +211,31 @@ def test_staged_smoother_matches_direct(torch_mod, precision, form, grid):
 
 @pytest.mark.parametrize("grid", [SMALL, CFG1], ids=["small", "385x97x8"])
 @pytest.mark.parametrize("lanes", [16, 32])
-def test_coarsest_scan_matches_sequential(torch_mod, grid, lanes):
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+def test_coarsest_scan_matches_sequential(torch_mod, grid, lanes, precision):
     """The coarsest level's lane-scan kernel (shape G) and its sequential
     kernels (shape 1) solve the same lines: same converged solution, iterates
-    within rounding, FP64."""
+    within rounding -- in the FP64 cycle and in the FP32 correction cycle
+    (whose coarsest sweeps run in FP32: iterates within FP32 rounding of the
+    corrections, solutions within the mixed path's 1e-8)."""
     f = wl.wavepacket_rhs(grid, tag="coarse")
     z = np.zeros_like(f)
+    kw = dict(precision=precision)
     with context(grid) as ctx:
-        assert ctx.line_shape(0, grid.levels) == 1
-        a, ca, _, _ = ctx.poi_uy(None, -f, z, 1e-9, max_cycles=20)
-        a2, _, _, _ = ctx.poi_uy(None, -f, z, 0.0, max_cycles=2)
-        ctx.set_line_shape(0, grid.levels, lanes)
-        b, cb, _, _ = ctx.poi_uy(None, -f, z, 1e-9, max_cycles=20)
-        b2, _, _, _ = ctx.poi_uy(None, -f, z, 0.0, max_cycles=2)
-    assert rel(b2, a2) < 1e-12, rel(b2, a2)
-    assert np.array_equal(ca, cb)
-    assert rel(b, a) < 1e-12, rel(b, a)
+        assert ctx.line_shape(precision, grid.levels) == 1
+        a, ca, _, _ = ctx.poi_uy(None, -f, z, 1e-9, max_cycles=20, **kw)
+        a2, _, _, _ = ctx.poi_uy(None, -f, z, 0.0, max_cycles=2, **kw)
+        ctx.set_line_shape(precision, grid.levels, lanes)
+        b, cb, _, _ = ctx.poi_uy(None, -f, z, 1e-9, max_cycles=20, **kw)
+        b2, _, _, _ = ctx.poi_uy(None, -f, z, 0.0, max_cycles=2, **kw)
+    if precision == 0:
+        assert rel(b2, a2) < 1e-12, rel(b2, a2)
+        assert np.array_equal(ca, cb)
+        assert rel(b, a) < 1e-12, rel(b, a)
+    else:
+        assert rel(b2, a2) < 1e-5, rel(b2, a2)
+        assert abs(int(cb.max()) - int(ca.max())) <= 1
+        assert rel(b, a) < 1e-8, rel(b, a)
 
 
 def _strip_shapes(ctx, g):

@@ -38,11 +38,6 @@ def tune(ctx, sweeps: int = 6, levels=None, precisions=(0, 1), verbose=False):
     for lvl in levels:
         entry = {"level": lvl, "us_per_sweep": {}}
         for prec in precisions:
-            if prec == 1 and lvl == ctx.grid.levels and entry.get("fp64") is not None:
-                # the coarsest solve runs in FP64 in both cycles
-                ctx.set_line_shape(1, lvl, entry["fp64"])
-                entry["fp32"] = entry["fp64"]
-                continue
             best, best_t = None, None
             orig = ctx.line_shape(prec, lvl)
             for lanes in CANDIDATES:

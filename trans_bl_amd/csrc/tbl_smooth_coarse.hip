@@ -441,5 +441,7 @@ int coarse_kernel_attributes()
 
 template int smooth_coarsest<double>(const LevelArgs<double> &, cx<double> *, const cx<double> *,
                                      const double *, int, double, const int *, int, hipStream_t);
+template int smooth_coarsest<float>(const LevelArgs<float> &, cx<float> *, const cx<float> *,
+                                    const float *, int, float, const int *, int, hipStream_t);
 
 }  // namespace tbl

@@ -422,13 +422,9 @@ int mixed_cycle(tbl_ctx *c, const Sched &s, double tol)
         restrict_level<float>(c->lv[l], c->lv[l + 1], K, c->E32[l], c->R32[l], c->E32[l + 1],
                               nullptr, c->S32[l + 1], 0, c->d_active, st);
     }
-    // coarsest correction solved in FP64
-    const size_t nc = c->lv[L - 1].plane * K;
-    convert<float, double>(nc, c->S32[L - 1], c->R[L - 1], 0, st);
-    convert<double, double>(nc, c->W[L - 1], c->W[L - 1], 1, st);
-    TRY(smooth_coarsest<double>(c->la_d[L - 1], c->W[L - 1], c->R[L - 1], c->lu_d[L - 1], s.nc,
-                                rf_of(s.nc), c->d_active, c->G64[L - 1], st));
-    convert<double, float>(nc, c->W[L - 1], c->E32[L - 1], 0, st);
+    // coarsest correction in FP32 too, from the zero the restriction left
+    TRY(smooth_coarsest<float>(c->la_f[L - 1], c->E32[L - 1], c->S32[L - 1], c->lu_f[L - 1], s.nc,
+                               static_cast<float>(rf_of(s.nc)), c->d_active, c->G32[L - 1], st));
     for (int l = L - 1; l >= 1; --l) {
         prolong_level<float>(c->lv[l - 1], c->lv[l], K, c->E32[l - 1], c->E32[l], nullptr, 0,
                              c->d_active, st);
@@ -661,8 +657,11 @@ extern "C" int tbl_time_smoother(tbl_ctx *c, int prec, int level, int sweeps, do
     const int l = level - 1;
     hipStream_t st = c->stream;
     TBL_HIP(hipEventRecord(c->ev0, st));
-    if (level == c->L)
-        // the coarsest level's solve (FP64 in both precisions), one launch
+    if (level == c->L && prec)
+        // the coarsest level's solve as each cycle runs it, one launch
+        TRY(smooth_coarsest<float>(c->la_f[l], c->E32[l], c->S32[l], c->lu_f[l], sweeps, 1.0f,
+                                   nullptr, c->G32[l], st));
+    else if (level == c->L)
         TRY(smooth_coarsest<double>(c->la_d[l], c->V[l], c->S[l], c->lu_d[l], sweeps, 1.0,
                                     nullptr, c->G64[l], st));
     else if (prec)
