@@ -81,12 +81,15 @@ int phase_kernel_attributes();
 int strip_kernel_attributes();
 int coarse_kernel_attributes();
 
+// zero_start: the sweeps start from v = 0 whatever v's memory holds (the
+// first sweep reads no column it has not relaxed itself); staged shapes
+// (G < 0) only, any other shape returns TBL_ERR_ARG
 template <typename T>
 int smooth_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
-                  const int *active, int G, cx<T> *halo, hipStream_t st);
+                  const int *active, int G, cx<T> *halo, hipStream_t st, bool zero_start = false);
 template <typename T>
 int phase_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
-                 const int *active, int G, hipStream_t st);
+                 const int *active, int G, hipStream_t st, bool zero_start = false);
 template <typename T>
 int strip_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
                  const int *active, int G, cx<T> *halo, hipStream_t st);

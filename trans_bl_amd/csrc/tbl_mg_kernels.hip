@@ -483,11 +483,12 @@ int lu_stride(int ny)
 // the x terms staged in LDS
 template <typename T>
 int smooth_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
-                  const int *active, int G, cx<T> *halo, hipStream_t st)
+                  const int *active, int G, cx<T> *halo, hipStream_t st, bool zero_start)
 {
+    if (zero_start && G >= 0) return TBL_ERR_ARG;
     if (G > kStripShape && G < kXStagedShape)
         return strip_sweeps<T>(L, V, S, lu, sweeps, rf, active, G - kStripShape, halo, st);
-    return phase_sweeps<T>(L, V, S, lu, sweeps, rf, active, G, st);
+    return phase_sweeps<T>(L, V, S, lu, sweeps, rf, active, G, st, zero_start);
 }
 
 template <typename T, typename TO>
@@ -577,10 +578,10 @@ void convert(size_t n, const cx<TA> *a, cx<TB> *b, int zero, hipStream_t st)
 // explicit instantiations
 template int smooth_sweeps<double>(const LevelArgs<double> &, cx<double> *, const cx<double> *,
                                    const double *, int, double, const int *, int, cx<double> *,
-                                   hipStream_t);
+                                   hipStream_t, bool);
 template int smooth_sweeps<float>(const LevelArgs<float> &, cx<float> *, const cx<float> *,
                                   const float *, int, float, const int *, int, cx<float> *,
-                                  hipStream_t);
+                                  hipStream_t, bool);
 template void defect<double, double>(const LevelArgs<double> &, const cx<double> *,
                                      const cx<double> *, cx<double> *, double, const int *,
                                      double *, hipStream_t);

@@ -35,7 +35,8 @@ constexpr int kStagedLines = 4;   // lines per workgroup of the staged smoother
 template <typename T, int G, int R, bool FULL>
 __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T> *V,
                                                             const cx<T> *S, const T *lu,
-                                                            int phase, T rf, const int *active)
+                                                            int phase, T rf, const int *active,
+                                                            int zs)
 {
     extern __shared__ unsigned char smem_raw[];
     const int LDW = L.ld;                                // rows held per line
@@ -69,13 +70,27 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
         // past ny read padding or zero and are not used.  xoff's arithmetic
         // follows.
         const int sg = -side;
-        const int o0 = side ? -sg * ld : -2 * ld, o1 = side ? sg * ld : -ld;
-        const int o2 = side ? 2 * sg * ld : ld, o3 = side ? 3 * sg * ld : 2 * ld;
-        const int o4 = side ? 4 * sg * ld : 2 * ld;
+        // neighbour columns i + d0 .. i + d4
+        const int d0 = side ? -sg : -2, d1 = side ? sg : -1, d2 = side ? 2 * sg : 1,
+                  d3 = side ? 3 * sg : 2, d4 = side ? 4 * sg : 2;
         constexpr unsigned ES = sizeof(cx<T>);
         const unsigned lo = static_cast<unsigned>(((k - kf) * L.plane + static_cast<size_t>(i) * ld + lane) * ES);
-        const unsigned b0 = lo + o0 * ES, b1 = lo + o1 * ES, b2 = lo + o2 * ES, b3 = lo + o3 * ES,
-                       b4 = lo + o4 * ES;
+        // zero start (zs: the first sweep from v = 0): a column this sweep
+        // has not relaxed yet -- the boundary columns, the border columns
+        // before phase 1, a colour before its phase, the line's own column
+        // -- reads as zero through an out-of-range offset, not from memory
+        auto col_off = [&](int d) -> unsigned {
+            const int c = i + d;
+            bool known = true;
+            if (zs) {
+                if (d == 0 || c <= 0 || c >= L.nx - 1) known = false;
+                else if (c == 1 || c == L.nx - 2) known = phase >= 1;
+                else known = (c - 2) % 3 + 1 < phase;
+            }
+            return known ? lo + static_cast<unsigned>(d * ld) * ES : kBufOff;
+        };
+        const unsigned b0 = col_off(d0), b1 = col_off(d1), b2 = col_off(d2), b3 = col_off(d3),
+                       b4 = col_off(d4), bv = col_off(0);
         cx<T> a0[R], a1[R], a2[R], a3[R], a4[R], vv[R], ss[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -86,7 +101,7 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
             a3[r] = buf_load(vr, b3 + rs, T(0));
             a4[r] = buf_load(vr, b4 + rs, T(0));
             if (FULL) {
-                vv[r] = buf_load(vr, lo + rs, T(0));
+                vv[r] = buf_load(vr, bv + rs, T(0));
                 ss[r] = buf_load(sr, lo + rs, T(0));
             }
         }
@@ -162,14 +177,14 @@ static size_t staged_lds(int ld, size_t esize)
 
 template <typename T, int G, int R, bool FULL>
 static void launch_staged(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int phase,
-                          T rf, const int *active, hipStream_t st)
+                          T rf, const int *active, hipStream_t st, int zs)
 {
     const size_t lines = static_cast<size_t>(phase_lines(phase, L.nx)) * L.K;
     if (!lines) return;
     hipLaunchKernelGGL((smooth_staged_kernel<T, G, R, FULL>),
                        dim3(static_cast<unsigned>((lines + kStagedLines - 1) / kStagedLines)),
                        dim3(kStagedLines * G), staged_lds<FULL>(L.ld, sizeof(cx<T>)), st, L, V, S,
-                       lu, phase, rf, active);
+                       lu, phase, rf, active, zs);
 }
 
 // One colour phase per launch.  G > 0: lanes per line with direct
@@ -178,9 +193,10 @@ static void launch_staged(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const
 // only the x terms staged.
 template <typename T>
 int phase_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
-                 const int *active, int G, hipStream_t st)
+                 const int *active, int G, hipStream_t st, bool zero_start)
 {
     const int form = G < 0 ? 1 : G > kXStagedShape ? 2 : 0;
+    if (zero_start && form != 1) return TBL_ERR_ARG;
     G = form == 1 ? -G : form == 2 ? G - kXStagedShape : G;
     const int R = line_rows_for(G, L.ny);
     if (R < 0) return TBL_ERR_GRID;
@@ -189,10 +205,11 @@ int phase_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, i
     for (int it = 0; it < sweeps; ++it)
         for (int ph = 0; ph < 4; ++ph) {
             bool done = false;
+            const int zs = zero_start && it == 0;
 #define X(GG, RR)                                                                       \
     if (!done && G == GG && R == RR) {                                                  \
-        if (form == 1) launch_staged<T, GG, RR, true>(L, V, S, lu, ph, rf, active, st);  \
-        else if (form == 2) launch_staged<T, GG, RR, false>(L, V, S, lu, ph, rf, active, st); \
+        if (form == 1) launch_staged<T, GG, RR, true>(L, V, S, lu, ph, rf, active, st, zs); \
+        else if (form == 2) launch_staged<T, GG, RR, false>(L, V, S, lu, ph, rf, active, st, 0); \
         else launch_phase<T, GG, RR>(L, V, S, lu, ph, rf, active, st);                  \
         done = true;                                                                    \
     }
@@ -221,8 +238,8 @@ int phase_kernel_attributes()
 }
 
 template int phase_sweeps<double>(const LevelArgs<double> &, cx<double> *, const cx<double> *,
-                                  const double *, int, double, const int *, int, hipStream_t);
+                                  const double *, int, double, const int *, int, hipStream_t, bool);
 template int phase_sweeps<float>(const LevelArgs<float> &, cx<float> *, const cx<float> *,
-                                 const float *, int, float, const int *, int, hipStream_t);
+                                 const float *, int, float, const int *, int, hipStream_t, bool);
 
 }  // namespace tbl

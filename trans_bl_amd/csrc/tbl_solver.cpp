@@ -412,11 +412,16 @@ int mixed_cycle(tbl_ctx *c, const Sched &s, double tol)
     defect<double, float>(c->la_d[0], c->V[0], c->S[0], c->S32[0], -1.0, c->d_active, c->d_resmax, st);
     hipLaunchKernelGGL(decide2_kernel, dim3((K + 255) / 256), dim3(256), 0, st, K, c->d_resmax,
                        tol, c->d_cycle, c->d_active, c->d_cycles, c->d_lastres, c->d_nactive);
-    convert<float, float>(c->lv[0].plane * K, c->E32[0], c->E32[0], 1, st);
+    // e starts at zero: the staged smoother's first sweep reads no column it
+    // has not relaxed itself (e's boundary columns are never written and its
+    // wall row only ever receives zeros, both zero since allocation); the
+    // other shapes clear e first
+    const bool zs = c->G32[0] < 0;
+    if (!zs) convert<float, float>(c->lv[0].plane * K, c->E32[0], c->E32[0], 1, st);
     for (int l = 0; l < L - 1; ++l) {
         TRY(smooth_sweeps<float>(c->la_f[l], c->E32[l], c->S32[l], c->lu_f[l], s.nu1,
                                  static_cast<float>(rf_of(s.nu1)), c->d_active, c->G32[l],
-                                 c->H32[l], st));
+                                 c->H32[l], st, zs && l == 0));
         defect<float, float>(c->la_f[l], c->E32[l], c->S32[l], c->R32[l], -1.0f, c->d_active,
                              nullptr, st);
         restrict_level<float>(c->lv[l], c->lv[l + 1], K, c->E32[l], c->R32[l], c->E32[l + 1],
