@@ -483,12 +483,13 @@ int lu_stride(int ny)
 // the x terms staged in LDS
 template <typename T>
 int smooth_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
-                  const int *active, int G, cx<T> *halo, hipStream_t st, bool zero_start)
+                  const int *active, int G, cx<T> *halo, hipStream_t st, bool zero_start,
+                  cx<double> *vup)
 {
-    if (zero_start && G >= 0) return TBL_ERR_ARG;
+    if ((zero_start || vup) && G >= 0) return TBL_ERR_ARG;
     if (G > kStripShape && G < kXStagedShape)
         return strip_sweeps<T>(L, V, S, lu, sweeps, rf, active, G - kStripShape, halo, st);
-    return phase_sweeps<T>(L, V, S, lu, sweeps, rf, active, G, st, zero_start);
+    return phase_sweeps<T>(L, V, S, lu, sweeps, rf, active, G, st, zero_start, vup);
 }
 
 template <typename T, typename TO>
@@ -578,10 +579,10 @@ void convert(size_t n, const cx<TA> *a, cx<TB> *b, int zero, hipStream_t st)
 // explicit instantiations
 template int smooth_sweeps<double>(const LevelArgs<double> &, cx<double> *, const cx<double> *,
                                    const double *, int, double, const int *, int, cx<double> *,
-                                   hipStream_t, bool);
+                                   hipStream_t, bool, cx<double> *);
 template int smooth_sweeps<float>(const LevelArgs<float> &, cx<float> *, const cx<float> *,
                                   const float *, int, float, const int *, int, cx<float> *,
-                                  hipStream_t, bool);
+                                  hipStream_t, bool, cx<double> *);
 template void defect<double, double>(const LevelArgs<double> &, const cx<double> *,
                                      const cx<double> *, cx<double> *, double, const int *,
                                      double *, hipStream_t);

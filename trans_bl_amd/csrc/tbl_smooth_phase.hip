@@ -36,7 +36,7 @@ template <typename T, int G, int R, bool FULL>
 __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T> *V,
                                                             const cx<T> *S, const T *lu,
                                                             int phase, T rf, const int *active,
-                                                            int zs)
+                                                            int zs, cx<double> *vup)
 {
     extern __shared__ unsigned char smem_raw[];
     const int LDW = L.ld;                                // rows held per line
@@ -152,7 +152,18 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int j = lane + r * G;
-                if (j >= 1 && j < ny) col[j] = Vs[j];
+                if (j >= 1 && j < ny) {
+                    const cx<T> e = Vs[j];
+                    col[j] = e;
+                    // vup (the last sweep of a correction solve): this
+                    // line's e is final, so it is added to the FP64 field
+                    // here (same layout), as the update kernel would
+                    if (vup) {
+                        cx<double> &u = vup[base + j];
+                        u = cx<double>{u.re + static_cast<double>(e.re),
+                                       u.im + static_cast<double>(e.im)};
+                    }
+                }
             }
         }
     }
@@ -177,14 +188,14 @@ static size_t staged_lds(int ld, size_t esize)
 
 template <typename T, int G, int R, bool FULL>
 static void launch_staged(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int phase,
-                          T rf, const int *active, hipStream_t st, int zs)
+                          T rf, const int *active, hipStream_t st, int zs, cx<double> *vup)
 {
     const size_t lines = static_cast<size_t>(phase_lines(phase, L.nx)) * L.K;
     if (!lines) return;
     hipLaunchKernelGGL((smooth_staged_kernel<T, G, R, FULL>),
                        dim3(static_cast<unsigned>((lines + kStagedLines - 1) / kStagedLines)),
                        dim3(kStagedLines * G), staged_lds<FULL>(L.ld, sizeof(cx<T>)), st, L, V, S,
-                       lu, phase, rf, active, zs);
+                       lu, phase, rf, active, zs, vup);
 }
 
 // One colour phase per launch.  G > 0: lanes per line with direct
@@ -193,10 +204,10 @@ static void launch_staged(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const
 // only the x terms staged.
 template <typename T>
 int phase_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
-                 const int *active, int G, hipStream_t st, bool zero_start)
+                 const int *active, int G, hipStream_t st, bool zero_start, cx<double> *vup)
 {
     const int form = G < 0 ? 1 : G > kXStagedShape ? 2 : 0;
-    if (zero_start && form != 1) return TBL_ERR_ARG;
+    if ((zero_start || vup) && form != 1) return TBL_ERR_ARG;
     G = form == 1 ? -G : form == 2 ? G - kXStagedShape : G;
     const int R = line_rows_for(G, L.ny);
     if (R < 0) return TBL_ERR_GRID;
@@ -206,10 +217,11 @@ int phase_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, i
         for (int ph = 0; ph < 4; ++ph) {
             bool done = false;
             const int zs = zero_start && it == 0;
+            cx<double> *up = it == sweeps - 1 ? vup : nullptr;
 #define X(GG, RR)                                                                       \
     if (!done && G == GG && R == RR) {                                                  \
-        if (form == 1) launch_staged<T, GG, RR, true>(L, V, S, lu, ph, rf, active, st, zs); \
-        else if (form == 2) launch_staged<T, GG, RR, false>(L, V, S, lu, ph, rf, active, st, 0); \
+        if (form == 1) launch_staged<T, GG, RR, true>(L, V, S, lu, ph, rf, active, st, zs, up); \
+        else if (form == 2) launch_staged<T, GG, RR, false>(L, V, S, lu, ph, rf, active, st, 0, nullptr); \
         else launch_phase<T, GG, RR>(L, V, S, lu, ph, rf, active, st);                  \
         done = true;                                                                    \
     }
@@ -238,8 +250,10 @@ int phase_kernel_attributes()
 }
 
 template int phase_sweeps<double>(const LevelArgs<double> &, cx<double> *, const cx<double> *,
-                                  const double *, int, double, const int *, int, hipStream_t, bool);
+                                  const double *, int, double, const int *, int, hipStream_t, bool,
+                                  cx<double> *);
 template int phase_sweeps<float>(const LevelArgs<float> &, cx<float> *, const cx<float> *,
-                                 const float *, int, float, const int *, int, hipStream_t, bool);
+                                 const float *, int, float, const int *, int, hipStream_t, bool,
+                                 cx<double> *);
 
 }  // namespace tbl

@@ -430,14 +430,18 @@ int mixed_cycle(tbl_ctx *c, const Sched &s, double tol)
     // coarsest correction in FP32 too, from the zero the restriction left
     TRY(smooth_coarsest<float>(c->la_f[L - 1], c->E32[L - 1], c->S32[L - 1], c->lu_f[L - 1], s.nc,
                                static_cast<float>(rf_of(s.nc)), c->d_active, c->G32[L - 1], st));
+    // v += e: inside the finest level's last post-sweep on the staged shapes
+    // (each line adds its final e), else by the update kernel
+    const bool fused = zs && s.nu2 > 0;
     for (int l = L - 1; l >= 1; --l) {
         prolong_level<float>(c->lv[l - 1], c->lv[l], K, c->E32[l - 1], c->E32[l], nullptr, 0,
                              c->d_active, st);
         TRY(smooth_sweeps<float>(c->la_f[l - 1], c->E32[l - 1], c->S32[l - 1], c->lu_f[l - 1],
                                  s.nu2, static_cast<float>(rf_of(s.nu2)), c->d_active,
-                                 c->G32[l - 1], c->H32[l - 1], st));
+                                 c->G32[l - 1], c->H32[l - 1], st, false,
+                                 fused && l == 1 ? c->V[0] : nullptr));
     }
-    update(c->lv[0], K, c->V[0], c->E32[0], c->d_active, st);
+    if (!fused) update(c->lv[0], K, c->V[0], c->E32[0], c->d_active, st);
     return TBL_OK;
 }
 
