@@ -197,7 +197,8 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
 // ---------------------------------------------------------------------------
 // restrict (poisson_par.f:796-874): inject v into the coarse level (and the
 // FAS copy w), full-weight the fine residual into the coarse source.
-// fas == false: correction scheme (coarse v = 0, no copy).
+// fas == 0: correction scheme (coarse v = 0, no copy); fas < 0: correction
+// scheme with coarse v left alone (a zero-start smoother follows).
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) restrict_kernel(Level F, Level Cl, int K,
@@ -214,11 +215,11 @@ __global__ void __launch_bounds__(256) restrict_kernel(Level F, Level Cl, int K,
     const cx<T> *vf = Vf + static_cast<size_t>(k) * F.plane;
     const cx<T> *rf = Rf + static_cast<size_t>(k) * F.plane;
     const size_t co = static_cast<size_t>(k) * Cl.plane + static_cast<size_t>(i) * Cl.ld + j;
-    if (fas) {
+    if (fas > 0) {
         const cx<T> v = vf[static_cast<size_t>(fi) * F.ld + fj];
         Vc[co] = v;
         Wc[co] = v;
-    } else {
+    } else if (fas == 0) {
         Vc[co] = cx<T>{T(0), T(0)};
     }
     if (i >= 1 && i <= Cl.nx - 2) {

@@ -412,20 +412,23 @@ int mixed_cycle(tbl_ctx *c, const Sched &s, double tol)
     defect<double, float>(c->la_d[0], c->V[0], c->S[0], c->S32[0], -1.0, c->d_active, c->d_resmax, st);
     hipLaunchKernelGGL(decide2_kernel, dim3((K + 255) / 256), dim3(256), 0, st, K, c->d_resmax,
                        tol, c->d_cycle, c->d_active, c->d_cycles, c->d_lastres, c->d_nactive);
-    // e starts at zero: the staged smoother's first sweep reads no column it
-    // has not relaxed itself (e's boundary columns are never written and its
-    // wall row only ever receives zeros, both zero since allocation); the
-    // other shapes clear e first
-    const bool zs = c->G32[0] < 0;
+    // e starts at zero on every level: the staged smoother's first sweep
+    // reads no column it has not relaxed itself (e's boundary columns are
+    // never written and its wall rows only ever receive zeros, all zero since
+    // allocation); on the other shapes and the coarsest level e is cleared
+    // first (level 1 by a fill, coarser ones by the restriction)
+    auto zero_start = [&](int l) { return l < L - 1 && c->G32[l] < 0; };
+    const bool zs = zero_start(0);
     if (!zs) convert<float, float>(c->lv[0].plane * K, c->E32[0], c->E32[0], 1, st);
     for (int l = 0; l < L - 1; ++l) {
         TRY(smooth_sweeps<float>(c->la_f[l], c->E32[l], c->S32[l], c->lu_f[l], s.nu1,
                                  static_cast<float>(rf_of(s.nu1)), c->d_active, c->G32[l],
-                                 c->H32[l], st, zs && l == 0));
+                                 c->H32[l], st, zero_start(l)));
         defect<float, float>(c->la_f[l], c->E32[l], c->S32[l], c->R32[l], -1.0f, c->d_active,
                              nullptr, st);
         restrict_level<float>(c->lv[l], c->lv[l + 1], K, c->E32[l], c->R32[l], c->E32[l + 1],
-                              nullptr, c->S32[l + 1], 0, c->d_active, st);
+                              nullptr, c->S32[l + 1], zero_start(l + 1) ? -1 : 0, c->d_active,
+                              st);
     }
     // coarsest correction in FP32 too, from the zero the restriction left
     TRY(smooth_coarsest<float>(c->la_f[L - 1], c->E32[L - 1], c->S32[L - 1], c->lu_f[L - 1], s.nc,
