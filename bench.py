@@ -66,7 +66,9 @@ def measured_traffic(kernel):
         return None
     for name, e in table.items():
         if kernel + "(" in name:
-            return e["hbm_bytes_per_launch"]
+            # the plain sweeps' bytes (the launches the roofline times) when
+            # recorded, else the average over all of the kernel's launches
+            return e.get("sweep_bytes_per_launch", e["hbm_bytes_per_launch"])
     return None
 METRIC = "v-Poisson mode-solves/s to 1e-9 residual; learned+MG vs plain MG on 1 GPU"
 

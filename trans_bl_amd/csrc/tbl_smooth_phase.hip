@@ -24,6 +24,7 @@ __global__ void __launch_bounds__(256) smooth_phase_kernel(LevelArgs<T> L, cx<T>
 }
 
 constexpr int kStagedLines = 4;   // lines per workgroup of the staged smoother
+constexpr int kXcds = 8;          // MI355X: 8 XCDs, one L2 each
 
 // Same relaxation with coalesced global traffic: the group first reads its
 // line lane-strided (row t, t+G, ...: contiguous across the wavefront),
@@ -45,15 +46,21 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
     cx<T> *Xs = base_s + static_cast<size_t>(grp) * (FULL ? 3 : 1) * LDW;
     cx<T> *Vs = Xs + LDW, *Ss = Vs + LDW;
     const int nlines = phase_lines(phase, L.nx);
-    const int gid = blockIdx.x * kStagedLines + grp;
+    // XCD-aware order: the hardware deals workgroups round-robin to the 8
+    // XCDs (separate L2s), so workgroup n takes block (n % 8) * per + n / 8
+    // -- consecutive blocks, whose lines share neighbour columns, run on one
+    // XCD one after the other (the grid is padded to 8 * per blocks)
+    const int per = static_cast<int>(gridDim.x) / kXcds;
+    const int blk = static_cast<int>(blockIdx.x % kXcds) * per + static_cast<int>(blockIdx.x / kXcds);
+    const int gid = blk * kStagedLines + grp;
     const int k = gid / nlines;
     const bool live = k < L.K && !(active && !active[k]);
     const int ny = L.ny, ld = L.ld;
     // buffer resources over the (at most two) modes of this workgroup's
     // lines: 32-bit offsets, the row steps of the loads in the instruction
-    const int g0 = blockIdx.x * kStagedLines;
+    const int g0 = blk * kStagedLines;
     const int kf = g0 / nlines;
-    const int kl = min((g0 + kStagedLines - 1) / nlines, L.K - 1);
+    const int kl = max(kf, min((g0 + kStagedLines - 1) / nlines, L.K - 1));   // padding: kf >= K
     const size_t span = static_cast<size_t>(kl - kf + 1) * L.plane * sizeof(cx<T>);
     const __amdgpu_buffer_rsrc_t vr = buf_rsrc(V + static_cast<size_t>(kf) * L.plane, span);
     const __amdgpu_buffer_rsrc_t sr = buf_rsrc(S + static_cast<size_t>(kf) * L.plane, span);
@@ -94,15 +101,18 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
         cx<T> a0[R], a1[R], a2[R], a3[R], a4[R], vv[R], ss[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
+            // rows past ny (up to G*R) read zero, not the next column
+            const bool in = lane + r * G < ny;
             const unsigned rs = r * G * ES;
-            a0[r] = buf_load(vr, b0 + rs, T(0));
-            a1[r] = buf_load(vr, b1 + rs, T(0));
-            a2[r] = buf_load(vr, b2 + rs, T(0));
-            a3[r] = buf_load(vr, b3 + rs, T(0));
-            a4[r] = buf_load(vr, b4 + rs, T(0));
+            auto at = [&](unsigned b) { return in ? b + rs : kBufOff; };
+            a0[r] = buf_load(vr, at(b0), T(0));
+            a1[r] = buf_load(vr, at(b1), T(0));
+            a2[r] = buf_load(vr, at(b2), T(0));
+            a3[r] = buf_load(vr, at(b3), T(0));
+            a4[r] = buf_load(vr, at(b4), T(0));
             if (FULL) {
-                vv[r] = buf_load(vr, bv + rs, T(0));
-                ss[r] = buf_load(sr, lo + rs, T(0));
+                vv[r] = buf_load(vr, at(bv), T(0));
+                ss[r] = buf_load(sr, at(lo), T(0));
             }
         }
 #pragma unroll
@@ -192,8 +202,10 @@ static void launch_staged(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const
 {
     const size_t lines = static_cast<size_t>(phase_lines(phase, L.nx)) * L.K;
     if (!lines) return;
-    hipLaunchKernelGGL((smooth_staged_kernel<T, G, R, FULL>),
-                       dim3(static_cast<unsigned>((lines + kStagedLines - 1) / kStagedLines)),
+    // blocks padded to a multiple of the XCD count (the kernel's block order)
+    const size_t blocks = (lines + kStagedLines - 1) / kStagedLines;
+    const size_t per = (blocks + kXcds - 1) / kXcds;
+    hipLaunchKernelGGL((smooth_staged_kernel<T, G, R, FULL>), dim3(static_cast<unsigned>(per * kXcds)),
                        dim3(kStagedLines * G), staged_lds<FULL>(L.ld, sizeof(cx<T>)), st, L, V, S,
                        lu, phase, rf, active, zs, vup);
 }

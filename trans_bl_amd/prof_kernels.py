@@ -18,7 +18,12 @@ by dispatch order.
 reads the two counter runs' CSV output (rocprofv3 --output-format csv) and
 writes, per kernel, the average FETCH_SIZE and WRITE_SIZE per dispatch, each
 scaled so that the 256 MiB copy (the copy kernel's first dispatch) reads and
-writes exactly 256 MiB, as HBM bytes per launch.
+writes exactly 256 MiB, as HBM bytes per launch.  For the smoother kernels it
+also writes the average over their first SWEEP_LAUNCHES dispatches alone
+(`sweep_bytes_per_launch`): the plain sweeps of step 2 (with the line shapes
+cached, nothing launches a smoother before them), the same launches that
+`tbl_time_smoother` times; the solves' sweeps also include first sweeps from
+zero (fewer reads) and sweeps that update the FP64 field (more writes).
 """
 from __future__ import annotations
 
@@ -31,6 +36,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 COPY_BYTES = 256 << 20
+SWEEP_LAUNCHES = 8   # two sweeps of four launches (border + three colours)
 
 
 def main():
@@ -104,6 +110,11 @@ def summarize(fetch_dir, write_dir, out):
         res["kernels"][name] = {"dispatches": len(e["fetch_kib"]),
                                 "fetch_bytes": round(f * 1024), "write_bytes": round(w * 1024),
                                 "hbm_bytes_per_launch": round((f + w) * 1024)}
+        if "smooth_" in name and len(e["fetch_kib"]) >= SWEEP_LAUNCHES:
+            n = SWEEP_LAUNCHES
+            fs8 = sum(e["fetch_kib"][:n]) / n
+            ws8 = sum(e["write_kib"][:n]) / n
+            res["kernels"][name]["sweep_bytes_per_launch"] = round((fs8 + ws8) * 1024)
     with open(out, "w") as fo:
         json.dump(res, fo, indent=1, sort_keys=True)
     print(json.dumps({"kernels": len(res["kernels"]), "fetch_scale": fs, "write_scale": ws}))
