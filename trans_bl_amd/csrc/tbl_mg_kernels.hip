@@ -61,6 +61,9 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
                                                            const int *active, double *resmax,
                                                            int strip, int nstrips, int jchunks)
 {
+    // fused multiply-adds in FP32 and in the mixed defect (FP64 in, FP32
+    // out: no reference counterpart); the FP64 defect rounds as written
+    constexpr bool kFuse = sizeof(T) == 4 || sizeof(TO) == 4;
     const int lane = threadIdx.x & 63;
     const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int k = blockIdx.y;
@@ -134,12 +137,17 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
                 xs = T(10) * w3 - T(15) * w2 - T(4) * w1 + T(14) * w0 - T(6) * vload(nx - 5)
                      + vload(nx - 6);
             } else {
-                // -w0 + 16 w1 - 30 w2 + 16 w3 - w4 (multiply-adds in FP32)
-                xs = mac(T(-1), w4, mac(T(16), w3, mac(T(-30), w2, mac(T(16), w1, T(-1) * w0))));
+                // -w0 + 16 w1 - 30 w2 + 16 w3 - w4 (fused where kFuse)
+                xs = mac_f<kFuse>(T(-1), w4,
+                                  mac_f<kFuse>(T(16), w3,
+                                               mac_f<kFuse>(T(-30), w2,
+                                                            mac_f<kFuse>(T(16), w1, T(-1) * w0))));
             }
             auto own_x = [&]() {
                 // -(w0 + w4) + 16 (w1 + w3) - 30 w2
-                return border ? xs : mac(T(-30), w2, dot2(T(-1), w0 + w4, T(16), w1 + w3));
+                return border ? xs
+                              : mac_f<kFuse>(T(-30), w2,
+                                             dot2_f<kFuse>(T(-1), w0 + w4, T(16), w1 + w3));
             };
             // y-neighbours from the adjacent lanes by DPP wavefront shifts
             // (no LDS round trip); the edge lanes 0, 1, 62, 63 are halo rows
@@ -157,8 +165,13 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
                     lv = b * iscx + c * q + d * ifac;
                 } else if (j <= ny - 3) {
                     // ax0 xm + ax1 xs + ax2 xp + cv0 vm2 + ... + cv4 vp2, summed left to right
-                    lv = mac(cv4, vp2, mac(cv3, vp1, mac(cv2, w2, mac(cv1, vm1, mac(cv0, vm2,
-                             mac(ax2, xp, dot2(ax0, xm, ax1, xs)))))));
+                    cx<T> t = dot2_f<kFuse>(ax0, xm, ax1, xs);
+                    t = mac_f<kFuse>(ax2, xp, t);
+                    t = mac_f<kFuse>(cv0, vm2, t);
+                    t = mac_f<kFuse>(cv1, vm1, t);
+                    t = mac_f<kFuse>(cv2, w2, t);
+                    t = mac_f<kFuse>(cv3, vp1, t);
+                    lv = mac_f<kFuse>(cv4, vp2, t);
                 } else if (j == ny - 2) {
                     const cx<T> c = L.pp[3] * vp1 + L.pp[2] * w2 + L.pp[1] * vm1;
                     const cx<T> d = k2b2 * w2;
@@ -169,7 +182,7 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
                     lv = own_x() * iscx + c * q + d * ifac - L.lp[1] * alpha2 * w2 * L.sqq_top;
                 }
                 const size_t o = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * ld + j;
-                const cx<T> r = mac(si, lv, s_cur);   // s + si lv
+                const cx<T> r = mac_f<kFuse>(si, lv, s_cur);   // s + si lv
                 out[o] = cx<TO>{static_cast<TO>(r.re), static_cast<TO>(r.im)};
                 if (resmax) {
                     // max of |r|^2; one square root per wavefront below

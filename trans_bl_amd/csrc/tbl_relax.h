@@ -30,27 +30,46 @@ constexpr bool kIsF32 = sizeof(T) == 4;
 //   mac(a, x, c)           = a x + c
 //   dot2(a, x, b, y)       = a x + b y
 //   dot2c(a, x, b, y, c)   = (a x + b y) + c
+// mac_f / dot2_f take the choice as a parameter (F: fused), for FP64 work
+// that is not bound to the reference's rounding.
 template <typename T>
-__device__ __forceinline__ T mac(T a, T x, T c)
+__device__ __forceinline__ T fused_ma(T a, T x, T c)
 {
     if constexpr (kIsF32<T>) return __builtin_fmaf(a, x, c);
+    else return __builtin_fma(a, x, c);
+}
+template <bool F, typename T>
+__device__ __forceinline__ T mac_f(T a, T x, T c)
+{
+    if constexpr (F) return fused_ma(a, x, c);
     else return a * x + c;
 }
-template <typename T>
-__device__ __forceinline__ cx<T> mac(T a, cx<T> x, cx<T> c)
+template <bool F, typename T>
+__device__ __forceinline__ cx<T> mac_f(T a, cx<T> x, cx<T> c)
 {
-    return {mac(a, x.re, c.re), mac(a, x.im, c.im)};
+    return {mac_f<F>(a, x.re, c.re), mac_f<F>(a, x.im, c.im)};
 }
-template <typename T>
-__device__ __forceinline__ T dot2(T a, T x, T b, T y)
+template <bool F, typename T>
+__device__ __forceinline__ T dot2_f(T a, T x, T b, T y)
 {
-    if constexpr (kIsF32<T>) return __builtin_fmaf(b, y, a * x);
+    if constexpr (F) return fused_ma(b, y, a * x);
     else return a * x + b * y;
 }
+template <bool F, typename T>
+__device__ __forceinline__ cx<T> dot2_f(T a, cx<T> x, T b, cx<T> y)
+{
+    return {dot2_f<F>(a, x.re, b, y.re), dot2_f<F>(a, x.im, b, y.im)};
+}
+template <typename T>
+__device__ __forceinline__ T mac(T a, T x, T c) { return mac_f<kIsF32<T>>(a, x, c); }
+template <typename T>
+__device__ __forceinline__ cx<T> mac(T a, cx<T> x, cx<T> c) { return mac_f<kIsF32<T>>(a, x, c); }
+template <typename T>
+__device__ __forceinline__ T dot2(T a, T x, T b, T y) { return dot2_f<kIsF32<T>>(a, x, b, y); }
 template <typename T>
 __device__ __forceinline__ cx<T> dot2(T a, cx<T> x, T b, cx<T> y)
 {
-    return {dot2(a, x.re, b, y.re), dot2(a, x.im, b, y.im)};
+    return dot2_f<kIsF32<T>>(a, x, b, y);
 }
 template <typename T>
 __device__ __forceinline__ T dot2c(T a, T x, T b, T y, T c)
