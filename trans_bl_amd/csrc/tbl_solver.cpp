@@ -417,7 +417,7 @@ int mixed_cycle(tbl_ctx *c, const Sched &s, double tol)
     // never written and its wall rows only ever receive zeros, all zero since
     // allocation); on the other shapes and the coarsest level e is cleared
     // first (level 1 by a fill, coarser ones by the restriction)
-    auto zero_start = [&](int l) { return l < L - 1 && c->G32[l] < 0; };
+    auto zero_start = [&](int l) { return s.nu1 > 0 && l < L - 1 && c->G32[l] < 0; };
     const bool zs = zero_start(0);
     if (!zs) convert<float, float>(c->lv[0].plane * K, c->E32[0], c->E32[0], 1, st);
     for (int l = 0; l < L - 1; ++l) {
