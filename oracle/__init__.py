@@ -1,7 +1,8 @@
 """ctypes front end of the CPU oracle (test infrastructure only).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
-leg may import this package, and only as the checker.  It wraps
+leg may import this package, and only as the reference that results are
+compared against (never as the thing measured or shipped).  It wraps
 ``libtbl_oracle.so`` (built from ``tbl_oracle.c`` by ``oracle/Makefile``) and
 the numpy coefficient generator in ``coefgen.py``.
 """

@@ -2,7 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY.  Nothing under ``oracle/`` is part of the product:
 only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
-``bench.py`` may import it, and only as the checker.
+``bench.py`` may import it, and only as the reference that results are
+compared against.
 
 What it restates
 ----------------

@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY: a plain-C restatement of the reference's FP64
  * algorithm, used by tests/, __graft_entry__.smoke() and bench.py's CPU
- * baseline as the checker.  The product never links or calls it.
+ * baseline as the reference that results are compared against.  The
+ * product never links or calls it.
  *
  * Arrays are Fortran order with the first index fastest, complex values
  * interleaved (re, im) -- the reference's complex*16 (ptsx, jmax, kfour)

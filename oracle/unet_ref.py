@@ -3,8 +3,8 @@
 A float64 numpy restatement of the network in trans_bl_amd/csrc/tbl_net.cpp
 (same topology, parameter layout and im2col ordering) with a hand-written
 backward pass.  The backward pass itself is pinned by central finite
-differences in tests/test_oracle_unet.py.  Used only as the checker of the
-GPU forward/backward kernels.
+differences in tests/test_oracle_unet.py.  Used only as the reference the
+GPU forward/backward kernels are compared against.
 
 Parameter layout: for each convolution in order E0A..OUT, the weights
 W[ks*ks*cin, cout] (row = tap*cin + ci, tap = (dh+1)*3 + (dw+1)), then the
