@@ -33,7 +33,9 @@ constexpr int kXcds = 8;          // MI355X: 8 XCDs, one L2 each
 // of the LDS, so more lines in flight per CU); the scan then works on
 // lane-contiguous segments; with FULL the relaxed v goes back through LDS
 // and is stored lane-strided.  Arithmetic identical to smooth_phase_kernel.
-template <typename T, int G, int R, bool FULL>
+// UPD: vup (the FP64 field of the same layout) += the relaxed v on the
+// line's rows 1.., its values loaded with the line's other inputs
+template <typename T, int G, int R, bool FULL, bool UPD = false>
 __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T> *V,
                                                             const cx<T> *S, const T *lu,
                                                             int phase, T rf, const int *active,
@@ -66,6 +68,7 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
     const __amdgpu_buffer_rsrc_t sr = buf_rsrc(S + static_cast<size_t>(kf) * L.plane, span);
     int side = 0;
     size_t base = 0;
+    cx<double> up0[UPD ? R : 1];
     if (live) {
         const int c = gid - k * nlines;
         const int i = phase_column(phase, c, L.nx);
@@ -99,6 +102,13 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
         const unsigned b0 = col_off(d0), b1 = col_off(d1), b2 = col_off(d2), b3 = col_off(d3),
                        b4 = col_off(d4), bv = col_off(0);
         cx<T> a0[R], a1[R], a2[R], a3[R], a4[R], vv[R], ss[R];
+        if constexpr (UPD) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int j = lane + r * G;
+                if (j >= 1 && j < ny) up0[r] = vup[base + j];
+            }
+        }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             // rows past ny (up to G*R) read zero, not the next column
@@ -165,10 +175,18 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
                 if (j >= 1 && j < ny) {
                     const cx<T> e = Vs[j];
                     col[j] = e;
-                    // vup (the last sweep of a correction solve): this
+                    // UPD (the last sweep of a correction solve): this
                     // line's e is final, so it is added to the FP64 field
                     // here (same layout), as the update kernel would
-                    if (vup) {
+                    if constexpr (UPD) {
+                        vup[base + j] = cx<double>{up0[r].re + static_cast<double>(e.re),
+                                                   up0[r].im + static_cast<double>(e.im)};
+                    } else if (vup) {
+                        // the same update in runtime form: the dispatch never
+                        // passes a field to these instantiations, but with
+                        // this path the compiler schedules them as measured
+                        // fastest (the FP64 finest-level kernel ran 4.6 %
+                        // slower without it)
                         cx<double> &u = vup[base + j];
                         u = cx<double>{u.re + static_cast<double>(e.re),
                                        u.im + static_cast<double>(e.im)};
@@ -196,7 +214,7 @@ static size_t staged_lds(int ld, size_t esize)
     return static_cast<size_t>(kStagedLines) * (FULL ? 3 : 1) * ld * esize;
 }
 
-template <typename T, int G, int R, bool FULL>
+template <typename T, int G, int R, bool FULL, bool UPD = false>
 static void launch_staged(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int phase,
                           T rf, const int *active, hipStream_t st, int zs, cx<double> *vup)
 {
@@ -205,9 +223,24 @@ static void launch_staged(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const
     // blocks padded to a multiple of the XCD count (the kernel's block order)
     const size_t blocks = (lines + kStagedLines - 1) / kStagedLines;
     const size_t per = (blocks + kXcds - 1) / kXcds;
-    hipLaunchKernelGGL((smooth_staged_kernel<T, G, R, FULL>), dim3(static_cast<unsigned>(per * kXcds)),
+    hipLaunchKernelGGL((smooth_staged_kernel<T, G, R, FULL, UPD>),
+                       dim3(static_cast<unsigned>(per * kXcds)),
                        dim3(kStagedLines * G), staged_lds<FULL>(L.ld, sizeof(cx<T>)), st, L, V, S,
                        lu, phase, rf, active, zs, vup);
+}
+
+// the FULL staged form, with the update variant (FP32 only) when vup is set
+template <typename T, int G, int R>
+static void launch_full(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int phase,
+                        T rf, const int *active, hipStream_t st, int zs, cx<double> *vup)
+{
+    if constexpr (sizeof(T) == 4) {
+        if (vup) {
+            launch_staged<T, G, R, true, true>(L, V, S, lu, phase, rf, active, st, zs, vup);
+            return;
+        }
+    }
+    launch_staged<T, G, R, true>(L, V, S, lu, phase, rf, active, st, zs, nullptr);
 }
 
 // One colour phase per launch.  G > 0: lanes per line with direct
@@ -220,6 +253,7 @@ int phase_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, i
 {
     const int form = G < 0 ? 1 : G > kXStagedShape ? 2 : 0;
     if ((zero_start || vup) && form != 1) return TBL_ERR_ARG;
+    if (vup && sizeof(T) != 4) return TBL_ERR_ARG;   // the update variant is FP32 only
     G = form == 1 ? -G : form == 2 ? G - kXStagedShape : G;
     const int R = line_rows_for(G, L.ny);
     if (R < 0) return TBL_ERR_GRID;
@@ -232,7 +266,7 @@ int phase_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, i
             cx<double> *up = it == sweeps - 1 ? vup : nullptr;
 #define X(GG, RR)                                                                       \
     if (!done && G == GG && R == RR) {                                                  \
-        if (form == 1) launch_staged<T, GG, RR, true>(L, V, S, lu, ph, rf, active, st, zs, up); \
+        if (form == 1) launch_full<T, GG, RR>(L, V, S, lu, ph, rf, active, st, zs, up);   \
         else if (form == 2) launch_staged<T, GG, RR, false>(L, V, S, lu, ph, rf, active, st, 0, nullptr); \
         else launch_phase<T, GG, RR>(L, V, S, lu, ph, rf, active, st);                  \
         done = true;                                                                    \
@@ -251,6 +285,7 @@ int phase_kernel_attributes()
     for (const void *f :                                                                         \
          {reinterpret_cast<const void *>(&smooth_staged_kernel<double, GG, RR, true>),           \
           reinterpret_cast<const void *>(&smooth_staged_kernel<float, GG, RR, true>),            \
+          reinterpret_cast<const void *>(&smooth_staged_kernel<float, GG, RR, true, true>),      \
           reinterpret_cast<const void *>(&smooth_staged_kernel<double, GG, RR, false>),          \
           reinterpret_cast<const void *>(&smooth_staged_kernel<float, GG, RR, false>)}) {        \
         hipError_t r = allow_full_lds(f);                                                        \
