@@ -42,19 +42,21 @@ ROWS = (1, 2, 3, 4, 5, 7, 9, 13, 17)
 
 
 def smoother_kernel(prec, lanes, ny):
-    """Name of the smoother kernel a line shape runs (as rocprofv3 prints it):
-    G direct, -G staged (x terms, v, S in LDS), 2000+G staged (x terms only),
-    1000+G fused-sweep strip."""
+    """Name of the smoother kernel a line shape runs in a plain sweep (as
+    rocprofv3 prints it): G direct, -G staged (x terms, v, S in LDS), 2000+G
+    staged (x terms only), 1000+G fused-sweep strip.  The staged kernel's
+    last template argument selects its updating form, which plain sweeps do
+    not use."""
     g = lanes - 2000 if lanes > 2000 else lanes - 1000 if lanes > 1000 else abs(lanes)
     need = -(-ny // g)
     r = next(x for x in ROWS if x >= need)
     t = "float" if prec else "double"
     if lanes > 2000:
-        return f"smooth_staged_kernel<{t}, {g}, {r}, false>"
+        return f"smooth_staged_kernel<{t}, {g}, {r}, false, false>"
     if lanes > 1000:
         return f"smooth_strip_kernel<{t}, {g}, {r}>"
     if lanes < 0:
-        return f"smooth_staged_kernel<{t}, {g}, {r}, true>"
+        return f"smooth_staged_kernel<{t}, {g}, {r}, true, false>"
     return f"smooth_phase_kernel<{t}, {g}, {r}>"
 
 
