@@ -343,8 +343,7 @@ extern "C" int tbl_guess_pairs(tbl_ctx *c, const double *d_f, const double *d_v,
     Guess *g = c->guess;
     hipStream_t st = c->stream;
     const Level &F = c->lv[0], &Cl = c->lv[g->lvl - 1];
-    gather(c->g.nx, c->g.ny, c->K, F, nullptr, d_f, d_v, c->g.beta, c->R[0], c->V[0], 1, st);
-    source(c->la_d[0], c->R[0], c->S[0], st);
+    gather_source(c->la_d[0], nullptr, d_f, d_v, c->g.beta, c->S[0], c->V[0], 1, st);
     restrict_chain(c, g->lvl - 1, st);
     build_features(c, st);
     const long n = static_cast<long>(c->K) * g->H * g->W;

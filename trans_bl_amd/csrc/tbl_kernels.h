@@ -113,6 +113,10 @@ void gather(int nx, int ny, int K, const Level &F, const double *wx, const doubl
             const double *uy, double beta, cx<double> *G, cx<double> *V, int direct_rhs,
             hipStream_t st);
 void source(const LevelArgs<double> &L, const cx<double> *G, cx<double> *S, hipStream_t st);
+// gather + source in one kernel (the user-layout g never stored)
+void gather_source(const LevelArgs<double> &L, const double *wx, const double *dz,
+                   const double *uy, double beta, cx<double> *S, cx<double> *V, int direct_rhs,
+                   hipStream_t st);
 void scatter(int nx, int ny, int K, const Level &F, const cx<double> *V, double *uy,
              hipStream_t st);
 void decide(int K, const double *resmax, double tol, int cycle, int *active, int *cycles,

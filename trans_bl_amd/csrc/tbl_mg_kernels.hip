@@ -352,6 +352,63 @@ __global__ void __launch_bounds__(256) gather_kernel(int nx, int ny, size_t ld, 
     }
 }
 
+// gather and function_source in one pass: g = kb*wx - dwzdx (or f) of rows
+// j0-1 .. j0+32 of a 32 x 32 tile goes through LDS, and the tile's S = B g
+// (source_kernel's weights and order) and v = uy are written in the internal
+// layout; g itself never reaches memory
+__global__ void __launch_bounds__(256) gather_source_kernel(LevelArgs<double> L,
+                                                            const cx<double> *wx,
+                                                            const cx<double> *dz,
+                                                            const cx<double> *uy, double beta,
+                                                            cx<double> *S, cx<double> *Vout,
+                                                            int direct_rhs)
+{
+    __shared__ cx<double> tg[34][33], tv[32][33];
+    const int nx = L.nx, ny = L.ny;
+    const int k = blockIdx.z;
+    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;   // 32 x 8
+    const double kb = -static_cast<double>(k) * beta;           // kb = (0, kb)
+    const size_t ubase = static_cast<size_t>(k) * nx * ny;
+    for (int r = ty; r < 34; r += 8) {
+        const int i = i0 + tx, j = j0 - 1 + r;
+        cx<double> g = {0, 0};
+        if (i < nx && j >= 0 && j < ny) {
+            const size_t u = ubase + static_cast<size_t>(j) * nx + i;
+            const cx<double> d = dz ? dz[u] : cx<double>{0, 0};
+            if (direct_rhs) {
+                g = d;
+            } else {
+                cx<double> kw = {0, 0};
+                if (wx) { const cx<double> w = wx[u]; kw = cmul(cx<double>{0.0, kb}, w); }
+                g = kw - d;
+            }
+        }
+        tg[r][tx] = g;
+    }
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + tx, j = j0 + r;
+        cx<double> v = {0, 0};
+        if (uy && i < nx && j < ny) v = uy[ubase + static_cast<size_t>(j) * nx + i];
+        tv[r][tx] = v;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r, j = j0 + tx;
+        if (i < nx && j < ny) {
+            // g[j + d] = tg[tx + 1 + d][r]
+            cx<double> s = {0, 0};
+            if (j == 1) s = L.sp[1] * tg[tx + 1][r] + L.sp[2] * tg[tx + 2][r];
+            else if (j >= 2 && j <= ny - 3)
+                s = L.cp[0] * tg[tx][r] + L.cp[1] * tg[tx + 1][r] + L.cp[2] * tg[tx + 2][r];
+            else if (j == ny - 2) s = tg[tx + 1][r];
+            const size_t o = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * L.ld + j;
+            S[o] = s;
+            Vout[o] = tv[tx][r];
+        }
+    }
+}
+
 // function_source weights (poisson_par.f:126-144), internal layout
 __global__ void __launch_bounds__(256) source_kernel(LevelArgs<double> L, const cx<double> *Gin,
                                                      cx<double> *S)
@@ -545,6 +602,17 @@ void gather(int nx, int ny, int K, const Level &F, const double *wx, const doubl
                        F.plane, reinterpret_cast<const cx<double> *>(wx),
                        reinterpret_cast<const cx<double> *>(dz),
                        reinterpret_cast<const cx<double> *>(uy), beta, G, V, direct_rhs);
+}
+
+void gather_source(const LevelArgs<double> &L, const double *wx, const double *dz,
+                   const double *uy, double beta, cx<double> *S, cx<double> *V, int direct_rhs,
+                   hipStream_t st)
+{
+    dim3 grid((L.nx + 31) / 32, (L.ny + 31) / 32, L.K);
+    hipLaunchKernelGGL(gather_source_kernel, grid, dim3(256), 0, st, L,
+                       reinterpret_cast<const cx<double> *>(wx),
+                       reinterpret_cast<const cx<double> *>(dz),
+                       reinterpret_cast<const cx<double> *>(uy), beta, S, V, direct_rhs);
 }
 
 void source(const LevelArgs<double> &L, const cx<double> *G, cx<double> *S, hipStream_t st)

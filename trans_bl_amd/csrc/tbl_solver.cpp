@@ -495,9 +495,8 @@ int solve_dev(tbl_ctx *c, const double *wx, const double *dz, double *uy, int di
     // source assembly: S = B g, v = uy (function_source, poisson_par.f:111-164)
     // guess 1: start from zero (boundary data taken as zero as well)
     const bool zero_start = opts && opts->guess == 1;
-    gather(c->g.nx, c->g.ny, K, F, wx, dz, zero_start ? nullptr : uy, c->g.beta, c->R[0], c->V[0],
-           direct_rhs, st);
-    source(c->la_d[0], c->R[0], c->S[0], st);
+    gather_source(c->la_d[0], wx, dz, zero_start ? nullptr : uy, c->g.beta, c->S[0], c->V[0],
+                  direct_rhs, st);
     int fallback = 0;
     double guess_res = 0.0;
     if (opts && (opts->guess == 2 || opts->guess == 3))
@@ -620,8 +619,7 @@ extern "C" int tbl_residual_dev(tbl_ctx *c, const double *d_wx, const double *d_
     hipStream_t st = c->stream;
     const int K = c->K;
     const Level &F = c->lv[0];
-    gather(c->g.nx, c->g.ny, K, F, d_wx, d_dwzdx, d_uy, c->g.beta, c->R[0], c->V[0], 0, st);
-    source(c->la_d[0], c->R[0], c->S[0], st);
+    gather_source(c->la_d[0], d_wx, d_dwzdx, d_uy, c->g.beta, c->S[0], c->V[0], 0, st);
     fill(c->d_resmax, K, 0.0, st);
     fill(reinterpret_cast<double *>(c->R[0]), F.plane * K * 2, 0.0, st);
     defect<double, double>(c->la_d[0], c->V[0], c->S[0], c->R[0], -1.0, nullptr, c->d_resmax, st);
@@ -710,8 +708,7 @@ int stage_issue(tbl_ctx *c, const double *f, double *v, double *const *d, double
     const int prec = opts ? opts->precision : 0;
     const int K = c->K;
     for (int q = 0; q < 4; ++q) TRY(deriv(c->dargs, q, v, d[q], st));
-    gather(c->g.nx, c->g.ny, K, c->lv[0], nullptr, f, v, c->g.beta, c->R[0], c->V[0], 1, st);
-    source(c->la_d[0], c->R[0], c->S[0], st);
+    gather_source(c->la_d[0], nullptr, f, v, c->g.beta, c->S[0], c->V[0], 1, st);
     fill_int(c->d_active, K, 1, st);
     fill_int(c->d_cycles, K, 0, st);
     fill_int(c->d_cycle, 1, 0, st);
