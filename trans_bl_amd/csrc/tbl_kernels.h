@@ -109,11 +109,7 @@ void restrict_level(const Level &F, const Level &Cl, int K, const cx<T> *Vf, con
 template <typename T>
 void prolong_level(const Level &F, const Level &Cl, int K, cx<T> *Vf, const cx<T> *Vc,
                    const cx<T> *Wc, int fas, const int *active, hipStream_t st);
-void gather(int nx, int ny, int K, const Level &F, const double *wx, const double *dz,
-            const double *uy, double beta, cx<double> *G, cx<double> *V, int direct_rhs,
-            hipStream_t st);
-void source(const LevelArgs<double> &L, const cx<double> *G, cx<double> *S, hipStream_t st);
-// gather + source in one kernel (the user-layout g never stored)
+// user layout -> internal layout and S = B g (function_source) in one kernel
 void gather_source(const LevelArgs<double> &L, const double *wx, const double *dz,
                    const double *uy, double beta, cx<double> *S, cx<double> *V, int direct_rhs,
                    hipStream_t st);

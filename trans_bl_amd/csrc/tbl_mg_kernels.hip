@@ -310,52 +310,12 @@ __global__ void __launch_bounds__(256) prolong_kernel(Level F, Level Cl, int K, 
 // ---------------------------------------------------------------------------
 // user layout (x fastest) <-> internal layout (y fastest) through LDS tiles
 // ---------------------------------------------------------------------------
-// g = kb*wx - dwzdx (wx may be NULL) into G (internal), uy into V (internal)
-__global__ void __launch_bounds__(256) gather_kernel(int nx, int ny, size_t ld, size_t plane,
-                                                     const cx<double> *wx, const cx<double> *dz,
-                                                     const cx<double> *uy, double beta,
-                                                     cx<double> *Gout, cx<double> *Vout,
-                                                     int direct_rhs)
-{
-    __shared__ cx<double> tg[32][33], tv[32][33];
-    const int k = blockIdx.z;
-    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
-    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;   // 32 x 8
-    const double kb = -static_cast<double>(k) * beta;           // kb = (0, kb)
-    const size_t ubase = static_cast<size_t>(k) * nx * ny;
-    for (int r = ty; r < 32; r += 8) {
-        const int i = i0 + tx, j = j0 + r;
-        cx<double> g = {0, 0}, v = {0, 0};
-        if (i < nx && j < ny) {
-            const size_t u = ubase + static_cast<size_t>(j) * nx + i;
-            cx<double> d = dz ? dz[u] : cx<double>{0, 0};
-            if (direct_rhs) {
-                g = d;                      // d holds f directly (solve with S = B f)
-            } else {
-                cx<double> kw = {0, 0};
-                if (wx) { const cx<double> w = wx[u]; kw = cmul(cx<double>{0.0, kb}, w); }
-                g = kw - d;
-            }
-            v = uy ? uy[u] : cx<double>{0, 0};
-        }
-        tg[r][tx] = g;
-        tv[r][tx] = v;
-    }
-    __syncthreads();
-    for (int r = ty; r < 32; r += 8) {
-        const int i = i0 + r, j = j0 + tx;
-        if (i < nx && j < ny) {
-            const size_t o = static_cast<size_t>(k) * plane + static_cast<size_t>(i) * ld + j;
-            Gout[o] = tg[tx][r];
-            Vout[o] = tv[tx][r];
-        }
-    }
-}
 
-// gather and function_source in one pass: g = kb*wx - dwzdx (or f) of rows
-// j0-1 .. j0+32 of a 32 x 32 tile goes through LDS, and the tile's S = B g
-// (source_kernel's weights and order) and v = uy are written in the internal
-// layout; g itself never reaches memory
+// function_source (poisson_par.f:111-164) with the layout change: g =
+// kb*wx - dwzdx (or f itself) of rows j0-1 .. j0+32 of a 32 x 32 tile goes
+// through LDS, and the tile's S = B g (row weights sp2/sp3 at Fortran j = 2,
+// cp1..3 on the centre rows, 1 at ny-1, 0 at 1 and ny) and v = uy are
+// written in the internal layout; g itself never reaches memory
 __global__ void __launch_bounds__(256) gather_source_kernel(LevelArgs<double> L,
                                                             const cx<double> *wx,
                                                             const cx<double> *dz,
@@ -409,23 +369,6 @@ __global__ void __launch_bounds__(256) gather_source_kernel(LevelArgs<double> L,
     }
 }
 
-// function_source weights (poisson_par.f:126-144), internal layout
-__global__ void __launch_bounds__(256) source_kernel(LevelArgs<double> L, const cx<double> *Gin,
-                                                     cx<double> *S)
-{
-    // grid (rows / 32, columns / 8, modes), block 32 x 8: no index division
-    const int k = blockIdx.z;
-    const int j = blockIdx.x * 32 + threadIdx.x, i = blockIdx.y * 8 + threadIdx.y;
-    if (j >= L.ny || i >= L.nx) return;
-    const size_t col = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * L.ld;
-    const size_t idx = col + j;
-    const cx<double> *g = Gin + col;
-    cx<double> s = {0, 0};
-    if (j == 1) s = L.sp[1] * g[1] + L.sp[2] * g[2];
-    else if (j >= 2 && j <= L.ny - 3) s = L.cp[0] * g[j - 1] + L.cp[1] * g[j] + L.cp[2] * g[j + 1];
-    else if (j == L.ny - 2) s = g[j];
-    S[idx] = s;
-}
 
 // internal V -> user uy on i=2..nx-1, j=2..ny (poisson_par.f:52-57)
 __global__ void __launch_bounds__(256) scatter_kernel(int nx, int ny, size_t ld, size_t plane,
@@ -593,16 +536,6 @@ void prolong_level(const Level &F, const Level &Cl, int K, cx<T> *Vf, const cx<T
                        active);
 }
 
-void gather(int nx, int ny, int K, const Level &F, const double *wx, const double *dz,
-            const double *uy, double beta, cx<double> *G, cx<double> *V, int direct_rhs,
-            hipStream_t st)
-{
-    dim3 grid((nx + 31) / 32, (ny + 31) / 32, K);
-    hipLaunchKernelGGL(gather_kernel, grid, dim3(256), 0, st, nx, ny, static_cast<size_t>(F.ld),
-                       F.plane, reinterpret_cast<const cx<double> *>(wx),
-                       reinterpret_cast<const cx<double> *>(dz),
-                       reinterpret_cast<const cx<double> *>(uy), beta, G, V, direct_rhs);
-}
 
 void gather_source(const LevelArgs<double> &L, const double *wx, const double *dz,
                    const double *uy, double beta, cx<double> *S, cx<double> *V, int direct_rhs,
@@ -615,11 +548,6 @@ void gather_source(const LevelArgs<double> &L, const double *wx, const double *d
                        reinterpret_cast<const cx<double> *>(uy), beta, S, V, direct_rhs);
 }
 
-void source(const LevelArgs<double> &L, const cx<double> *G, cx<double> *S, hipStream_t st)
-{
-    hipLaunchKernelGGL(source_kernel, dim3((L.ny + 31) / 32, (L.nx + 7) / 8, L.K), dim3(32, 8), 0,
-                       st, L, G, S);
-}
 
 void scatter(int nx, int ny, int K, const Level &F, const cx<double> *V, double *uy,
              hipStream_t st)
