@@ -72,7 +72,8 @@ typedef struct tbl_solve_opts {
                          for precision 0, 2 for the FP32 correction cycle */
     int nu2;          /* post-smoothing sweeps, 0 -> 1 (poisson_par.f:49) */
     int ncoarse;      /* coarsest-level sweeps, 0 -> 60 (20*itera, :45) for
-                         precision 0, 10 for the FP32 correction cycle */
+                         precision 0; for the FP32 correction cycle 5 when the
+                         coarsest level has at most 25 rows, else 10 */
     int use_graph;    /* 1: replay the V-cycle as a captured hipGraph */
     int guess;        /* 0: start from uy (the reference's warm start,
                             poisson_par.f:157-161); 1: start from zero;

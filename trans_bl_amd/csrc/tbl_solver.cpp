@@ -324,15 +324,20 @@ struct Sched {
 // on wavepacket workloads from 257x65 to 2049x257, 2 pre-sweeps and 10
 // coarsest sweeps took up to one more V-cycle than 3 and 10 but less time
 // to the same 1e-9 residual on all but the smallest grids.  5 coarsest
-// sweeps were faster again where ny <= 193 but needed up to 15 cycles on
-// grids with ny = 257.
+// sweeps took as many cycles as 10 where ny <= 193 (coarsest ny <= 25) but
+// up to 15 cycles on grids with ny = 257 (coarsest ny 33), so the count
+// follows the coarsest level's ny.
 constexpr int kMixedPreSweeps = 2;
 constexpr int kMixedCoarseSweeps = 10;
+constexpr int kMixedCoarseSweepsShort = 5;   // coarsest ny <= kShortCoarseNy
+constexpr int kShortCoarseNy = 25;
 
-Sched sched_of(const tbl_solve_opts *o)
+Sched sched_of(const tbl_solve_opts *o, int coarsest_ny)
 {
     const bool mixed = o && o->precision == 1;
-    Sched s{11, mixed ? kMixedPreSweeps : 3, 1, mixed ? kMixedCoarseSweeps : 60};
+    const int nc_mixed =
+        coarsest_ny <= kShortCoarseNy ? kMixedCoarseSweepsShort : kMixedCoarseSweeps;
+    Sched s{11, mixed ? kMixedPreSweeps : 3, 1, mixed ? nc_mixed : 60};
     if (o) {
         if (o->max_cycles > 0) s.maxc = o->max_cycles;
         if (o->nu1 > 0) s.nu1 = o->nu1;
@@ -483,7 +488,7 @@ int solve_dev(tbl_ctx *c, const double *wx, const double *dz, double *uy, int di
               tbl_solve_stats *stats)
 {
     if (!c || !dz || !uy) return TBL_ERR_ARG;
-    const Sched s = sched_of(opts);
+    const Sched s = sched_of(opts, c->lv[c->L - 1].ny);
     const int prec = opts ? opts->precision : 0;
     if (prec < 0 || prec > 1) return TBL_ERR_ARG;
     if (opts && (opts->guess < 0 || opts->guess > 3)) return TBL_ERR_ARG;
@@ -704,7 +709,7 @@ int stage_issue(tbl_ctx *c, const double *f, double *v, double *const *d, double
                 const tbl_solve_opts *opts)
 {
     hipStream_t st = c->stream;
-    const Sched s = sched_of(opts);
+    const Sched s = sched_of(opts, c->lv[c->L - 1].ny);
     const int prec = opts ? opts->precision : 0;
     const int K = c->K;
     for (int q = 0; q < 4; ++q) TRY(deriv(c->dargs, q, v, d[q], st));
