@@ -209,6 +209,29 @@ def test_staged_smoother_matches_direct(torch_mod, precision, form, grid):
     assert np.array_equal(a, b) and np.array_equal(ca, cb)
 
 
+@pytest.mark.parametrize("pattern", ["staged-direct", "direct-staged"])
+def test_mixed_cycle_level_forms_interleaved(torch_mod, pattern):
+    """The correction cycle starts e from zero without clearing it on the
+    staged levels and clears it (fill or restriction) on the others, and
+    folds the FP64 update into the finest staged sweep: with staged and
+    direct forms alternating over the levels, the solve is bit-identical to
+    the all-direct one."""
+    g = CFG1
+    f = wl.wavepacket_rhs(g, tag="interleaved")
+    z = np.zeros_like(f)
+    with context(g) as ctx:
+        lanes = [_lanes(ctx.line_shape(1, lvl)) for lvl in range(1, g.levels)]
+        for lvl in range(1, g.levels):
+            ctx.set_line_shape(1, lvl, lanes[lvl - 1])
+        a, ca, _, _ = ctx.poi_uy(None, -f, z, 1e-9, precision=1, max_cycles=20)
+        first = 0 if pattern == "staged-direct" else 1
+        for lvl in range(1, g.levels):
+            staged = (lvl - 1) % 2 == first
+            ctx.set_line_shape(1, lvl, -lanes[lvl - 1] if staged else lanes[lvl - 1])
+        b, cb, _, _ = ctx.poi_uy(None, -f, z, 1e-9, precision=1, max_cycles=20)
+    assert np.array_equal(a, b) and np.array_equal(ca, cb)
+
+
 @pytest.mark.parametrize("grid", [SMALL, CFG1], ids=["small", "385x97x8"])
 @pytest.mark.parametrize("lanes", [16, 32])
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
