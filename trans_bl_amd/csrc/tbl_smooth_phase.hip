@@ -174,7 +174,9 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
                 const int j = lane + r * G;
                 if (j >= 1 && j < ny) {
                     const cx<T> e = Vs[j];
-                    col[j] = e;
+                    // the updating sweep's last colour is read by no later
+                    // phase, and the next correction solve starts from zero
+                    if (!(UPD && phase == 3)) col[j] = e;
                     // UPD (the last sweep of a correction solve): this
                     // line's e is final, so it is added to the FP64 field
                     // here (same layout), as the update kernel would
