@@ -219,10 +219,16 @@ __device__ __forceinline__ void coarse_chain(const T *rb, T *col, const T *__res
     for (int j = 1; j < NY; ++j) col[2 * j] = fma(rf, Y[j] - col[2 * j], col[2 * j]);
 }
 
+// threads per workgroup: 512, or 1024 when a row set takes 64 lanes (NY >
+// 32: twice the line groups per stage-1 pass; measured 7-10 % faster there,
+// and slower for the shorter lines).  Stage 2 needs 2 threads per line, so
+// the dispatch requires 2 * lines <= kCoarse2sThreads (the smaller count).
 constexpr int kCoarse2sThreads = 512;
+template <int NY>
+constexpr int kCoarse2sThreadsFor = NY > 32 ? 1024 : kCoarse2sThreads;
 
 template <typename T, int NY>
-__global__ void __launch_bounds__(kCoarse2sThreads) smooth_coarsest_2s_kernel(LevelArgs<T> L, cx<T> *__restrict__ V,
+__global__ void __launch_bounds__(kCoarse2sThreadsFor<NY>) smooth_coarsest_2s_kernel(LevelArgs<T> L, cx<T> *__restrict__ V,
                                                                   const cx<T> *__restrict__ S,
                                                                   const T *__restrict__ lu,
                                                                   int sweeps, T rf,
@@ -252,7 +258,7 @@ __global__ void __launch_bounds__(kCoarse2sThreads) smooth_coarsest_2s_kernel(Le
     // the reference's row forms exactly (zero terms add nothing, and the
     // two-term sum of row 2 only commutes)
     constexpr int RW = NY <= 32 ? 32 : 64;
-    constexpr int NG = kCoarse2sThreads / RW;
+    constexpr int NG = kCoarse2sThreadsFor<NY> / RW;
     const int jr = threadIdx.x % RW, cs = threadIdx.x / RW;
     T ca = T(0), cb = T(1), cc = T(0);
     if (jr == 1) {
@@ -360,7 +366,8 @@ int smooth_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu
         bool done = false;
 #define X(NY)                                                                                 \
     if (!done && L.ny == NY) {                                                                \
-        hipLaunchKernelGGL((smooth_coarsest_2s_kernel<T, NY>), dim3(L.K), dim3(kCoarse2sThreads), \
+        hipLaunchKernelGGL((smooth_coarsest_2s_kernel<T, NY>), dim3(L.K),                  \
+                           dim3(kCoarse2sThreadsFor<NY>),                                     \
                            shm, st,                                                           \
                            L, V, S, lu, sweeps, rf, active);                                  \
         done = true;                                                                          \
