@@ -219,13 +219,15 @@ __device__ __forceinline__ void coarse_chain(const T *rb, T *col, const T *__res
     for (int j = 1; j < NY; ++j) col[2 * j] = fma(rf, Y[j] - col[2 * j], col[2 * j]);
 }
 
-// threads per workgroup: 512, or 1024 when a row set takes 64 lanes (NY >
-// 32: twice the line groups per stage-1 pass; measured 7-10 % faster there,
-// and slower for the shorter lines).  Stage 2 needs 2 threads per line, so
-// the dispatch requires 2 * lines <= kCoarse2sThreads (the smaller count).
+// threads per workgroup: 512 for NY <= 16, 768 up to 32 rows, 1024 when a
+// row set takes 64 lanes (NY > 32) -- more line groups per stage-1 pass;
+// measured per sweep on 129-column coarsest levels: NY = 25 2-5 % and
+// NY = 33 7-10 % faster than 512, while 1024 at NY = 25 and 768 at NY = 13
+// were slower.  Stage 2 needs 2 threads per line, so the dispatch requires
+// 2 * lines <= kCoarse2sThreads (the smallest count).
 constexpr int kCoarse2sThreads = 512;
 template <int NY>
-constexpr int kCoarse2sThreadsFor = NY > 32 ? 1024 : kCoarse2sThreads;
+constexpr int kCoarse2sThreadsFor = NY > 32 ? 1024 : NY > 16 ? 768 : kCoarse2sThreads;
 
 template <typename T, int NY>
 __global__ void __launch_bounds__(kCoarse2sThreadsFor<NY>) smooth_coarsest_2s_kernel(LevelArgs<T> L, cx<T> *__restrict__ V,
