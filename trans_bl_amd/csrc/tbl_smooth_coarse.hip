@@ -222,8 +222,7 @@ __device__ __forceinline__ void coarse_chain(const T *rb, T *col, const T *__res
 // threads per workgroup: 512 for NY <= 16, 768 up to 32 rows, 1024 when a
 // row set takes 64 lanes (NY > 32) -- more line groups per stage-1 pass;
 // measured per sweep on 129-column coarsest levels: NY = 25 2-5 % and
-// NY = 33 7-10 % faster than 512, while 1024 at NY = 25 and 768 at NY = 13
-// were slower.  Stage 2 needs 2 threads per line, so the dispatch requires
+// NY = 33 7-10 % faster than 512, while 1024 was slower at NY = 25 and 13.  Stage 2 needs 2 threads per line, so the dispatch requires
 // 2 * lines <= kCoarse2sThreads (the smallest count).
 constexpr int kCoarse2sThreads = 512;
 template <int NY>
