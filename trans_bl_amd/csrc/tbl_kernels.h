@@ -84,8 +84,10 @@ int coarse_kernel_attributes();
 // zero_start: the sweeps start from v = 0 whatever v's memory holds (the
 // first sweep reads no column it has not relaxed itself); vup: the last
 // sweep also adds each relaxed value to this FP64 field of the same layout
-// (vup += v on rows 1.., interior columns -- the correction cycle's update).
-// Staged shapes (G < 0) only; any other shape returns TBL_ERR_ARG.
+// (vup += v on rows 1.., interior columns -- the correction cycle's update)
+// and does not store its last colour's lines into V, which nothing reads
+// afterwards (V is left incomplete).  Staged shapes (G < 0) only, vup FP32
+// only; any other use returns TBL_ERR_ARG.
 template <typename T>
 int smooth_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
                   const int *active, int G, cx<T> *halo, hipStream_t st, bool zero_start = false,
