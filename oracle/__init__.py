@@ -35,6 +35,17 @@ class Params(C.Structure):
     ]
 
 
+class Metrics(C.Structure):
+    """or_metrics: create_ctes (main_par.f:1828-1892) per level."""
+    _fields_ = [
+        ("ptsx", C.c_int * MAXLVL), ("ptsy", C.c_int * MAXLVL),
+        ("dy0", C.c_double * MAXLVL), ("stf", C.c_double * MAXLVL),
+        ("dx2", C.c_double * MAXLVL),
+        ("qdy", C.c_double * 1024),
+        ("qdy2", (C.c_double * 1024) * MAXLVL),
+    ]
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
@@ -55,10 +66,24 @@ def lib():
             getattr(L, name).restype = C.c_int
         L.or_poi_uy.argtypes = [P, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp]
         L.or_poi_uy.restype = C.c_int
+        L.or_poi_uy_modes.argtypes = [P, vp, vp, vp, C.c_double, C.c_int, C.c_int, vp, vp, vp,
+                                      C.c_int]
+        L.or_poi_uy_modes.restype = C.c_int
         L.or_apply_L_level.argtypes = [P, C.c_int, C.c_int, vp, vp]
         L.or_apply_L_level.restype = C.c_int
         L.or_band5_factor.argtypes = [vp, C.c_int, vp]
         L.or_band5_solve.argtypes = [vp, C.c_int, vp, vp]
+        M = C.POINTER(Metrics)
+        L.or_create_ctes.argtypes = [P, M]
+        L.or_create_ctes.restype = C.c_int
+        L.or_restrict.argtypes = [P, M, C.c_int, vp, vp, vp, vp, vp]
+        L.or_restrict.restype = C.c_int
+        L.or_correct_prolong.argtypes = [P, M, C.c_int, vp, vp, vp]
+        L.or_correct_prolong.restype = C.c_int
+        L.or_source.argtypes = [P, C.c_int, vp, vp, vp]
+        L.or_source.restype = C.c_int
+        L.or_sor.argtypes = [P, M, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int]
+        L.or_sor.restype = C.c_int
         _lib = L
     return _lib
 
@@ -128,6 +153,26 @@ def poi_uy(p, wx, dwzdx, uy, erromax, max_cycles=11, order=0):
     return out, cyc, res
 
 
+def poi_uy_modes(p, wx, dwzdx, uy, erromax, modes, max_cycles=11, order=0):
+    """or_poi_uy on the listed modes only (0-based mode indices, as the
+    fields' first axis).  Returns (uy_out, cycles, residuals) for all modes;
+    entries of unlisted modes are the inputs' (cycles 0)."""
+    wx = np.ascontiguousarray(wx, dtype=np.complex128)
+    dz = np.ascontiguousarray(dwzdx, dtype=np.complex128)
+    out = np.array(uy, dtype=np.complex128, order="C", copy=True)
+    cyc = np.zeros(p.nk, dtype=np.int32)
+    res = np.zeros(p.nk, dtype=np.float64)
+    sel = np.ascontiguousarray(np.asarray(modes, dtype=np.int32) + 1)
+    vp = C.c_void_p
+    rc = lib().or_poi_uy_modes(C.byref(p), wx.ctypes.data_as(vp), dz.ctypes.data_as(vp),
+                               out.ctypes.data_as(vp), erromax, max_cycles, order,
+                               cyc.ctypes.data_as(vp), res.ctypes.data_as(vp),
+                               sel.ctypes.data_as(vp), len(sel))
+    if rc:
+        raise ValueError(f"or_poi_uy_modes rejected the call (rc={rc})")
+    return out, cyc, res
+
+
 def apply_L(p, k, v, lvl=1):
     """Operator L of mode k (1-based) on level lvl; v complex (ny_l, nx_l)."""
     v = np.ascontiguousarray(v, dtype=np.complex128)
@@ -152,3 +197,69 @@ def band5_solve(a, al, rhs):
     lib().or_band5_solve(a.ctypes.data_as(C.c_void_p), a.shape[0],
                          al.ctypes.data_as(C.c_void_p), rhs.ctypes.data_as(C.c_void_p))
     return rhs
+
+
+def metrics(p):
+    """create_ctes of the oracle: level sizes, spacings and metrics."""
+    m = Metrics()
+    if lib().or_create_ctes(C.byref(p), C.byref(m)):
+        raise ValueError("or_create_ctes rejected the grid")
+    return m
+
+
+def level_shape(p, lvl):
+    """(ny_l, nx_l) of multigrid level lvl (1-based)."""
+    m = metrics(p)
+    return m.ptsy[lvl - 1], m.ptsx[lvl - 1]
+
+
+def restrict(p, lvl, vf, rf, vc=None, wc=None, Sc=None):
+    """restrict (poisson_par.f:796-874) from level lvl to lvl+1 on one mode:
+    vf, rf complex (ny_l, nx_l); returns (vc, wc, Sc) of the coarse level
+    (entries the routine does not write keep the given arrays' values, zero
+    by default)."""
+    m = metrics(p)
+    shp = (m.ptsy[lvl], m.ptsx[lvl])
+    out = []
+    for a in (vc, wc, Sc):
+        out.append(np.zeros(shp, dtype=np.complex128) if a is None
+                   else np.array(a, dtype=np.complex128, order="C", copy=True))
+    vf, pvf = _c(np.asarray(vf, dtype=np.complex128))
+    rf, prf = _c(np.asarray(rf, dtype=np.complex128))
+    vp = C.c_void_p
+    lib().or_restrict(C.byref(p), C.byref(m), lvl, pvf, prf,
+                      *[a.ctypes.data_as(vp) for a in out])
+    return tuple(out)
+
+
+def correct_prolong(p, lvl, vc, wc, vf):
+    """correction + intpol (poisson_par.f:877-942): lvl is the coarse level;
+    returns (vf + P(vc - wc), vc after the correction step)."""
+    m = metrics(p)
+    vc = np.array(vc, dtype=np.complex128, order="C", copy=True)
+    vf = np.array(vf, dtype=np.complex128, order="C", copy=True)
+    wc, pwc = _c(np.asarray(wc, dtype=np.complex128))
+    vp = C.c_void_p
+    lib().or_correct_prolong(C.byref(p), C.byref(m), lvl, vc.ctypes.data_as(vp), pwc,
+                             vf.ctypes.data_as(vp))
+    return vf, vc
+
+
+def source(p, k, wx, dwzdx):
+    """function_source (poisson_par.f:111-164) of mode k (1-based): wx, dwzdx
+    complex (nk, ny, nx); returns S (ny, nx)."""
+    wx, pw = _c(np.asarray(wx, dtype=np.complex128))
+    dz, pd = _c(np.asarray(dwzdx, dtype=np.complex128))
+    S = np.zeros((p.ny, p.nx), dtype=np.complex128)
+    lib().or_source(C.byref(p), k, pw, pd, S.ctypes.data_as(C.c_void_p))
+    return S
+
+
+def sor(p, lvl, k, v, S, itera, order=0):
+    """itera line-relaxation sweeps (sor, poisson_par.f:167-226) of mode k on
+    level lvl; returns the relaxed v."""
+    m = metrics(p)
+    v = np.array(v, dtype=np.complex128, order="C", copy=True)
+    S, pS = _c(np.asarray(S, dtype=np.complex128))
+    lib().or_sor(C.byref(p), C.byref(m), lvl, k, v.ctypes.data_as(C.c_void_p), pS, itera, order)
+    return v

@@ -623,16 +623,58 @@ static int check_grid(const or_params *p)
     return 0;
 }
 
-/* poi_uy: poisson_par.f:6-64 */
-int or_poi_uy(const or_params *p, const or_cplx *wx, const or_cplx *dwzdx,
-              or_cplx *uy, double erromax, int max_cycles, int order,
-              int *cycles, double *res)
+/* poi_uy: poisson_par.f:6-64, the body of the mode loop (poisson_par.f:30)
+   for mode k (1-based) with work planes allocated per call */
+static int poi_uy_mode(const or_params *p, const or_metrics *m, int k, const or_cplx *wx,
+                       const or_cplx *dwzdx, or_cplx *uy, double erromax, int max_cycles,
+                       int order, int *cycles, double *res, or_cplx **V, or_cplx **S,
+                       or_cplx **W, or_cplx **R)
+{
+    int msh = p->msh, nx = p->nx, ny = p->ny;
+    size_t off = (size_t)nx * ny * (k - 1);
+    or_source(p, k, wx, dwzdx, S[0]);
+    memcpy(V[0], uy + off, sizeof(or_cplx) * nx * ny);
+    double err = 0.0;
+    int done = 0, vc;
+    for (vc = 1; vc <= max_cycles && !done; ++vc) {
+        for (int l = 1; l <= msh - 1; ++l) {
+            or_sor(p, m, l, k, V[l - 1], S[l - 1], 3, order);
+            or_defect(p, m, l, k, V[l - 1], S[l - 1], R[l - 1], -1.0);
+            if (l == 1) {   /* verifica_poi + get_max_value, 67-108, 945-963 */
+                err = 0.0;
+                for (int j = 2; j <= ny; ++j)
+                    for (int i = 2; i <= nx - 1; ++i) {
+                        double a = cabs(AT(R[0], nx, i, j));
+                        if (a > err) err = a;
+                    }
+                if (err < erromax) { done = 1; break; }
+            }
+            or_restrict(p, m, l, V[l - 1], R[l - 1], V[l], W[l], S[l]);
+            or_defect(p, m, l + 1, k, V[l], S[l], S[l], 1.0);
+        }
+        if (done) break;
+        or_sor(p, m, msh, k, V[msh - 1], S[msh - 1], 60, order);
+        for (int l = msh; l >= 2; --l) {
+            or_correct_prolong(p, m, l, V[l - 1], W[l - 1], V[l - 2]);
+            or_sor(p, m, l - 1, k, V[l - 2], S[l - 2], 1, order);
+        }
+    }
+    if (cycles) cycles[k - 1] = done ? vc - 1 : max_cycles;
+    if (res) res[k - 1] = err;
+    for (int j = 2; j <= ny; ++j)
+        for (int i = 2; i <= nx - 1; ++i) AT(uy + off, nx, i, j) = AT(V[0], nx, i, j);
+    return 0;
+}
+
+static int poi_uy_modes(const or_params *p, const or_cplx *wx, const or_cplx *dwzdx,
+                        or_cplx *uy, double erromax, int max_cycles, int order,
+                        int *cycles, double *res, const int *modes, int nsel)
 {
     int rc = check_grid(p);
     if (rc) return rc;
     or_metrics m;
     if (or_create_ctes(p, &m)) return -1;
-    int msh = p->msh, nx = p->nx, ny = p->ny;
+    int msh = p->msh;
     or_cplx *V[OR_MAXLVL], *S[OR_MAXLVL], *W[OR_MAXLVL], *R[OR_MAXLVL];
     for (int l = 0; l < msh; ++l) {
         size_t n = (size_t)m.ptsx[l] * m.ptsy[l];
@@ -641,44 +683,30 @@ int or_poi_uy(const or_params *p, const or_cplx *wx, const or_cplx *dwzdx,
         W[l] = calloc(n, sizeof(or_cplx));
         R[l] = calloc(n, sizeof(or_cplx));
     }
-    for (int k = 1; k <= p->nk; ++k) {
-        size_t off = (size_t)nx * ny * (k - 1);
-        or_source(p, k, wx, dwzdx, S[0]);
-        memcpy(V[0], uy + off, sizeof(or_cplx) * nx * ny);
-        double err = 0.0;
-        int done = 0, vc;
-        for (vc = 1; vc <= max_cycles && !done; ++vc) {
-            for (int l = 1; l <= msh - 1; ++l) {
-                or_sor(p, &m, l, k, V[l - 1], S[l - 1], 3, order);
-                or_defect(p, &m, l, k, V[l - 1], S[l - 1], R[l - 1], -1.0);
-                if (l == 1) {   /* verifica_poi + get_max_value, 67-108, 945-963 */
-                    err = 0.0;
-                    for (int j = 2; j <= ny; ++j)
-                        for (int i = 2; i <= nx - 1; ++i) {
-                            double a = cabs(AT(R[0], nx, i, j));
-                            if (a > err) err = a;
-                        }
-                    if (err < erromax) { done = 1; break; }
-                }
-                or_restrict(p, &m, l, V[l - 1], R[l - 1], V[l], W[l], S[l]);
-                or_defect(p, &m, l + 1, k, V[l], S[l], S[l], 1.0);
-            }
-            if (done) break;
-            or_sor(p, &m, msh, k, V[msh - 1], S[msh - 1], 60, order);
-            for (int l = msh; l >= 2; --l) {
-                or_correct_prolong(p, &m, l, V[l - 1], W[l - 1], V[l - 2]);
-                or_sor(p, &m, l - 1, k, V[l - 2], S[l - 2], 1, order);
-            }
-        }
-        if (cycles) cycles[k - 1] = done ? vc - 1 : max_cycles;
-        if (res) res[k - 1] = err;
-        for (int j = 2; j <= ny; ++j)
-            for (int i = 2; i <= nx - 1; ++i) AT(uy + off, nx, i, j) = AT(V[0], nx, i, j);
+    for (int s = 0; s < nsel; ++s) {
+        int k = modes ? modes[s] : s + 1;
+        if (k < 1 || k > p->nk) { rc = -5; break; }
+        poi_uy_mode(p, &m, k, wx, dwzdx, uy, erromax, max_cycles, order, cycles, res, V, S, W,
+                    R);
     }
     for (int l = 0; l < msh; ++l) {
         free(V[l]); free(S[l]); free(W[l]); free(R[l]);
     }
-    return 0;
+    return rc;
+}
+
+int or_poi_uy(const or_params *p, const or_cplx *wx, const or_cplx *dwzdx,
+              or_cplx *uy, double erromax, int max_cycles, int order,
+              int *cycles, double *res)
+{
+    return poi_uy_modes(p, wx, dwzdx, uy, erromax, max_cycles, order, cycles, res, NULL, p->nk);
+}
+
+int or_poi_uy_modes(const or_params *p, const or_cplx *wx, const or_cplx *dwzdx,
+                    or_cplx *uy, double erromax, int max_cycles, int order,
+                    int *cycles, double *res, const int *modes, int nsel)
+{
+    return poi_uy_modes(p, wx, dwzdx, uy, erromax, max_cycles, order, cycles, res, modes, nsel);
 }
 
 int or_apply_L_level(const or_params *p, int lvl, int k, const or_cplx *v, or_cplx *out)

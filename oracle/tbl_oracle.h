@@ -73,6 +73,12 @@ int or_poi_uy(const or_params *p, const or_cplx *wx, const or_cplx *dwzdx,
               or_cplx *uy, double erromax, int max_cycles, int order,
               int *cycles, double *res);
 
+/* same for the listed modes only (1-based k; the others of uy untouched,
+   cycles/res written at index k-1) */
+int or_poi_uy_modes(const or_params *p, const or_cplx *wx, const or_cplx *dwzdx,
+                    or_cplx *uy, double erromax, int max_cycles, int order,
+                    int *cycles, double *res, const int *modes, int nsel);
+
 /* finest-level operator for one mode, all points (i=2..nx-1, j=2..ny get
    (L v); others 0) -- used to assemble the dense system in tests. */
 int or_apply_L(const or_params *p, int k, const or_cplx *v, or_cplx *out);

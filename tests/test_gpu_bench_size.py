@@ -1,0 +1,101 @@
+"""GPU parity at the benchmark's full size, 1025x193 with 64 modes
+(kz = 0.1 k, dy0 = 2e-4, stretch 1.015), on the wavepacket right-hand side.
+
+The oracle runs the selected modes k = 0, 1, 31, 63 (the GPU always solves
+all 64 together):
+
+* FP64 iterates after 3 V-cycles against the oracle with the GPU's colour
+  line order: 1e-12 relative;
+* the FP64 solution after 14 cycles (at the FP64 floor) against the oracle
+  with the reference's lexicographic order: the two orders reach the same
+  discrete solution, and the relative l2 difference is printed and bounded;
+* the four derivatives of the GPU's solution against the oracle's
+  derivatives of the same field, all 64 modes: 1e-12 relative;
+* FP32 smoothing under FP64 defect correction to the 1e-9 residual: v and
+  its four derivatives against the oracle's FP64 fixed point (relative l2
+  printed; v within 1e-8).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import trans_bl_amd as tb
+from trans_bl_amd import workload as wl
+from conftest import rel, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+G = wl.BATCH_1025x193x64
+MODES = [0, 1, 31, 63]
+KINDS = ("x", "xx", "y", "yy")
+
+
+@pytest.fixture(scope="module")
+def setup(gpu_ready):
+    coefs = oracle.coefgen.generate(G.stf, G.levels)
+    p = oracle.make_params(G.nx, G.ny, G.nmodes, G.levels, G.dx, G.dy0, G.stf, G.alpha,
+                           G.beta, G.fac_y, coefs=coefs)
+    f = wl.wavepacket_rhs(G, tag="cfg2-parity")
+    ctx = tb.Context(G, coefs=coefs)
+    yield p, f, ctx
+    ctx.close()
+
+
+@pytest.fixture(scope="module")
+def fixed_point(setup):
+    p, f, ctx = setup
+    z = np.zeros_like(f)
+    ref, _, rres = oracle.poi_uy_modes(p, z, -f, z, 0.0, MODES, max_cycles=14, order=0)
+    got, cyc, res, _ = ctx.poi_uy(None, -f, z, 0.0, max_cycles=14)
+    return ref, got, rres, res
+
+
+def test_fp64_iterates_match_colour_oracle(setup):
+    p, f, ctx = setup
+    z = np.zeros_like(f)
+    ref, rc, _ = oracle.poi_uy_modes(p, z, -f, z, 0.0, MODES, max_cycles=3, order=1)
+    got, cyc, _, _ = ctx.poi_uy(None, -f, z, 0.0, max_cycles=3)
+    assert np.all(cyc == 3)
+    for k in MODES:
+        e = rel(got[k], ref[k])
+        print(f"1025x193x64 FP64 3-cycle iterate, mode {k}: rel {e:.2e}")
+        assert e < 1e-12, (k, e)
+
+
+def test_fp64_fixed_point_matches_reference_order(fixed_point):
+    ref, got, rres, res = fixed_point
+    for k in MODES:
+        e = rel_l2(got[k], ref[k])
+        print(f"1025x193x64 FP64 fixed point vs lexicographic oracle, mode {k}: rel l2 {e:.2e} "
+              f"(residuals {res[k]:.1e} GPU, {rres[k]:.1e} oracle)")
+        assert e < 1e-12, (k, e)
+
+
+def test_derivatives_match_oracle(setup, fixed_point):
+    p, f, ctx = setup
+    _, got, _, _ = fixed_point
+    for kind in KINDS + ("yfv",):
+        d = getattr(ctx, {"x": "derparx", "xx": "derparxx", "y": "dery", "yy": "deryy",
+                          "yfv": "deryfv"}[kind])(got)
+        dref = oracle.deriv(p, kind, got)
+        e = rel(d, dref)
+        print(f"1025x193x64 derivative {kind} (all modes): rel {e:.2e}")
+        assert e < 1e-12, (kind, e)
+
+
+def test_mixed_precision_against_fp64_oracle(setup, fixed_point):
+    p, f, ctx = setup
+    ref, _, _, _ = fixed_point
+    z = np.zeros_like(f)
+    got, cyc, res, _ = ctx.poi_uy(None, -f, z, 1e-9, precision=1, guess=1)
+    assert np.all(res < 1e-9)
+    ev = max(rel_l2(got[k], ref[k]) for k in MODES)
+    print(f"1025x193x64 mixed precision v vs FP64 oracle (modes {MODES}): rel l2 {ev:.2e}")
+    assert ev < 1e-8
+    for kind in KINDS:
+        d = getattr(ctx, {"x": "derparx", "xx": "derparxx", "y": "dery",
+                          "yy": "deryy"}[kind])(got)
+        dref = oracle.deriv(p, kind, ref)
+        e = max(rel_l2(d[k], dref[k]) for k in MODES)
+        print(f"1025x193x64 mixed precision d/{kind} vs FP64 oracle: rel l2 {e:.2e}")
+        assert e < 1e-8, (kind, e)

@@ -124,3 +124,23 @@ def test_training_reduces_loss(torch_mod):
     for _ in range(60):
         last = net.train_step(xd.data_ptr(), td.data_ptr(), 4, 1e-3)
     assert last < 0.3 * first, (first, last)
+
+
+def test_set_params_after_forward_repacks_weights(torch_mod):
+    """New weights given after a forward reach every layer, the 3x3
+    convolutions' packed copies included."""
+    torch = torch_mod
+    from trans_bl_amd import guess
+    rng = np.random.default_rng(21)
+    x, _ = _problem(rng, n=1)
+    net = guess.Net(16, 24, 1, seed=5)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.zeros((1, 16, 24, 2), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    net.forward(xd.data_ptr(), yd.data_ptr(), 1)            # packs the first weights
+    theta = U.to_bf16(guess.Net(16, 24, 1, seed=77).get_params())
+    net.set_params(theta.astype(np.float32))
+    net.forward(xd.data_ptr(), yd.data_ptr(), 1)
+    out = U.UNet(theta, bf16=True).forward(x)
+    y = yd.cpu().numpy()
+    assert np.linalg.norm(y - out) / np.linalg.norm(out) < 1e-2

@@ -276,6 +276,11 @@ int check_grid(const tbl_grid &g)
     // independent and rows 1..5 to exist
     if ((g.nx - 1) / d + 1 < 8 || (g.ny - 1) / d + 1 < 5) return TBL_ERR_GRID;
     if (g.ny > kMaxNy) return TBL_ERR_GRID;
+    // kernels address up to two modes' planes of a level through one buffer
+    // resource (32-bit byte offsets, 0x80000000 the out-of-range offset):
+    // keep four finest-level FP64 planes below 2 GiB
+    const long long plane = static_cast<long long>(g.nx) * (((g.ny + 7) / 8) * 8);
+    if (4 * plane * 16 >= (1LL << 31)) return TBL_ERR_GRID;
     return TBL_OK;
 }
 

@@ -288,6 +288,10 @@ extern "C" int tbl_net_create(int H, int W, int max_batch, unsigned seed, tbl_ne
 {
     if (!out || H < 8 || W < 8 || H % 8 || W % 8 || max_batch < 1) return TBL_ERR_ARG;
     *out = nullptr;
+    // the convolutions address an activation tensor (at most 128 bf16
+    // channels per pixel) through one buffer resource: 32-bit byte offsets,
+    // with 0x80000000 as the out-of-range offset of a masked load
+    if (static_cast<long long>(max_batch) * H * W * 128 * 2 >= (1LL << 31)) return TBL_ERR_ARG;
     tbl_net *n = new (std::nothrow) tbl_net();
     if (!n) return TBL_ERR_ALLOC;
     n->H = H;
@@ -344,6 +348,7 @@ extern "C" int tbl_net_set_params(tbl_net *n, const float *host)
     if (!n || !host) return TBL_ERR_ARG;
     TBL_HIP(hipMemcpy(n->w, host, sizeof(float) * n->nparam, hipMemcpyHostToDevice));
     to_bf16(n->nparam, n->w, n->wb, n->stream);
+    n->wp_stale = true;   // the next forward repacks the 3x3 weights
     TBL_HIP(hipStreamSynchronize(n->stream));
     return TBL_OK;
 }

@@ -25,6 +25,15 @@ cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
       integer l
 
       if (c_associated(tbl_ctx_handle)) return
+      ! my_form 0 and 4 (immersed, LES) give the mean mode's top row a
+      ! source from the base flow (duexmdx, uyb; poisson_par.f:145-153)
+      ! that this layer does not pass: refuse them instead of solving a
+      ! different top row
+      if (my_form .eq. 0 .or. my_form .eq. 4) then
+        write(*,*) 'tbl_setup: my_form ', my_form,
+     &             ' not supported (only 1 and 2)'
+        stop
+      end if
       grid%nx = ptsx
       grid%ny = jmax
       grid%nmodes = kfour
