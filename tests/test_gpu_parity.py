@@ -182,7 +182,59 @@ def test_grid_rejection(gpu_ready):
 
 
 def _lanes(code):
+    if code > 3000:
+        return (code - 3000) % 100
     return code - 2000 if code > 2000 else code - 1000 if code > 1000 else abs(code)
+
+
+@pytest.mark.parametrize("grid", [SMALL, CFG1], ids=["small", "385x97x8"])
+@pytest.mark.parametrize("wsel", [0, 1, 5])
+def test_one_launch_sweep_matches_direct(torch_mod, wsel, grid):
+    """The one-launch out-of-place sweep (all four colour phases of a strip
+    in one workgroup, halo lines recomputed) relaxes the same lines in the
+    same order with the same arithmetic as the phase-per-launch forms: the
+    correction cycle's solve is bit-identical, the zero-start first sweep and
+    the fused FP64 update included."""
+    g = grid
+    f = wl.wavepacket_rhs(g, tag="sweep")
+    z = np.zeros_like(f)
+    with context(g) as ctx:
+        lanes = [_lanes(ctx.line_shape(1, lvl)) for lvl in range(1, g.levels)]
+        for lvl in range(1, g.levels):
+            ctx.set_line_shape(1, lvl, lanes[lvl - 1])
+        a, ca, _, _ = ctx.poi_uy(None, -f, z, 1e-9, precision=1, max_cycles=20)
+        a3, _, _, _ = ctx.poi_uy(None, -f, z, 0.0, precision=1, max_cycles=3, use_graph=1)
+        for lvl in range(1, g.levels):
+            ctx.set_line_shape(1, lvl, 3000 + 100 * wsel + lanes[lvl - 1])
+        b, cb, _, _ = ctx.poi_uy(None, -f, z, 1e-9, precision=1, max_cycles=20)
+        b3, _, _, _ = ctx.poi_uy(None, -f, z, 0.0, precision=1, max_cycles=3, use_graph=1)
+        # FP64 cycles do not take the one-launch form
+        with pytest.raises(tb.TblError):
+            ctx.set_line_shape(0, 1, 3000 + lanes[0])
+    assert np.array_equal(a, b) and np.array_equal(ca, cb)
+    assert np.array_equal(a3, b3)
+
+
+@pytest.mark.parametrize("pattern", ["sweep-direct", "direct-sweep", "sweep-staged"])
+def test_one_launch_sweep_interleaved_with_other_forms(torch_mod, pattern):
+    """Levels on the out-of-place sweep (e moving between two buffers) next to
+    levels smoothed in place: same bits as all-direct."""
+    g = CFG1
+    f = wl.wavepacket_rhs(g, tag="sweep-mix")
+    z = np.zeros_like(f)
+    with context(g) as ctx:
+        lanes = [_lanes(ctx.line_shape(1, lvl)) for lvl in range(1, g.levels)]
+        for lvl in range(1, g.levels):
+            ctx.set_line_shape(1, lvl, lanes[lvl - 1])
+        a, ca, _, _ = ctx.poi_uy(None, -f, z, 1e-9, precision=1, max_cycles=20)
+        first, other = pattern.split("-")
+        for lvl in range(1, g.levels):
+            form = first if (lvl - 1) % 2 == 0 else other
+            code = {"sweep": 3000 + lanes[lvl - 1], "direct": lanes[lvl - 1],
+                    "staged": -lanes[lvl - 1]}[form]
+            ctx.set_line_shape(1, lvl, code)
+        b, cb, _, _ = ctx.poi_uy(None, -f, z, 1e-9, precision=1, max_cycles=20)
+    assert np.array_equal(a, b) and np.array_equal(ca, cb)
 
 
 @pytest.mark.parametrize("grid", [SMALL, CFG1], ids=["small", "385x97x8"])

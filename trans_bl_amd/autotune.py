@@ -3,7 +3,8 @@
 For every multigrid level and both smoother precisions it times each valid
 line shape -- 16, 32 or 64 lanes of a wavefront per y-line, with direct
 (lane-contiguous) or LDS-staged (coalesced) global access per colour phase,
-or the fused-sweep strip form (whole sweep per launch in LDS) -- on the
+the fused-sweep strip form (whole sweep per launch in LDS), or the one-launch
+out-of-place sweep (a strip of columns per workgroup, all colour phases) -- on the
 context's own grid with hipEvents (tbl_time_smoother), and keeps the fastest.
 Results are per size class (level shape), so contexts of the same grid reuse
 them through the JSON cache.
@@ -15,11 +16,12 @@ import os
 
 from . import TblError
 
-# direct, LDS-staged (negative), x-terms-staged (2000 + lanes) and
-# fused-sweep strip (1000 + lanes) forms; 1 = the coarsest level's sequential
-# kernels (rejected on other levels; any lane form there selects the
-# lane-scan coarsest kernel)
-CANDIDATES = (1, 64, 32, 16, -64, -32, -16, 2064, 2032, 2016, 1064, 1032, 1016)
+# direct, LDS-staged (negative), x-terms-staged (2000 + lanes), fused-sweep
+# strip (1000 + lanes) and one-launch out-of-place sweep (3000 + lanes; FP32
+# only) forms; 1 = the coarsest level's sequential kernels (rejected on
+# other levels; any lane form there selects the lane-scan coarsest kernel)
+CANDIDATES = (1, 64, 32, 16, -64, -32, -16, 2064, 2032, 2016, 1064, 1032, 1016, 3064, 3032,
+              3016)
 
 
 def size_class(ctx, level):

@@ -35,6 +35,7 @@ SOURCES = [
     "tbl_smooth_phase.hip",
     "tbl_smooth_strip.hip",
     "tbl_smooth_coarse.hip",
+    "tbl_smooth_sweep.hip",
     "tbl_deriv_kernels.hip",
     "tbl_unet.hip",
     "tbl_net.cpp",

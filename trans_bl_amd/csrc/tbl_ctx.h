@@ -22,8 +22,9 @@ struct tbl_ctx {
     std::vector<tbl::Level> lv;
     // FP64 multigrid state per level: v, source, FAS injected copy, residual
     std::vector<tbl::cx<double> *> V, S, W, R;
-    // FP32 correction-scheme state per level (mixed precision)
-    std::vector<tbl::cx<float> *> E32, S32, R32;
+    // FP32 correction-scheme state per level (mixed precision); E32b is
+    // the second e buffer of the out-of-place sweep (all but the coarsest)
+    std::vector<tbl::cx<float> *> E32, S32, R32, E32b;
     std::vector<double *> qdy2_d;
     std::vector<float *> qdy2_f;
     std::vector<double *> lu_d;   // [K][2][5][ld] per level

@@ -56,11 +56,20 @@ constexpr int kCoarseSeqShape = 1;
 // sequential coarsest form
 inline int sweep_shape(int code) { return code == kCoarseSeqShape ? 16 : code; }
 
-// Line-shape codes: G, -G, kStripShape + G or kXStagedShape + G.
+// One-launch, out-of-place sweep (smooth_sweep_kernel): line shape
+// kSweepShape + 100 * wsel + G, G lanes per line, wsel the strip-width
+// selector (0 = default width).
+constexpr int kSweepShape = 3000;
+inline bool shape_is_sweep(int code) { return code > kSweepShape && code < kSweepShape + 600; }
+inline int sweep_wsel(int code) { return (code - kSweepShape) / 100; }
+
+// Line-shape codes: G, -G, kStripShape + G, kXStagedShape + G or
+// kSweepShape + 100 * wsel + G.
 inline bool shape_is_strip(int code) { return code > kStripShape && code < kXStagedShape; }
 inline int shape_lanes(int code)
 {
-    return code > kXStagedShape ? code - kXStagedShape
+    return shape_is_sweep(code) ? (code - kSweepShape) % 100
+           : code > kXStagedShape ? code - kXStagedShape
            : code > kStripShape ? code - kStripShape
            : code < 0           ? -code
                                 : code;
@@ -77,6 +86,7 @@ inline size_t strip_halo_elems(const StripGeom &g, int ld, int K)
     return g.nstrips > 1 ? static_cast<size_t>(K) * (g.nstrips - 1) * 2 * kStripHalo * ld : 0;
 }
 int set_kernel_attributes();
+int sweep_kernel_attributes();
 int phase_kernel_attributes();
 int strip_kernel_attributes();
 int coarse_kernel_attributes();
@@ -92,6 +102,15 @@ template <typename T>
 int smooth_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
                   const int *active, int G, cx<T> *halo, hipStream_t st, bool zero_start = false,
                   cx<double> *vup = nullptr);
+// `sweeps` one-launch sweeps alternating between Va and Vb (out of place),
+// starting from Va, or from zero with zero_start; *cur receives the buffer
+// that holds the result.  vup: as for smooth_sweeps (FP32 only).
+template <typename T>
+int sweep_sweeps(const LevelArgs<T> &L, cx<T> *Va, cx<T> *Vb, const cx<T> *S, const T *lu,
+                 int sweeps, T rf, const int *active, int G, int wsel, hipStream_t st,
+                 bool zero_start, cx<double> *vup, cx<T> **cur);
+int sweep_strip_width(int nx, int K, int wsel);
+size_t sweep_lds(int ld, size_t esize, int G);   // dynamic LDS of one workgroup
 template <typename T>
 int phase_sweeps(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu, int sweeps, T rf,
                  const int *active, int G, hipStream_t st, bool zero_start = false,

@@ -466,6 +466,7 @@ int set_kernel_attributes()
     int rc = phase_kernel_attributes();
     if (rc == TBL_OK) rc = strip_kernel_attributes();
     if (rc == TBL_OK) rc = coarse_kernel_attributes();
+    if (rc == TBL_OK) rc = sweep_kernel_attributes();
     return rc;
 }
 

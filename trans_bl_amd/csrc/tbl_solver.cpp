@@ -110,6 +110,7 @@ static int create_impl(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx *c)
     for (int l = 0; l < L; ++l) c->lv[l] = make_level(c->m.nx[l], c->m.ny[l], c->m.dx2[l]);
     c->V.assign(L, nullptr); c->S.assign(L, nullptr); c->W.assign(L, nullptr); c->R.assign(L, nullptr);
     c->E32.assign(L, nullptr); c->S32.assign(L, nullptr); c->R32.assign(L, nullptr);
+    c->E32b.assign(L, nullptr);
     c->qdy2_d.assign(L, nullptr); c->qdy2_f.assign(L, nullptr);
     c->lu_d.assign(L, nullptr); c->lu_f.assign(L, nullptr);
     c->G64.assign(L, 0); c->G32.assign(L, 0);
@@ -121,7 +122,10 @@ static int create_impl(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx *c)
         TRY(dalloc(&c->S[l], n));
         TRY(dalloc(&c->W[l], n));
         TRY(dalloc(&c->R[l], n));
-        TRY(dalloc(&c->E32[l], n));
+        // (E32b: the second half of E32's allocation, for the out-of-place
+        // sweep's single buffer resource over both)
+        TRY(dalloc(&c->E32[l], l < L - 1 ? 2 * n : n));
+        if (l < L - 1) c->E32b[l] = c->E32[l] + n;
         TRY(dalloc(&c->S32[l], n));
         TRY(dalloc(&c->R32[l], n));
         // metric v_qdy2(:, l)
@@ -407,6 +411,23 @@ int fas_cycle(tbl_ctx *c, const Sched &s, double tol)
     return TBL_OK;
 }
 
+// FP32 smoothing sweeps of level l of the correction cycle on e = *cur: in
+// place, or out of place between E32 and E32b with the one-launch sweep
+// (then *cur moves to the buffer that holds the result)
+int mixed_sweeps(tbl_ctx *c, int l, cx<float> **cur, int n, bool zs, cx<double> *vup)
+{
+    const int G = c->G32[l];
+    const float rf = static_cast<float>(rf_of(n));
+    if (shape_is_sweep(G)) {
+        cx<float> *alt = *cur == c->E32[l] ? c->E32b[l] : c->E32[l];
+        return sweep_sweeps<float>(c->la_f[l], *cur, alt, c->S32[l], c->lu_f[l], n, rf,
+                                   c->d_active, shape_lanes(G), sweep_wsel(G), c->stream, zs, vup,
+                                   cur);
+    }
+    return smooth_sweeps<float>(c->la_f[l], *cur, c->S32[l], c->lu_f[l], n, rf, c->d_active, G,
+                                c->H32[l], c->stream, zs, vup);
+}
+
 // One FP64 defect-correction step with an FP32 correction-scheme V-cycle
 int mixed_cycle(tbl_ctx *c, const Sched &s, double tol)
 {
@@ -417,39 +438,38 @@ int mixed_cycle(tbl_ctx *c, const Sched &s, double tol)
     defect<double, float>(c->la_d[0], c->V[0], c->S[0], c->S32[0], -1.0, c->d_active, c->d_resmax, st);
     hipLaunchKernelGGL(decide2_kernel, dim3((K + 255) / 256), dim3(256), 0, st, K, c->d_resmax,
                        tol, c->d_cycle, c->d_active, c->d_cycles, c->d_lastres, c->d_nactive);
-    // e starts at zero on every level: the staged smoother's first sweep
-    // reads no column it has not relaxed itself (e's boundary columns are
-    // never written and its wall rows only ever receive zeros, all zero since
-    // allocation); on the other shapes and the coarsest level e is cleared
-    // first (level 1 by a fill, coarser ones by the restriction)
-    auto zero_start = [&](int l) { return s.nu1 > 0 && l < L - 1 && c->G32[l] < 0; };
+    // e of each level (the out-of-place sweep moves it between two buffers)
+    std::vector<cx<float> *> e(c->E32);
+    // e starts at zero on every level: the staged and one-launch smoothers'
+    // first sweep reads no column it has not relaxed itself (e's boundary
+    // columns are never written and its wall rows only ever receive zeros,
+    // all zero since allocation); on the other shapes and the coarsest level
+    // e is cleared first (level 1 by a fill, coarser ones by the restriction)
+    auto zero_start = [&](int l) {
+        return s.nu1 > 0 && l < L - 1 && (c->G32[l] < 0 || shape_is_sweep(c->G32[l]));
+    };
     const bool zs = zero_start(0);
-    if (!zs) convert<float, float>(c->lv[0].plane * K, c->E32[0], c->E32[0], 1, st);
+    if (!zs) convert<float, float>(c->lv[0].plane * K, e[0], e[0], 1, st);
     for (int l = 0; l < L - 1; ++l) {
-        TRY(smooth_sweeps<float>(c->la_f[l], c->E32[l], c->S32[l], c->lu_f[l], s.nu1,
-                                 static_cast<float>(rf_of(s.nu1)), c->d_active, c->G32[l],
-                                 c->H32[l], st, zero_start(l)));
-        defect<float, float>(c->la_f[l], c->E32[l], c->S32[l], c->R32[l], -1.0f, c->d_active,
+        TRY(mixed_sweeps(c, l, &e[l], s.nu1, zero_start(l), nullptr));
+        defect<float, float>(c->la_f[l], e[l], c->S32[l], c->R32[l], -1.0f, c->d_active,
                              nullptr, st);
-        restrict_level<float>(c->lv[l], c->lv[l + 1], K, c->E32[l], c->R32[l], c->E32[l + 1],
-                              nullptr, c->S32[l + 1], zero_start(l + 1) ? -1 : 0, c->d_active,
-                              st);
+        restrict_level<float>(c->lv[l], c->lv[l + 1], K, e[l], c->R32[l], e[l + 1], nullptr,
+                              c->S32[l + 1], zero_start(l + 1) ? -1 : 0, c->d_active, st);
     }
     // coarsest correction in FP32 too, from the zero the restriction left
-    TRY(smooth_coarsest<float>(c->la_f[L - 1], c->E32[L - 1], c->S32[L - 1], c->lu_f[L - 1], s.nc,
+    TRY(smooth_coarsest<float>(c->la_f[L - 1], e[L - 1], c->S32[L - 1], c->lu_f[L - 1], s.nc,
                                static_cast<float>(rf_of(s.nc)), c->d_active, c->G32[L - 1], st));
-    // v += e: inside the finest level's last post-sweep on the staged shapes
-    // (each line adds its final e), else by the update kernel
+    // v += e: inside the finest level's last post-sweep on the staged and
+    // one-launch shapes (each line adds its final e), else by the update
+    // kernel
     const bool fused = zs && s.nu2 > 0;
     for (int l = L - 1; l >= 1; --l) {
-        prolong_level<float>(c->lv[l - 1], c->lv[l], K, c->E32[l - 1], c->E32[l], nullptr, 0,
-                             c->d_active, st);
-        TRY(smooth_sweeps<float>(c->la_f[l - 1], c->E32[l - 1], c->S32[l - 1], c->lu_f[l - 1],
-                                 s.nu2, static_cast<float>(rf_of(s.nu2)), c->d_active,
-                                 c->G32[l - 1], c->H32[l - 1], st, false,
-                                 fused && l == 1 ? c->V[0] : nullptr));
+        prolong_level<float>(c->lv[l - 1], c->lv[l], K, e[l - 1], e[l], nullptr, 0, c->d_active,
+                             st);
+        TRY(mixed_sweeps(c, l - 1, &e[l - 1], s.nu2, false, fused && l == 1 ? c->V[0] : nullptr));
     }
-    if (!fused) update(c->lv[0], K, c->V[0], c->E32[0], c->d_active, st);
+    if (!fused) update(c->lv[0], K, c->V[0], e[0], c->d_active, st);
     return TBL_OK;
 }
 
@@ -645,11 +665,16 @@ extern "C" int tbl_set_line_shape(tbl_ctx *c, int prec, int level, int lanes)
         return TBL_OK;
     }
     const bool strip = shape_is_strip(lanes);
+    const bool sweep = shape_is_sweep(lanes);
     const int a = shape_lanes(lanes);
     if (!c || level < 1 || level > c->L || (a != 16 && a != 32 && a != 64))
         return TBL_ERR_ARG;
     const Level &lv = c->lv[level - 1];
     if (line_rows_for(a, lv.ny) < 0) return TBL_ERR_GRID;
+    // one-launch sweep: FP32 correction cycle only, not on the coarsest level
+    if (sweep && (prec == 0 || level == c->L || sweep_wsel(lanes) > 5 ||
+                  sweep_lds(lv.ld, sizeof(cx<float>), a) > 160 * 1024))
+        return TBL_ERR_GRID;
     // strip form: not on the coarsest level, and only for rows per lane that
     // stay in registers with two lines in flight (FP64 <= 3, FP32 <= 5)
     if (strip && (level == c->L || line_rows_for(a, lv.ny) > (prec ? 5 : 3) ||
@@ -679,7 +704,12 @@ extern "C" int tbl_time_smoother(tbl_ctx *c, int prec, int level, int sweeps, do
     else if (level == c->L)
         TRY(smooth_coarsest<double>(c->la_d[l], c->V[l], c->S[l], c->lu_d[l], sweeps, 1.0,
                                     nullptr, c->G64[l], st));
-    else if (prec)
+    else if (prec && shape_is_sweep(c->G32[l])) {
+        cx<float> *cur = nullptr;
+        TRY(sweep_sweeps<float>(c->la_f[l], c->E32[l], c->E32b[l], c->S32[l], c->lu_f[l], sweeps,
+                                1.2f, nullptr, shape_lanes(c->G32[l]), sweep_wsel(c->G32[l]), st,
+                                false, nullptr, &cur));
+    } else if (prec)
         TRY(smooth_sweeps<float>(c->la_f[l], c->E32[l], c->S32[l], c->lu_f[l], sweeps, 1.2f,
                                  nullptr, c->G32[l], c->H32[l], st));
     else
@@ -693,6 +723,7 @@ extern "C" int tbl_time_smoother(tbl_ctx *c, int prec, int level, int sweeps, do
     *ms = t;
     if (launches) {
         if (level == c->L) *launches = 1;
+        else if (shape_is_sweep((prec ? c->G32 : c->G64)[l])) *launches = sweeps;
         else if (shape_is_strip((prec ? c->G32 : c->G64)[l])) *launches = 2 * sweeps;
         else *launches = 4 * sweeps;
     }
