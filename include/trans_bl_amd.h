@@ -174,6 +174,17 @@ int tbl_deriv_dev(tbl_ctx *ctx, int kind, const double *d_fc, double *d_out);
 int tbl_residual_dev(tbl_ctx *ctx, const double *d_wx, const double *d_dwzdx,
                      const double *d_uy, double *d_out, double *residual);
 
+/* Same as tbl_residual_dev, allocating nothing: no reference counterpart;
+   for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc).  Its work
+   fields (three FP64 complex fields of the finest level and one double per
+   mode, each 256-byte aligned) live in a fixed 64 MiB device array of the
+   library, which every GPU the library runs on holds until the process
+   ends; grids that need more are rejected with TBL_ERR_ARG.  All calls on
+   one GPU share that array: calls that overlap on one GPU race. */
+int tbl_residual_dev_noalloc(tbl_ctx *ctx, const double *d_wx, const double *d_dwzdx,
+                             const double *d_uy, double *d_out, double *residual);
+
 /* Line-solve shape of one multigrid level's smoother: `lanes` (16, 32 or
    64) lanes of a wavefront share one y-line; a negative value selects the
    same shape with LDS-staged, coalesced loads and stores; 2000 + lanes

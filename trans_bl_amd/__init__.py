@@ -70,7 +70,7 @@ EXPORTS = [
     "tbl_version", "tbl_last_error", "tbl_coefs_generate", "tbl_coefs_read", "tbl_create", "tbl_destroy",
     "tbl_set_stream", "tbl_synchronize", "tbl_poi_uy", "tbl_poi_uy_dev", "tbl_solve_rhs_dev", "tbl_derparx",
     "tbl_derparxx", "tbl_dery", "tbl_deryy", "tbl_deryfv", "tbl_deriv_dev", "tbl_residual_dev",
-    "tbl_set_line_shape", "tbl_get_line_shape", "tbl_time_smoother",
+    "tbl_residual_dev_noalloc", "tbl_set_line_shape", "tbl_get_line_shape", "tbl_time_smoother",
     "tbl_net_create", "tbl_net_destroy", "tbl_net_num_params", "tbl_net_get_params",
     "tbl_net_set_params", "tbl_net_forward", "tbl_net_gradients", "tbl_net_train_step",
     "tbl_net_synchronize", "tbl_mfma_gemm_bf16", "tbl_guess_attach", "tbl_guess_shape",
@@ -106,6 +106,7 @@ def lib():
                                C.POINTER(SolveStats)], C.c_int),
         "tbl_deriv_dev": ([vp, ip, vp, vp], C.c_int),
         "tbl_residual_dev": ([vp, vp, vp, vp, vp, vp], C.c_int),
+        "tbl_residual_dev_noalloc": ([vp, vp, vp, vp, vp, vp], C.c_int),
         "tbl_set_line_shape": ([vp, ip, ip, ip], C.c_int),
         "tbl_get_line_shape": ([vp, ip, ip], C.c_int),
         "tbl_time_smoother": ([vp, ip, ip, ip, C.POINTER(dp), C.POINTER(ip)], C.c_int),
@@ -292,6 +293,16 @@ class Context:
         _check(lib().tbl_residual_dev(self.h, C.c_void_p(wx_ptr) if wx_ptr else None,
                                       C.c_void_p(dwzdx_ptr), C.c_void_p(uy_ptr),
                                       C.c_void_p(out_ptr), _ptr(res)), "tbl_residual_dev")
+        return res
+
+    def residual_dev_noalloc(self, wx_ptr, dwzdx_ptr, uy_ptr, out_ptr):
+        """residual_dev without device allocation (the library's fixed work
+        array)."""
+        res = np.zeros(self.grid.nmodes, np.float64)
+        _check(lib().tbl_residual_dev_noalloc(self.h, C.c_void_p(wx_ptr) if wx_ptr else None,
+                                              C.c_void_p(dwzdx_ptr), C.c_void_p(uy_ptr),
+                                              C.c_void_p(out_ptr), _ptr(res)),
+               "tbl_residual_dev_noalloc")
         return res
 
     def set_line_shape(self, prec, level, lanes):

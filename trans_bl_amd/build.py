@@ -41,6 +41,7 @@ SOURCES = [
     "tbl_net.cpp",
     "tbl_guess.hip",
     "tbl_batch.cpp",
+    "tbl_noalloc.hip",
 ]
 
 

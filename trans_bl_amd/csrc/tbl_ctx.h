@@ -62,6 +62,7 @@ namespace tbl {
 int guess_apply(tbl_ctx *c, hipStream_t st, bool accept_all, int *fallback,
                 double *guess_res_max);
 int guess_free(tbl_ctx *c);
+int noalloc_setup(int device);
 int solve_dev(tbl_ctx *ctx, const double *wx, const double *dz, double *uy, int direct_rhs,
               double tol, const tbl_solve_opts *opts, int *cycles, double *residual,
               tbl_solve_stats *stats);

@@ -103,6 +103,8 @@ static int create_impl(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx *c)
     else if (!generate_coefs(c->g.stf, c->g.levels, &c->c)) return TBL_ERR_ARG;
     build_metrics(c->g, c->m);
     TBL_HIP(hipGetDevice(&c->device));
+    // the device address of the allocation-free entry point's array
+    TRY(noalloc_setup(c->device));
     c->K = c->g.nmodes;
     c->L = c->g.levels;
     const int K = c->K, L = c->L;
