@@ -166,6 +166,13 @@ int tbl_deryfv(tbl_ctx *ctx, const double *fc, double *ddy);
    kind: 0 derparx, 1 derparxx, 2 dery, 3 deryy, 4 deryfv. */
 int tbl_deriv_dev(tbl_ctx *ctx, int kind, const double *d_fc, double *d_out);
 
+/* The four derivatives of one field in two passes over it (derparx,
+   derparxx, dery, deryy of the same fc, as drv_gv calls them,
+   main_par.f:1364-1377; derivs_par.f:80-129, 164-177).  Device pointers,
+   asynchronous on the context's stream; any output may be NULL. */
+int tbl_derivs_dev(tbl_ctx *ctx, const double *d_fc, double *d_ddx, double *d_d2dx2,
+                   double *d_ddy, double *d_d2dy2);
+
 /* Apply the fine-grid operator: out = S - L v on i=2..nx-1, j=2..ny
    (defect with si=-1, poisson_par.f:543-793) for every mode, with
    S = B(kb*wx - dwzdx) (function_source, poisson_par.f:111-164).  Device

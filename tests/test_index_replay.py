@@ -91,3 +91,56 @@ def _staged_reach(NP, W, MT):
 def test_staged_conv_taps_stay_in_the_lds_stage(W, MT):
     lo, hi, NQ = _staged_reach(1 << 20, W, MT)
     assert lo >= 0 and hi < NQ
+
+
+# ---- derivative kernels (tbl_deriv_kernels.hip) ------------------------------
+def _x_kernel_reach(nx, ny, K):
+    """deriv_x_kernel<R>: every wavefront (one x-line) stages the line in its
+    LDS slice of ldx = nx|1 points and reads clamped windows; global loads
+    and stores stay on its line.  Returns (R, max LDS index, LDS elements,
+    max global index, elements)."""
+    R = next(r for r in (1, 2, 3, 5, 7, 9, 17) if 64 * r >= nx)
+    ldx = nx | 1
+    lines = ny * K
+    blocks = (lines + 3) // 4
+    w = np.arange(4)
+    line = np.arange(blocks)[:, None] * 4 + w[None, :]
+    live = line < lines
+    lane = np.arange(64)
+    gi = (lane[:, None] + 64 * np.arange(R)[None, :])            # loads, i < nx
+    gi = gi[gi < nx]
+    i0 = lane * R
+    q = np.arange(R + 4)
+    win = np.clip(i0[:, None] + q[None, :] - 2, 0, nx - 1)        # LDS window
+    lds_max = 3 * ldx + max(gi.max(), win.max())
+    gmax = (line[live].max() * nx + nx - 1)
+    return R, lds_max, 4 * ldx, gmax, lines * nx
+
+
+def _y_kernel_reach(nx, ny, K):
+    G = 32 if (ny + 31) // 32 <= 9 else 64
+    R = next(r for r in (1, 2, 3, 4, 5, 7, 9) if G * r >= ny)
+    TX = 256 // G
+    ldy = ny | 1
+    e = np.arange(TX * ny)
+    x, j = e % TX, e // TX
+    tile_max = (x * ldy + j).max()
+    lane = np.arange(G)
+    win = np.clip(lane[:, None] * R + np.arange(R + 4)[None, :] - 2, 0, ny - 1)
+    col_max = (TX - 1) * ldy + win.max()
+    # global: x0 + x < nx masked, rows j < ny, modes k < K
+    gmax = (K - 1) * nx * ny + (ny - 1) * nx + (nx - 1)
+    return G, R, max(tile_max, col_max), TX * ldy, gmax, K * nx * ny
+
+
+@pytest.mark.parametrize("nx, ny, K", [(129, 33, 4), (385, 97, 8), (1025, 193, 64),
+                                       (1088, 33, 2), (65, 545, 2), (257, 576, 2)])
+def test_derivative_kernels_stay_in_bounds(nx, ny, K):
+    if (nx + 63) // 64 <= 17:
+        R, lds_max, lds_n, gmax, n = _x_kernel_reach(nx, ny, K)
+        assert lds_max < lds_n and gmax < n and 64 * R >= nx
+        assert 4 * (nx | 1) * 16 <= 160 * 1024
+    if ny <= 64 * 9:
+        G, R, lds_max, lds_n, gmax, n = _y_kernel_reach(nx, ny, K)
+        assert lds_max < lds_n and gmax < n and G * R >= ny
+        assert lds_n * 16 <= 160 * 1024

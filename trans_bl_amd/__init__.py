@@ -69,7 +69,7 @@ class SolveStats(C.Structure):
 EXPORTS = [
     "tbl_version", "tbl_last_error", "tbl_coefs_generate", "tbl_coefs_read", "tbl_create", "tbl_destroy",
     "tbl_set_stream", "tbl_synchronize", "tbl_poi_uy", "tbl_poi_uy_dev", "tbl_solve_rhs_dev", "tbl_derparx",
-    "tbl_derparxx", "tbl_dery", "tbl_deryy", "tbl_deryfv", "tbl_deriv_dev", "tbl_residual_dev",
+    "tbl_derparxx", "tbl_dery", "tbl_deryy", "tbl_deryfv", "tbl_deriv_dev", "tbl_derivs_dev", "tbl_residual_dev",
     "tbl_residual_dev_noalloc", "tbl_set_line_shape", "tbl_get_line_shape", "tbl_time_smoother",
     "tbl_net_create", "tbl_net_destroy", "tbl_net_num_params", "tbl_net_get_params",
     "tbl_net_set_params", "tbl_net_forward", "tbl_net_gradients", "tbl_net_train_step",
@@ -105,6 +105,7 @@ def lib():
         "tbl_solve_rhs_dev": ([vp, vp, vp, dp, C.POINTER(SolveOpts), vp, vp,
                                C.POINTER(SolveStats)], C.c_int),
         "tbl_deriv_dev": ([vp, ip, vp, vp], C.c_int),
+        "tbl_derivs_dev": ([vp, vp, vp, vp, vp, vp], C.c_int),
         "tbl_residual_dev": ([vp, vp, vp, vp, vp, vp], C.c_int),
         "tbl_residual_dev_noalloc": ([vp, vp, vp, vp, vp, vp], C.c_int),
         "tbl_set_line_shape": ([vp, ip, ip, ip], C.c_int),
@@ -287,6 +288,12 @@ class Context:
     def deriv_dev(self, kind, in_ptr, out_ptr):
         _check(lib().tbl_deriv_dev(self.h, kind, C.c_void_p(in_ptr), C.c_void_p(out_ptr)),
                "tbl_deriv_dev")
+
+    def derivs_dev(self, in_ptr, dx_ptr=0, dxx_ptr=0, dy_ptr=0, dyy_ptr=0):
+        """d/dx, d2/dx2, d/dy, d2/dy2 of one field (device pointers; 0 skips
+        an output)."""
+        p = [C.c_void_p(x) if x else None for x in (dx_ptr, dxx_ptr, dy_ptr, dyy_ptr)]
+        _check(lib().tbl_derivs_dev(self.h, C.c_void_p(in_ptr), *p), "tbl_derivs_dev")
 
     def residual_dev(self, wx_ptr, dwzdx_ptr, uy_ptr, out_ptr):
         res = np.zeros(self.grid.nmodes, np.float64)

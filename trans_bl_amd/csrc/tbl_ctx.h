@@ -41,7 +41,7 @@ struct tbl_ctx {
     int *h_nactive = nullptr;     // pinned
 
     // derivatives
-    double *d_qdy = nullptr, *d_thomas = nullptr;
+    double *d_qdy = nullptr, *d_thomas = nullptr, *d_tseg = nullptr;
     tbl::DerivArgs dargs{};
 
     // staging for the host-pointer entry points (user layout)

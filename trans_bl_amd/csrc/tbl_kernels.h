@@ -154,11 +154,24 @@ struct DerivArgs {
     double fp_sd[8], sp_sd[10], cp_sd[8], pp_sd[10], lp_sd[8];
     const double *qdy;     // v_qdy
     const double *qdy2;    // v_qdy2(:,1)
-    // Thomas factors per scheme: a, bet, gam (kind-major, stride nmax)
-    const double *thomas;  // [kind][3][nmax]
+    // Thomas factors per scheme: a, bet, gam, 1/bet (kind-major, stride nmax)
+    const double *thomas;  // [kind][kThomasFields][nmax]
     int nmax;
+    // the same factors in lane-segment order for the LDS-staged kernels:
+    // x kinds 0, 1 ([kind][a, 1/bet, next gam][xR][64]), then y kinds 2, 3, 4
+    // ([kind][3][yR][yG]); xR / yR = 0: that direction takes the long-line
+    // kernel
+    const double *tseg;
+    int xR, yG, yR;
 };
+constexpr int kThomasFields = 4;
+void deriv_shapes(int nx, int ny, int *xR, int *yG, int *yR);
 
 int deriv(const DerivArgs &A, int kind, const double *f, double *out, hipStream_t st);
+// d/dx, d2/dx2, d/dy, d2/dy2 of one field (any output may be null): two
+// launches, each reading f once
+int derivs4(const DerivArgs &A, const double *f, double *dx, double *dxx, double *dy, double *dyy,
+            hipStream_t st);
+int deriv_kernel_attributes();
 
 }  // namespace tbl

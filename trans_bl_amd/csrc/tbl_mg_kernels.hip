@@ -467,6 +467,7 @@ int set_kernel_attributes()
     if (rc == TBL_OK) rc = strip_kernel_attributes();
     if (rc == TBL_OK) rc = coarse_kernel_attributes();
     if (rc == TBL_OK) rc = sweep_kernel_attributes();
+    if (rc == TBL_OK) rc = deriv_kernel_attributes();
     return rc;
 }
 

@@ -197,17 +197,41 @@ static int create_impl(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx *c)
 
     // derivatives: metrics and Thomas factors of the five schemes
     const int nmax = std::max(c->g.nx, c->g.ny);
-    std::vector<double> th(static_cast<size_t>(5) * 3 * nmax, 0.0);
+    std::vector<double> th(static_cast<size_t>(5) * kThomasFields * nmax, 0.0);
     for (int kind = 0; kind < 5; ++kind) {
         std::vector<double> a, b, cc, gam, bet;
         deriv_lhs(c->g, c->c, kind, a, b, cc);
         thomas_factors(a, b, cc, gam, bet);
-        double *t = &th[static_cast<size_t>(kind) * 3 * nmax];
+        double *t = &th[static_cast<size_t>(kind) * kThomasFields * nmax];
         for (size_t j = 0; j < a.size(); ++j) {
             t[j] = a[j];
             t[nmax + j] = bet[j];
             t[2 * nmax + j] = gam[j];
+            t[3 * nmax + j] = 1.0 / bet[j];
         }
+    }
+    // lane-segment copies for the LDS-staged kernels
+    int xR = 0, yG = 0, yR = 0;
+    deriv_shapes(c->g.nx, c->g.ny, &xR, &yG, &yR);
+    std::vector<double> ts;
+    for (int kind = 0; kind < 5; ++kind) {
+        const bool x = kind <= 1;
+        const int G = x ? 64 : yG, R = x ? xR : yR, n = x ? c->g.nx : c->g.ny;
+        const double *t = &th[static_cast<size_t>(kind) * kThomasFields * nmax];
+        const size_t base = ts.size(), per = static_cast<size_t>(G) * R;
+        ts.resize(base + 3 * per);
+        for (int r = 0; r < R; ++r)
+            for (int lane = 0; lane < G; ++lane) {
+                const int i = lane * R + r;
+                const size_t e = base + static_cast<size_t>(r) * G + lane;
+                ts[e] = i < n ? t[i] : 0.0;                            // a
+                ts[e + per] = i < n ? t[3 * nmax + i] : 1.0;           // 1/bet
+                ts[e + 2 * per] = i + 1 < n ? t[2 * nmax + i + 1] : 0.0;   // gam of row i+1
+            }
+    }
+    if (!ts.empty()) {
+        TRY(dalloc(&c->d_tseg, ts.size()));
+        TBL_HIP(hipMemcpy(c->d_tseg, ts.data(), sizeof(double) * ts.size(), hipMemcpyHostToDevice));
     }
     TRY(dalloc(&c->d_thomas, th.size()));
     TBL_HIP(hipMemcpy(c->d_thomas, th.data(), sizeof(double) * th.size(), hipMemcpyHostToDevice));
@@ -233,6 +257,10 @@ static int create_impl(const tbl_grid *grid, const tbl_coefs *coefs, tbl_ctx *c)
     A.qdy2 = c->qdy2_d[0];
     A.thomas = c->d_thomas;
     A.nmax = nmax;
+    A.tseg = c->d_tseg;
+    A.xR = xR;
+    A.yG = yG;
+    A.yR = yR;
     TBL_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamDefault));
     c->stream = c->own_stream;
     TBL_HIP(hipEventCreate(&c->ev0));
@@ -272,7 +300,7 @@ extern "C" int tbl_destroy(tbl_ctx *c)
     for (auto *p : c->H64) fr(p);
     for (auto *p : c->H32) fr(p);
     fr(c->d_active); fr(c->d_cycles); fr(c->d_nactive); fr(c->d_cycle);
-    fr(c->d_resmax); fr(c->d_lastres); fr(c->d_qdy); fr(c->d_thomas);
+    fr(c->d_resmax); fr(c->d_lastres); fr(c->d_qdy); fr(c->d_thomas); fr(c->d_tseg);
     fr(c->d_u0); fr(c->d_u1); fr(c->d_u2);
     if (c->h_nactive) (void)hipHostFree(c->h_nactive);
     for (auto &g : c->graph)
@@ -633,6 +661,17 @@ static int deriv_host(tbl_ctx *c, int kind, const double *fc, double *out)
     return TBL_OK;
 }
 
+extern "C" int tbl_derivs_dev(tbl_ctx *c, const double *d_fc, double *d_ddx, double *d_d2dx2,
+                              double *d_ddy, double *d_d2dy2)
+{
+    if (!c || !d_fc) return TBL_ERR_ARG;
+    for (const double *o : {d_ddx, d_d2dx2, d_ddy, d_d2dy2})
+        if (o == d_fc) return TBL_ERR_ARG;
+    TRY(derivs4(c->dargs, d_fc, d_ddx, d_d2dx2, d_ddy, d_d2dy2, c->stream));
+    TBL_HIP(hipGetLastError());
+    return TBL_OK;
+}
+
 extern "C" int tbl_derparx(tbl_ctx *c, const double *fc, double *o) { return deriv_host(c, 0, fc, o); }
 extern "C" int tbl_derparxx(tbl_ctx *c, const double *fc, double *o) { return deriv_host(c, 1, fc, o); }
 extern "C" int tbl_dery(tbl_ctx *c, const double *fc, double *o) { return deriv_host(c, 2, fc, o); }
@@ -745,7 +784,7 @@ int stage_issue(tbl_ctx *c, const double *f, double *v, double *const *d, double
     const Sched s = sched_of(opts, c->lv[c->L - 1].ny);
     const int prec = opts ? opts->precision : 0;
     const int K = c->K;
-    for (int q = 0; q < 4; ++q) TRY(deriv(c->dargs, q, v, d[q], st));
+    TRY(derivs4(c->dargs, v, d[0], d[1], d[2], d[3], st));
     gather_source(c->la_d[0], nullptr, f, v, c->g.beta, c->S[0], c->V[0], 1, st);
     fill_int(c->d_active, K, 1, st);
     fill_int(c->d_cycles, K, 0, st);
