@@ -9,15 +9,20 @@ max|S - L v| < 1e-9 (source assembly, cycles and copy-out included), with
 inputs resident in HBM.
 
 Paths: plain FP64 FAS multigrid (the reference's cycle with the GPU's colour
-line ordering), and the flagship path -- FP32 line smoothing under FP64
-defect correction, started from the learned initial guess when a trained
-network is present (trans_bl_amd/guess_*.bin), else from zero.  `value` is
-the flagship path's whole-job mode-solves/s; the plain path is reported
-beside it.
+line ordering), and FP32 line smoothing under FP64 defect correction started
+from zero, from the learned initial guess (when a trained network is
+present, trans_bl_amd/weights/guess_*.bin) and from the previous stage's
+solution (packets 0.4 upstream: the reference's warm start, reported beside).
+`value` is the whole-job mode-solves/s of the mixed path from zero, or from
+the learned guess when that is faster by more than 1 %; the plain FP64 path
+is reported beside it.
 
 Prints ONE JSON line.  Also reports the smoother's HBM roofline (live hipEvent
-timing of the level-1 smoother kernel) and a CPU baseline (the CPU oracle's
-reference-ordered FP64 solve on a bounded sample of modes, on the host cores).
+timing of the level-1 smoother kernel), the four derivatives' roofline, and a
+CPU baseline: the reference's own Fortran poi_uy (compiled from its sources
+into oracle/_ref) on a bounded sample of the workload, one process per host
+core, with the CPU oracle timed on the same sample for calibration (the
+oracle alone when the reference driver is absent).
 """
 from __future__ import annotations
 
@@ -89,7 +94,6 @@ def parse():
 
 def cpu_worker(args):
     import oracle
-    from trans_bl_amd import workload as wl
     g, f = args
     p = oracle.make_params(g.nx, g.ny, g.nmodes, g.levels, g.dx, g.dy0, g.stf, g.alpha, g.beta)
     z = np.zeros_like(f)
@@ -98,7 +102,39 @@ def cpu_worker(args):
     return time.perf_counter() - t, cyc.tolist(), res.tolist()
 
 
-def cpu_baseline(grid, f, nmodes, workers):
+def cpu_baseline(grid, f, ncpu):
+    """The reference's poi_uy on modes k=0..7 of the workload (its driver
+    built for the same grid with 8 modes), `ncpu` processes at once; the CPU
+    oracle (port, reference line order, FP64) on the same 8 modes in one
+    process for the calibration ratio.  None when the driver is absent."""
+    import dataclasses
+    from oracle import reference as R
+    name = "1025x193x8"
+    if not R.available(name):
+        return None
+    f8 = np.ascontiguousarray(f[:8])
+    z = np.zeros_like(f8)
+    try:
+        wall, secs, cyc = R.run_parallel(name, z, -f8, z, 1e-9, ncpu)
+    except Exception as e:  # noqa: BLE001 - report, fall back to the port
+        print(f"reference driver failed: {e}", file=sys.stderr)
+        return None
+    g8 = dataclasses.replace(grid, nmodes=8)
+    t_port, cyc_port, _ = cpu_worker((g8, f8))
+    ref_one = float(np.median(secs))
+    return {"value": round(8 * ncpu / wall, 4), "unit": "mode-solves/s", "cores": ncpu,
+            "kind": "reference",
+            "sample": f"modes k=0..7 of the same workload (1025x193, kz=0.1k) from zero to "
+                      f"1e-9 by the reference's own poi_uy (Fortran, compiled from its sources, "
+                      f"one MPI rank), {ncpu} processes at once on {ncpu} cores; {wall:.1f} s",
+            "cycles": cyc.tolist(),
+            "calibration": {"reference_s_per_8_modes": round(ref_one, 3),
+                            "oracle_s_per_8_modes": round(t_port, 3),
+                            "oracle_over_reference_speed": round(ref_one / t_port, 3),
+                            "oracle_cycles": cyc_port}}
+
+
+def cpu_baseline_port(grid, f, nmodes, workers):
     """CPU oracle (reference ordering, FP64) on `nmodes` modes, one process
     per mode, `workers` at a time."""
     import dataclasses
@@ -132,14 +168,16 @@ def main():
     cpu = None
     if not a.no_cpu:
         ncpu = min(len(os.sched_getaffinity(0)), 16)
-        nm = a.cpu_modes or ncpu
-        wall_c, out = cpu_baseline(grid, f, nm, ncpu)
-        cpu = {"value": round(nm / wall_c, 4), "unit": "mode-solves/s", "cores": ncpu,
-               "kind": "port",
-               "sample": f"{nm} of the 64 modes (k=0..{nm - 1}) of the same workload, CPU "
-                         f"oracle FP64 with the reference's lexicographic line order, one "
-                         f"process per mode, {ncpu} at a time; {wall_c:.1f} s",
-               "cycles": [o[1][-1] for o in out]}
+        cpu = cpu_baseline(grid, f, ncpu)
+        if cpu is None:
+            nm = a.cpu_modes or ncpu
+            wall_c, out = cpu_baseline_port(grid, f, nm, ncpu)
+            cpu = {"value": round(nm / wall_c, 4), "unit": "mode-solves/s", "cores": ncpu,
+                   "kind": "port",
+                   "sample": f"{nm} of the 64 modes (k=0..{nm - 1}) of the same workload, CPU "
+                             f"oracle FP64 with the reference's lexicographic line order, one "
+                             f"process per mode, {ncpu} at a time; {wall_c:.1f} s",
+                   "cycles": [o[1][-1] for o in out]}
 
     import torch
     import trans_bl_amd as tb
@@ -164,7 +202,14 @@ def main():
             print(f"learned guess unavailable: {e}", file=sys.stderr)
             guess = None
 
-    def run(prec, learned, steps, warm):
+    # the previous stage's solution: the same packets 0.4 upstream
+    fprev = torch.from_numpy(wl.wavepacket_rhs(grid, tag="bench", shift=-0.4)).to("cuda")
+    uprev = torch.zeros_like(fd)
+    torch.cuda.synchronize()
+    ctx.solve_rhs_dev(fprev.data_ptr(), uprev.data_ptr(), a.tol, precision=1, max_cycles=40,
+                      use_graph=1, guess=1)
+
+    def run(prec, learned, steps, warm, start="zero"):
         # no garbage-collector pass inside a timed loop
         gc.collect()
         gc.disable()
@@ -173,11 +218,17 @@ def main():
             if it == warm:
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
-            ud.zero_()
             kw = dict(precision=prec, max_cycles=40, use_graph=1)
             if learned:
+                ud.zero_()
                 cyc, res, st = guess.solve(ctx, fd, ud, a.tol, **kw)
+            elif start == "warm":
+                # warm start: v = the previous stage's solution (guess 0)
+                ud.copy_(uprev)
+                cyc, res, st = ctx.solve_rhs_dev(fd.data_ptr(), ud.data_ptr(), a.tol, guess=0,
+                                                 **kw)
             else:
+                ud.zero_()
                 cyc, res, st = ctx.solve_rhs_dev(fd.data_ptr(), ud.data_ptr(), a.tol, guess=1, **kw)
             if it >= warm:
                 stats.append((cyc.copy(), res.copy(), st))
@@ -188,13 +239,17 @@ def main():
 
     wall64, st64 = run(0, False, a.steps, a.warmup)
     v64 = ud.clone()
-    flag_learned = guess is not None
-    wallm, stm = run(1, flag_learned, a.steps, a.warmup)
-    vm = ud.clone()
-    mixed_zero = None
-    if flag_learned:
-        wallz, stz = run(1, False, a.steps, a.warmup)
-        mixed_zero = (wallz, stz)
+    wallz, stz = run(1, False, a.steps, a.warmup)
+    vz = ud.clone()
+    learned = None
+    if guess is not None:
+        walll, stl = run(1, True, a.steps, a.warmup)
+        learned = (walll, stl, ud.clone())
+    wallw, stw = run(1, False, a.steps, a.warmup, start="warm")
+    # value: the mixed path from zero, or from the learned guess when that is
+    # faster by more than 1 %
+    flag_learned = learned is not None and learned[0] < wallz / 1.01
+    wallm, stm, vm = (learned if flag_learned else (wallz, stz, vz))
 
     # residual-tested form of the learned start (guess=2): fallback rate
     fallback = None
@@ -219,10 +274,41 @@ def main():
     ratem = K * a.steps / wallm
     diff = float(torch.linalg.vector_norm(vm - v64) / torch.linalg.vector_norm(v64))
 
+    def path(wall, stats):
+        return dict(summar(stats), value=round(K * a.steps / wall, 2),
+                    ms_per_step=round(wall / a.steps * 1e3, 3))
+
+    # ---- the four derivatives of one field (two launches) on the same grid,
+    # timed with hipEvents on the stream the library launches them on
+    dstream = torch.cuda.Stream()
+    outs = [torch.empty_like(fd) for _ in range(4)]
+    torch.cuda.synchronize()
+    with torch.cuda.stream(dstream):
+        ctx.set_stream(dstream.cuda_stream)
+        ctx.derivs_dev(vz.data_ptr(), *[o.data_ptr() for o in outs])
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(dstream)
+        nrep = 10
+        for _ in range(nrep):
+            ctx.derivs_dev(vz.data_ptr(), *[o.data_ptr() for o in outs])
+        e1.record(dstream)
+    torch.cuda.synchronize()
+    ctx.set_stream(None)
+    d_us = e0.elapsed_time(e1) / nrep * 1e3
+    n1 = grid.nx * grid.ny * K
+    # algorithmic: each direction reads v once and writes two derivatives
+    d_bytes = 6 * 16 * n1
+    roof_d = {"bound": "hbm", "achieved": round(d_bytes / d_us / 1e3, 1), "peak": HBM_PEAK_GBS,
+              "unit": "GB/s", "frac": round(d_bytes / d_us / 1e3 / HBM_PEAK_GBS, 4),
+              "traffic": None,
+              "kernel": "deriv_x_kernel<17> + deriv_y_kernel<32, 7> (d/dx, d2/dx2, d/dy, d2/dy2 "
+                        "of one field)",
+              "us_per_call": round(d_us, 1), "bytes_per_call": d_bytes}
+    del outs
+
     # ---- roofline of the dominant kernel: level-1 line smoother ----------
     # algorithmic bytes per sweep: read v, read S, write v over every point of
     # the level (3 passes), complex FP64 = 16 B, FP32 = 8 B
-    n1 = grid.nx * grid.ny * K
     roof = {}
     for prec, esize in ((1, 8), (0, 16)):
         name = smoother_kernel(prec, ctx.line_shape(prec, 1), grid.ny)
@@ -262,14 +348,16 @@ def main():
                            ms_per_step=round(wall64 / a.steps * 1e3, 3)),
         "speedup_vs_plain_fp64": round(ratem / rate64, 3),
         "learned_guess_residual_test": fallback,
-        "mixed_zero_start": None if mixed_zero is None else dict(
-            summar(mixed_zero[1]), value=round(K * a.steps / mixed_zero[0], 2),
-            ms_per_step=round(mixed_zero[0] / a.steps * 1e3, 3)),
+        "mixed_zero_start": path(wallz, stz),
+        "mixed_learned_start": None if learned is None else path(learned[0], learned[1]),
+        "mixed_warm_start": dict(path(wallw, stw),
+                                 start="previous stage's solution (packets 0.4 upstream)"),
         "flagship_vs_fp64_rel_l2": diff,
         "line_shapes": {k: {"fp64": v.get("fp64"), "fp32": v.get("fp32")}
                         for k, v in sorted(tuned.items())},
         "roofline": roof[1],
         "roofline_fp64_smoother": roof[0],
+        "roofline_derivs": roof_d,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -44,6 +44,8 @@ CONFIGS = {
     "129x33x4": (129, 33, 4, 4, 0.03, 1e-3, 1.03, 0.2, 0.6),
     "385x97x8": (385, 97, 8, 4, 0.02, 5e-4, 1.02, 0.0, 0.2),
     "1025x193x64": (1025, 193, 64, 4, 0.02, 2e-4, 1.015, 0.0, 0.1),
+    # the benchmark grid with its first 8 modes: the CPU baseline's sample
+    "1025x193x8": (1025, 193, 8, 4, 0.02, 2e-4, 1.015, 0.0, 0.1),
 }
 
 
@@ -222,6 +224,48 @@ def run(name: str, wx, dwzdx, uy, erromax: float, calls: int = 1, timeout: float
         return {"uy": fields[0], "x": fields[1], "xx": fields[2], "y": fields[3],
                 "yy": fields[4], "yfv": fields[5], "cycles": cycles, "residual": res,
                 "seconds": seconds, "wall": wall}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def run_parallel(name: str, wx, dwzdx, uy, erromax: float, copies: int,
+                 timeout: float = 600.0):
+    """`copies` processes of the reference driver at once on the same inputs
+    (one core each): returns the wall time of the batch, each process's
+    poi_uy seconds and the per-mode cycle counts of the first."""
+    if not available(name):
+        raise RuntimeError(f"reference driver {name} is not built")
+    nx, ny, nk = CONFIGS[name][0], CONFIGS[name][1], CONFIGS[name][2]
+    shape = (nk, ny, nx)
+    tmp = tempfile.mkdtemp(prefix="tbl_refpar_")
+    try:
+        fin = os.path.join(tmp, "in.bin")
+        with open(fin, "wb") as f:
+            for a in (wx, dwzdx, uy):
+                a = np.ascontiguousarray(a, dtype=np.complex128)
+                assert a.shape == shape
+                f.write(a.tobytes())
+        procs = []
+        t0 = time.perf_counter()
+        for c in range(copies):
+            fout = os.path.join(tmp, f"out{c}.bin")
+            procs.append(subprocess.Popen(
+                ["bash", "-c", 'ulimit -s unlimited && exec "$@"', "ref", exe_path(name),
+                 coefs_path(name), fin, fout, repr(float(erromax)), "1"],
+                stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, cwd=tmp))
+        outs = [p.communicate(timeout=timeout) for p in procs]
+        wall = time.perf_counter() - t0
+        secs, cycles = [], np.zeros(nk, np.int32)
+        for c, (so, se) in enumerate(outs):
+            if procs[c].returncode != 0:
+                raise RuntimeError(f"reference run failed:\n{so[-1000:]}\n{se[-1000:]}")
+            for line in so.splitlines():
+                t = line.split()
+                if line.strip().startswith("poi_uy seconds"):
+                    secs.append(float(t[-1]))
+                elif c == 0 and len(t) == 3 and t[0].isdigit():
+                    cycles[int(t[0]) - 1] = int(t[1])
+        return wall, secs, cycles
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 

@@ -45,7 +45,7 @@ def params(name):
                                  g.beta, coefs=ref_coefs(name))
 
 
-@pytest.mark.parametrize("name", list(R.CONFIGS))
+@pytest.mark.parametrize("name", ["129x33x4", "385x97x8", "1025x193x64"])
 def test_generators_match_reference_generator(name):
     """The oracle's and the library's coefficient generators at the
     workloads' stretches (1.03, 1.02, 1.015) against the reference's
