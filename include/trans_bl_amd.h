@@ -18,6 +18,8 @@
  *   tbl_dery(_dev)       dery(ddy, fc)                derivs_par.f:116-129
  *   tbl_deryy(_dev)      deryy(d2dy2, fc)             derivs_par.f:164-177
  *   tbl_deryfv(_dev)     deryfv(ddy, fc)              derivs_par.f:148-161
+ *   tbl_checkpoint_*     escreve (data_RR.bin)        escreve_par.f:6-33
+ *   tbl_baseflow_*       escreve (based_RR.bin)       baseflow2D/main_par_2D.f:578-595
  */
 #ifndef TRANS_BL_AMD_H
 #define TRANS_BL_AMD_H
@@ -118,6 +120,23 @@ int tbl_coefs_generate(double stf, int levels, tbl_coefs *out);
    reads).  Rejects a file whose first record is the older 7-entry implicit
    first-point scheme instead of the 6-entry explicit one. */
 int tbl_coefs_read(const char *path, int levels, tbl_coefs *out);
+
+/* The reference's field files (Fortran sequential unformatted records, a
+   4-byte length before and after each record; records above 2^31 - 1 bytes
+   are refused).  Host arrays in the user layout.
+   Checkpoint data_RR.bin (escreve, escreve_par.f:6-33; read back by
+   main_par.f:763-767): record 1 the time step t, record 2 nfields complex
+   fields (nx, ny, nmodes): ux, uy, uz, wx, wy, wz (nfields 6) and th
+   (nfields 7, my_form 2). */
+int tbl_checkpoint_write(const char *path, int nx, int ny, int nmodes, int t, int nfields,
+                         const double *const *fields);
+int tbl_checkpoint_read(const char *path, int nx, int ny, int nmodes, int *t, int nfields,
+                        double *const *fields);
+/* Base flow based_RR.bin (baseflow2D/main_par_2D.f:578-595; read by
+   main_par.f:847-850): one record of ux, uy, wz, real (nx, ny) each. */
+int tbl_baseflow_write(const char *path, int nx, int ny, const double *ux, const double *uy,
+                       const double *wz);
+int tbl_baseflow_read(const char *path, int nx, int ny, double *ux, double *uy, double *wz);
 
 /* Create a context on the current HIP device: checks the grid like
    verifica (main_par.f:60-134), builds the multigrid metrics (create_ctes),

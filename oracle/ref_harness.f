@@ -12,11 +12,15 @@ c     tables in the record order main_par.f:852-873 reads, create_ctes,
 c     derivs_k; then poi_uy on wx, dwzdx and the initial guess uy read
 c     from a file, and the five derivatives of the result.
 c
-c     usage: ref_harness COEFS IN OUT ERROMAX CALLS
+c     usage: ref_harness COEFS IN OUT ERROMAX CALLS [T]
 c       IN   stream file: wx, dwzdx, uy (complex*16 (ptsx,jmax,kfour))
 c       OUT  stream file: uy after the calls, then derparx, derparxx,
 c            dery, deryy, deryfv of it
 c       CALLS  poi_uy calls in a row (each warm-starts from the last)
+c       T    when given: after the calls, ux = wy = dwzdx and uz = wz = wx
+c            are set and the reference's checkpoint writer escreve(T)
+c            (escreve_par.f:6-33) writes data_00.bin in the working
+c            directory
 c     stdout: poi_uy's own line per mode (k, V-cycles, residual) and
 c     the wall time of the poi_uy calls (mpi_wtime).
 cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
@@ -28,7 +32,7 @@ cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
       include 'comm.var'
       include 'comm.coef'
       include 'mpif.h'
-      integer p, calls, c
+      integer p, calls, c, tstep
       real*8 erromax, t0, t1
       complex*16 dwzdx(ptsx,jmax,kfour), d(ptsx,jmax,kfour)
       character*1024 fcoef, fin, fout, arg
@@ -80,6 +84,16 @@ cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
         call poi_uy(dwzdx, erromax)
       end do
       t1 = mpi_wtime()
+
+      if (command_argument_count() .ge. 6) then
+        call get_command_argument(6, arg)
+        read(arg, *) tstep
+        ux = dwzdx
+        wy = dwzdx
+        uz = wx
+        wz = wx
+        call escreve(tstep)
+      end if
 
       open(3, file=trim(fout), access='stream', form='unformatted',
      &     status='replace')

@@ -179,12 +179,15 @@ def read_coefs(name: str) -> dict:
     return out
 
 
-def run(name: str, wx, dwzdx, uy, erromax: float, calls: int = 1, timeout: float = 600.0):
+def run(name: str, wx, dwzdx, uy, erromax: float, calls: int = 1, timeout: float = 600.0,
+        checkpoint_t: int | None = None):
     """poi_uy (calls in a row) and the five derivatives of the result, by the
     reference's code.  Fields are complex (nk, ny, nx) arrays (C view of the
     Fortran (ptsx, jmax, kfour)).  Returns a dict with uy, the derivatives
     (x, xx, y, yy, yfv), per-mode cycles and residuals of the last call, and
-    the wall time of the calls."""
+    the wall time of the calls.  With checkpoint_t the reference's own
+    checkpoint writer runs after the calls and the file's bytes are returned
+    as "data_bin"."""
     if not available(name):
         raise RuntimeError(f"reference driver {name} is not built")
     nx, ny, nk = CONFIGS[name][0], CONFIGS[name][1], CONFIGS[name][2]
@@ -201,10 +204,11 @@ def run(name: str, wx, dwzdx, uy, erromax: float, calls: int = 1, timeout: float
         env = dict(os.environ)
         t0 = time.perf_counter()
         # poi_uy's level planes are local arrays: give the stack room
+        extra = [] if checkpoint_t is None else [str(int(checkpoint_t))]
         r = subprocess.run(["bash", "-c", 'ulimit -s unlimited && exec "$@"', "ref",
                             exe_path(name), coefs_path(name), fin, fout, repr(float(erromax)),
-                            str(calls)], capture_output=True, text=True, timeout=timeout,
-                           env=env, cwd=tmp)
+                            str(calls), *extra], capture_output=True, text=True,
+                           timeout=timeout, env=env, cwd=tmp)
         wall = time.perf_counter() - t0
         if r.returncode != 0:
             raise RuntimeError(f"reference run failed:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
@@ -221,9 +225,13 @@ def run(name: str, wx, dwzdx, uy, erromax: float, calls: int = 1, timeout: float
                 res[k - 1] = float(t[2].replace("D", "E"))
             elif line.strip().startswith("poi_uy seconds"):
                 seconds = float(t[-1])
-        return {"uy": fields[0], "x": fields[1], "xx": fields[2], "y": fields[3],
-                "yy": fields[4], "yfv": fields[5], "cycles": cycles, "residual": res,
-                "seconds": seconds, "wall": wall}
+        out = {"uy": fields[0], "x": fields[1], "xx": fields[2], "y": fields[3],
+               "yy": fields[4], "yfv": fields[5], "cycles": cycles, "residual": res,
+               "seconds": seconds, "wall": wall}
+        if checkpoint_t is not None:
+            with open(os.path.join(tmp, "data_00.bin"), "rb") as f:
+                out["data_bin"] = f.read()
+        return out
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 

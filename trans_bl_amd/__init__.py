@@ -75,7 +75,8 @@ EXPORTS = [
     "tbl_net_set_params", "tbl_net_forward", "tbl_net_gradients", "tbl_net_train_step",
     "tbl_net_synchronize", "tbl_mfma_gemm_bf16", "tbl_guess_attach", "tbl_guess_shape",
     "tbl_guess_pairs", "tbl_batch_create", "tbl_batch_destroy", "tbl_batch_context",
-    "tbl_batch_stage", "tbl_batch_results",
+    "tbl_batch_stage", "tbl_batch_results", "tbl_checkpoint_write", "tbl_checkpoint_read",
+    "tbl_baseflow_write", "tbl_baseflow_read",
 ]
 
 _lib = None
@@ -143,6 +144,54 @@ def coefs_read(path: str, levels: int) -> dict:
     c = CCoefs()
     _check(lib().tbl_coefs_read(path.encode(), levels, C.byref(c)), "tbl_coefs_read")
     return coefs_to_dict(c, levels)
+
+
+def checkpoint_write(path: str, t: int, fields) -> None:
+    """Write the reference's checkpoint file data_RR.bin (escreve,
+    escreve_par.f:6-33): t and 6 (or 7) complex fields of shape (nmodes, ny, nx)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in fields]
+    nk, ny, nx = arrs[0].shape
+    ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    L = lib()
+    L.tbl_checkpoint_write.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.c_void_p]
+    _check(L.tbl_checkpoint_write(path.encode(), nx, ny, nk, t, len(arrs), ptrs),
+           "tbl_checkpoint_write")
+
+
+def checkpoint_read(path: str, nx: int, ny: int, nmodes: int, nfields: int = 6):
+    """Read data_RR.bin: (t, list of nfields complex (nmodes, ny, nx) arrays)."""
+    arrs = [np.zeros((nmodes, ny, nx), np.complex128) for _ in range(nfields)]
+    ptrs = (C.c_void_p * nfields)(*[a.ctypes.data for a in arrs])
+    t = C.c_int()
+    L = lib()
+    L.tbl_checkpoint_read.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(C.c_int), C.c_int, C.c_void_p]
+    _check(L.tbl_checkpoint_read(path.encode(), nx, ny, nmodes, C.byref(t), nfields, ptrs),
+           "tbl_checkpoint_read")
+    return t.value, arrs
+
+
+def baseflow_write(path: str, ux, uy, wz) -> None:
+    """Write based_RR.bin (ux, uy, wz real (ny, nx))."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (ux, uy, wz)]
+    ny, nx = arrs[0].shape
+    L = lib()
+    L.tbl_baseflow_write.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]
+    _check(L.tbl_baseflow_write(path.encode(), nx, ny, *[a.ctypes.data for a in arrs]),
+           "tbl_baseflow_write")
+
+
+def baseflow_read(path: str, nx: int, ny: int):
+    """Read based_RR.bin: (ux, uy, wz) real (ny, nx)."""
+    arrs = [np.zeros((ny, nx), np.float64) for _ in range(3)]
+    L = lib()
+    L.tbl_baseflow_read.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                    C.c_void_p]
+    _check(L.tbl_baseflow_read(path.encode(), nx, ny, *[a.ctypes.data for a in arrs]),
+           "tbl_baseflow_read")
+    return tuple(arrs)
 
 
 def coefs_to_dict(c: CCoefs, levels: int) -> dict:

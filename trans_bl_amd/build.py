@@ -42,10 +42,11 @@ SOURCES = [
     "tbl_guess.hip",
     "tbl_batch.cpp",
     "tbl_noalloc.hip",
+    "tbl_files.cpp",
 ]
 
 
-HOST_ONLY = {"tbl_host.cpp"}
+HOST_ONLY = {"tbl_host.cpp", "tbl_files.cpp"}
 
 
 def _compile(src):
