@@ -63,6 +63,8 @@ typedef struct tbl_coefs {
     double cp_poi[TBL_MAX_LEVELS][8];
     double pp_poi[TBL_MAX_LEVELS][4];
     double lp_poi[TBL_MAX_LEVELS][5];
+    double w_at_w[9];   /* w_at_w_coef: wall vorticity from uy (coefs.bin record 15) */
+    double dwydy[8];    /* dwydy_coef: one-sided d/dy at the wall (record 16)      */
 } tbl_coefs;
 
 /* Solver options.  Zero-initialised fields take the reference's values. */

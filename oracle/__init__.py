@@ -32,6 +32,7 @@ class Params(C.Structure):
         ("cp_sd", C.c_double * 8), ("pp_sd", C.c_double * 10), ("lp_sd", C.c_double * 8),
         ("sp_poi", (C.c_double * 7) * MAXLVL), ("cp_poi", (C.c_double * 8) * MAXLVL),
         ("pp_poi", (C.c_double * 4) * MAXLVL), ("lp_poi", (C.c_double * 5) * MAXLVL),
+        ("w_at_w", C.c_double * 9), ("dwydy", C.c_double * 8),
     ]
 
 
@@ -84,6 +85,20 @@ def lib():
         L.or_source.restype = C.c_int
         L.or_sor.argtypes = [P, M, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int]
         L.or_sor.restype = C.c_int
+        L.or_lhs5.argtypes = [C.c_int, C.c_double, vp]
+        L.or_ludecomp.argtypes = [C.c_int, vp, vp]
+        L.or_lusolve.argtypes = [C.c_int, vp, vp, C.c_size_t, C.c_int]
+        for name, n in (("or_poi_ux", 4), ("or_poi_uz", 3), ("or_wx_wall", 5),
+                        ("or_wz_wall", 3), ("or_filter_trid", 3), ("or_outuy", 3)):
+            getattr(L, name).argtypes = [P] + [vp] * n
+            getattr(L, name).restype = C.c_int
+        L.or_der2wydxdy.argtypes = [P, vp, C.c_int, vp]
+        L.or_der2wydxdy.restype = C.c_int
+        L.or_bzone.argtypes = [P, vp, C.c_int, vp, vp, vp, vp]
+        L.or_bzone.restype = C.c_int
+        L.or_bdfc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, vp]
+        L.or_loop.argtypes = [P, C.c_double, vp, vp] + [vp] * 7 + [C.c_int, vp, vp]
+        L.or_loop.restype = C.c_int
         _lib = L
     return _lib
 
@@ -109,6 +124,11 @@ def make_params(nx, ny, nk, msh, dx, dy0, stf, alpha=0.0, beta=0.2, fac_y=1.0,
         for l in range(msh):
             for i, v in enumerate(tab[l]):
                 arr[l][i] = v
+    for name in ("w_at_w", "dwydy"):
+        if name in coefs:
+            arr = getattr(p, name)
+            for i, v in enumerate(np.asarray(coefs[name], dtype=np.float64)):
+                arr[i] = v
     return p
 
 
@@ -263,3 +283,133 @@ def sor(p, lvl, k, v, S, itera, order=0):
     S, pS = _c(np.asarray(S, dtype=np.complex128))
     lib().or_sor(C.byref(p), C.byref(m), lvl, k, v.ctypes.data_as(C.c_void_p), pS, itera, order)
     return v
+
+
+# ---- the rest of loop (tbl_oracle_loop.c) -----------------------------------
+def _cf(a, shape):
+    a = np.array(a, dtype=np.complex128, order="C", copy=True)
+    assert a.shape == shape, (a.shape, shape)
+    return a
+
+
+def _vp(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _rc(rc, name):
+    if rc:
+        raise ValueError(f"{name} rejected the grid (rc={rc})")
+
+
+def lhs5(n, k2b2):
+    a = np.zeros((n, 5))
+    lib().or_lhs5(n, k2b2, _vp(a))
+    return a
+
+
+def ludecomp(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    lu = np.zeros_like(a)
+    lib().or_ludecomp(a.shape[0], _vp(a), _vp(lu))
+    return lu
+
+
+def lusolve(lu, rhs, variant=0):
+    lu = np.ascontiguousarray(lu, dtype=np.float64)
+    r = np.array(rhs, dtype=np.complex128, copy=True)
+    lib().or_lusolve(lu.shape[0], _vp(lu), _vp(r), 1, variant)
+    return r
+
+
+def poi_ux(p, uy, wy, ux):
+    """poi_ux (loop_par.f:428-556): returns (ux, duydy)."""
+    shp = _field(p)
+    uy, wy, ux = _cf(uy, shp), _cf(wy, shp), _cf(ux, shp)
+    duydy = np.zeros(shp, np.complex128)
+    _rc(lib().or_poi_ux(C.byref(p), _vp(uy), _vp(wy), _vp(ux), _vp(duydy)), "or_poi_ux")
+    return ux, duydy
+
+
+def poi_uz(p, wy, duydy, uz):
+    shp = _field(p)
+    wy, du, uz = _cf(wy, shp), _cf(duydy, shp), _cf(uz, shp)
+    _rc(lib().or_poi_uz(C.byref(p), _vp(wy), _vp(du), _vp(uz)), "or_poi_uz")
+    return uz
+
+
+def der2wydxdy(p, wy, k):
+    wy = _cf(wy, _field(p))
+    d = np.zeros(p.nx, np.complex128)
+    _rc(lib().or_der2wydxdy(C.byref(p), _vp(wy), k, _vp(d)), "or_der2wydxdy")
+    return d
+
+
+def wx_wall(p, uy, wy, d2uydx2, wx):
+    """wx_wall (loop_par.f:174-255): returns (wx, lapv (nk, nx))."""
+    shp = _field(p)
+    uy, wy, wx = _cf(uy, shp), _cf(wy, shp), _cf(wx, shp)
+    d2 = _cf(d2uydx2, (p.nk, p.nx))
+    lapv = np.zeros((p.nk, p.nx), np.complex128)
+    _rc(lib().or_wx_wall(C.byref(p), _vp(uy), _vp(wy), _vp(d2), _vp(wx), _vp(lapv)),
+        "or_wx_wall")
+    return wx, lapv
+
+
+def wz_wall(p, wx, lapv, wz):
+    shp = _field(p)
+    wx, wz = _cf(wx, shp), _cf(wz, shp)
+    lapv = _cf(lapv, (p.nk, p.nx))
+    _rc(lib().or_wz_wall(C.byref(p), _vp(wx), _vp(lapv), _vp(wz)), "or_wz_wall")
+    return wz
+
+
+def bzone(p, bdfc, wx, wy, wz, th=None, wall_row=False):
+    shp = _field(p)
+    f = [_cf(a, shp) for a in (wx, wy, wz)]
+    th = None if th is None else _cf(th, shp)
+    b = np.ascontiguousarray(bdfc, dtype=np.float64)
+    assert b.shape == (p.nx, 2)
+    lib().or_bzone(C.byref(p), _vp(b), int(wall_row), *[_vp(a) for a in f], _vp(th))
+    return (*f, th)
+
+
+def filter_trid(p, wx, wy, wz):
+    shp = _field(p)
+    f = [_cf(a, shp) for a in (wx, wy, wz)]
+    _rc(lib().or_filter_trid(C.byref(p), *[_vp(a) for a in f]), "or_filter_trid")
+    return tuple(f)
+
+
+def outuy(p, wz, uy):
+    """outuy (loop_par.f:48-119): returns (uy, dwzdx)."""
+    shp = _field(p)
+    wz, uy = _cf(wz, shp), _cf(uy, shp)
+    d = np.zeros(shp, np.complex128)
+    _rc(lib().or_outuy(C.byref(p), _vp(wz), _vp(uy), _vp(d)), "or_outuy")
+    return uy, d
+
+
+def bdfc(nx, i0, i3, i4):
+    """Buffer-zone table (main_par.f:738-755) as (nx, 2): C view of the
+    Fortran bdfc(2, imax)."""
+    b = np.zeros((nx, 2))
+    lib().or_bdfc(nx, i0, i3, i4, _vp(b))
+    return b
+
+
+FIELDS = ("ux", "uy", "uz", "wx", "wy", "wz", "th")
+
+
+def loop(p, erro, bdfc_, d2uydx2, fields, order=0):
+    """loop(erro) (main_par.f:1762-1785): fields is a dict of the complex
+    (nk, ny, nx) arrays ux, uy, uz, wx, wy, wz (th optional: my_form 2).
+    Returns (new fields dict, cycles, residuals) of the poi_uy call."""
+    shp = _field(p)
+    f = {k: _cf(fields[k], shp) for k in FIELDS if fields.get(k) is not None}
+    b = np.ascontiguousarray(bdfc_, dtype=np.float64)
+    d2 = _cf(d2uydx2, (p.nk, p.nx))
+    cyc = np.zeros(p.nk, np.int32)
+    res = np.zeros(p.nk)
+    _rc(lib().or_loop(C.byref(p), erro, _vp(b), _vp(d2), *[_vp(f.get(k)) for k in FIELDS],
+                      order, _vp(cyc), _vp(res)), "or_loop")
+    return f, cyc, res

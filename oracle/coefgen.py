@@ -189,6 +189,27 @@ def poisson_schemes(strl):
     return dict(sp_poi=sp, cp_poi=cp, pp_poi=pp, lp_poi=lp)
 
 
+# ---------------------------------------------------------------------------
+# wall tables: derivative.f90:1618-1757 (w_at_w), 1887-1998 (dwydy), solved in
+# real(kind=10) (derivative.f90:1838-1843)
+# ---------------------------------------------------------------------------
+def wall_schemes(stf):
+    s = LD(stf)
+    hp = _cum([LD(1), s, s**2, s**3, s**4, s**5])     # rows 2..7 above the wall
+    # dwydy: f'(wall) = sum_j c(j+1) f(j), j = 1..7, with c(1) = 1
+    dwydy = solve_scheme([(LHS, D1, 0)] + [(RHS, F, h) for h in [LD(0)] + hp], {0: 1}, 6)
+    # w_at_w: c(1) f''(wall) + c(2) f'(wall) = -sum_j c(j+2) f(j), exact to the
+    # 7th moment, normalised by c(9) = -1/18.  The reference's first-moment row
+    # also carries -c(3) (derivative.f90:1645-1653), which moves only c(2);
+    # c(2) enters no formula of the path.
+    w = solve_scheme([(LHS, D2, 0), (LHS, D1, 0)] + [(RHS, F, h) for h in [LD(0)] + hp],
+                     {8: LD(-1) / LD(18)}, 7)
+    w[1] = w[2] + sum(w[3 + n] * hp[n] for n in range(6))
+    if abs(w[1]) < FLUSH:
+        w[1] = 0
+    return dict(w_at_w=w, dwydy=dwydy)
+
+
 def level_stretch(stf, lvl):
     """stf**(2**(lvl-1)) as the reference evaluates it (real*8 power)."""
     return float(stf) ** (2 ** (lvl - 1))
@@ -204,6 +225,8 @@ def generate(stf, levels):
     for k, v in fd_schemes(stf).items():
         out[k] = v.astype(np.float64)
     for k, v in sd_schemes(stf).items():
+        out[k] = v.astype(np.float64)
+    for k, v in wall_schemes(stf).items():
         out[k] = v.astype(np.float64)
     per = [poisson_schemes(level_stretch(stf, l)) for l in range(1, levels + 1)]
     for k in ("sp_poi", "cp_poi", "pp_poi", "lp_poi"):

@@ -4,7 +4,9 @@ The recipe compiles the reference's sources where they lie
 (``/root/reference``: the ten objects of its makefile, unchanged) with
 amdflang and the image's MPICH, against a ``par.for`` configured for one
 workload -- the reference's own per-run configuration step (every size is a
-PARAMETER of par.for) -- and links them with ``oracle/ref_harness.f``.  The
+PARAMETER of par.for) -- and links them with ``oracle/ref_harness.f``
+(poi_uy and the derivatives) and ``oracle/ref_loop.f`` (the other routines
+of ``loop``).  The
 coefficient file comes from the reference's own generator
 (pre_processing/coef_generation.f90 and its modules, with the constants
 module written by the reference's convert.pl from the same par.for), run
@@ -94,12 +96,16 @@ def exe_path(name: str) -> str:
     return os.path.join(OUT, name, "ref_harness")
 
 
+def loop_exe_path(name: str) -> str:
+    return os.path.join(OUT, name, "ref_loop")
+
+
 def coefs_path(name: str) -> str:
     return os.path.join(OUT, name, "coefs.bin")
 
 
 def available(name: str) -> bool:
-    return os.path.exists(exe_path(name)) and os.path.exists(coefs_path(name))
+    return all(os.path.exists(f) for f in (exe_path(name), loop_exe_path(name), coefs_path(name)))
 
 
 def buildable() -> bool:
@@ -134,8 +140,10 @@ def build(name: str, verbose: bool = True) -> str:
         # GNU ld: it places the COMMON blocks (gigabytes at the larger sizes)
         # after the Fortran runtime's own data, which is built for the small
         # code model; no relaxation of the GOT loads of those blocks
-        _run([FC, "-fuse-ld=bfd", "-mcmodel=medium", "-o", exe, "harness.o", *objs, "-L", MPI_LIB,
-              "-lmpifort", "-lmpi", f"-Wl,-rpath,{MPI_LIB}", "-Wl,--no-relax"], tmp)
+        _run([FC, *flags, "-c", os.path.join(HERE, "ref_loop.f"), "-o", "loop.o"], tmp)
+        for main, out in (("harness.o", exe), ("loop.o", loop_exe_path(name))):
+            _run([FC, "-fuse-ld=bfd", "-mcmodel=medium", "-o", out, main, *objs, "-L", MPI_LIB,
+                  "-lmpifort", "-lmpi", f"-Wl,-rpath,{MPI_LIB}", "-Wl,--no-relax"], tmp)
         # coefficient file: the reference's generator for this par.for
         gen = os.path.join(tmp, "gen")
         os.makedirs(os.path.join(gen, "pre_processing"))
@@ -176,6 +184,8 @@ def read_coefs(name: str) -> dict:
     out = {n: recs[q] for q, n in enumerate(names)}
     for q, (n, w) in enumerate((("sp_poi", 7), ("cp_poi", 8), ("pp_poi", 4), ("lp_poi", 5))):
         out[n] = recs[10 + q].reshape(msh, w)
+    out["w_at_w"] = recs[14]
+    out["dwydy"] = recs[15]
     return out
 
 
@@ -280,3 +290,58 @@ def run_parallel(name: str, wx, dwzdx, uy, erromax: float, copies: int,
 
 if __name__ == "__main__":
     build_all()
+
+
+LOOP_FIELDS = ("ux", "uy", "uz", "wx", "wy", "wz", "th")
+
+
+def run_loop(name: str, fields: dict, d2uydx2, bdfc, erro: float, what: str = "loop",
+             timeout: float = 600.0):
+    """One of the reference's loop routines (ref_loop.f: loop, poi, wall,
+    bzone, bzonew, filter, outuy) on the given fields -- complex (nk, ny, nx)
+    arrays ux .. th (missing ones zero), d2uydx2 complex (nk, nx), bdfc real
+    (nx, 2).  Returns the fields after the call, the extra output ("duydy",
+    "lapv" or "dwzdx") and, for loop, poi_uy's per-mode cycles and residuals."""
+    if not available(name):
+        raise RuntimeError(f"reference driver {name} is not built")
+    nx, ny, nk = CONFIGS[name][0], CONFIGS[name][1], CONFIGS[name][2]
+    shape = (nk, ny, nx)
+    tmp = tempfile.mkdtemp(prefix="tbl_refloop_")
+    try:
+        fin = os.path.join(tmp, "in.bin")
+        fout = os.path.join(tmp, "out.bin")
+        with open(fin, "wb") as f:
+            for k in LOOP_FIELDS:
+                a = fields.get(k)
+                a = np.zeros(shape, np.complex128) if a is None else a
+                f.write(np.ascontiguousarray(a, dtype="<c16").tobytes())
+            f.write(np.ascontiguousarray(d2uydx2, dtype="<c16").tobytes())
+            f.write(np.ascontiguousarray(bdfc, dtype="<f8").tobytes())
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        r = subprocess.run(["bash", "-c", 'ulimit -s unlimited && exec "$@"', "ref",
+                            loop_exe_path(name), coefs_path(name), fin, fout, repr(float(erro)),
+                            what], capture_output=True, text=True, timeout=timeout, env=env,
+                           cwd=tmp)
+        if r.returncode != 0:
+            raise RuntimeError(f"ref_loop failed:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        b = np.fromfile(fout, dtype="<c16")
+        n3 = nk * ny * nx
+        out = {k: b[q * n3:(q + 1) * n3].reshape(shape).copy()
+               for q, k in enumerate(LOOP_FIELDS)}
+        rest = b[len(LOOP_FIELDS) * n3:]
+        if what in ("poi", "outuy"):
+            out["duydy" if what == "poi" else "dwzdx"] = rest.reshape(shape).copy()
+        elif what == "wall":
+            out["lapv"] = rest.reshape(nk, nx).copy()
+        cycles = np.zeros(nk, np.int32)
+        res = np.zeros(nk, np.float64)
+        for line in r.stdout.splitlines():
+            t = line.split()
+            if len(t) == 3 and t[0].isdigit():
+                cycles[int(t[0]) - 1] = int(t[1])
+                res[int(t[0]) - 1] = float(t[2].replace("D", "E"))
+        out["cycles"] = cycles
+        out["residual"] = res
+        return out
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)

@@ -15,6 +15,7 @@
 #define TBL_ORACLE_H
 
 #include <complex.h>
+#include <stddef.h>
 
 #define OR_MAXLVL 8
 
@@ -29,6 +30,7 @@ typedef struct or_params {
     double fp_sd[8], sp_sd[10], cp_sd[8], pp_sd[10], lp_sd[8];
     double sp_poi[OR_MAXLVL][7], cp_poi[OR_MAXLVL][8];
     double pp_poi[OR_MAXLVL][4], lp_poi[OR_MAXLVL][5];
+    double w_at_w[9], dwydy[8];   /* w_at_w_coef, dwydy_coef (comm.coef:13) */
 } or_params;
 
 /* create_ctes (main_par.f:1828-1892).  qdy: ny; qdy2: msh*ny ([lvl][j]). */
@@ -93,5 +95,43 @@ int or_restrict(const or_params *p, const or_metrics *m, int lvl,
                 or_cplx *Sc);
 int or_correct_prolong(const or_params *p, const or_metrics *m, int lvl,
                        or_cplx *vc, const or_cplx *wc, or_cplx *vf);
+
+/* ---- the rest of loop (main_par.f:1762-1785): tbl_oracle_loop.c --------- */
+/* lhs5 (loop_par.f:258-299): a is n x 5, row i holds columns i-2..i+2 */
+void or_lhs5(int n, double k2b2, double *a);
+/* ludecomp (penta_par.f:6-45): lu n x 5 -- columns 1-2 the unit-lower
+   multipliers (offsets -2, -1), 3-5 the upper band (0, +1, +2) */
+void or_ludecomp(int n, const double *a, double *lu);
+/* lusolver at one rank (penta_par.f:139-208) on one line of stride `stride`;
+   variant 1 = lusolver_1's order of the two forward terms (53-136) */
+void or_lusolve(int n, const double *lu, or_cplx *rhs, size_t stride, int variant);
+
+/* poi_ux (loop_par.f:428-556): duydy = deryfv(uy) out; ux in/out */
+int or_poi_ux(const or_params *p, const or_cplx *uy, const or_cplx *wy, or_cplx *ux,
+              or_cplx *duydy);
+/* poi_uz (loop_par.f:559-688) */
+int or_poi_uz(const or_params *p, const or_cplx *wy, const or_cplx *duydy, or_cplx *uz);
+/* der2wydxdy for mode k (loop_par.f:300-361): d (nx) out */
+int or_der2wydxdy(const or_params *p, const or_cplx *wy, int k, or_cplx *d);
+/* wx_wall (loop_par.f:174-255): lapv (nx, nk) out; wx in/out */
+int or_wx_wall(const or_params *p, const or_cplx *uy, const or_cplx *wy,
+               const or_cplx *d2uydx2, or_cplx *wx, or_cplx *lapv);
+/* wz_wall (loop_par.f:364-425): wz in/out */
+int or_wz_wall(const or_params *p, const or_cplx *wx, const or_cplx *lapv, or_cplx *wz);
+/* bzone (loop_par.f:6-45, rows j >= 2) or bzonew (691-725, row 1), my_form 1
+   (th NULL) or 2; bdfc is Fortran (2, nx) */
+int or_bzone(const or_params *p, const double *bdfc, int wall_row, or_cplx *wx, or_cplx *wy,
+             or_cplx *wz, or_cplx *th);
+/* filter_trid (filter_par.f:144-178, 181-332, derivs_par.f:1009-1037) */
+int or_filter_trid(const or_params *p, or_cplx *wx, or_cplx *wy, or_cplx *wz);
+/* outuy (loop_par.f:48-119), my_form 1 or 2: dwzdx out, uy(nx, 2..ny, :) = 0 */
+int or_outuy(const or_params *p, const or_cplx *wz, or_cplx *uy, or_cplx *dwzdx);
+/* buffer-zone table (main_par.f:738-755), Fortran (2, nx); i0, i3, i4 1-based */
+void or_bdfc(int nx, int i0, int i3, int i4, double *bdfc);
+/* loop(erro) (main_par.f:1762-1785), my_form 1 (th NULL) or 2.  order as
+   or_poi_uy; cycles/res per mode of the poi_uy call. */
+int or_loop(const or_params *p, double erro, const double *bdfc, const or_cplx *d2uydx2,
+            or_cplx *ux, or_cplx *uy, or_cplx *uz, or_cplx *wx, or_cplx *wy, or_cplx *wz,
+            or_cplx *th, int order, int *cycles, double *res);
 
 #endif

@@ -47,6 +47,7 @@ class CCoefs(C.Structure):
         ("pp_sd", C.c_double * 10), ("lp_sd", C.c_double * 8),
         ("sp_poi", (C.c_double * 7) * MAX_LEVELS), ("cp_poi", (C.c_double * 8) * MAX_LEVELS),
         ("pp_poi", (C.c_double * 4) * MAX_LEVELS), ("lp_poi", (C.c_double * 5) * MAX_LEVELS),
+        ("w_at_w", C.c_double * 9), ("dwydy", C.c_double * 8),
     ]
 
 

@@ -184,6 +184,27 @@ bool generate_coefs(double stf, int levels, tbl_coefs *c)
         add_f(u, pts);
         ok &= solve_scheme(u, {{0, 1}}, 6, c->lp_sd);
     }
+    // wall tables (derivative.f90:1618-1757, 1887-1998): rows 1..7 from the wall
+    const auto hp = walk({1, s, s * s, s * s * s, s * s * s * s, s * s * s * s * s});
+    {   // dwydy: f'(wall) = sum_j c(j+1) f(j), c(1) = 1
+        std::vector<Unknown> u{{+1, KD1, 0}};
+        std::vector<ld> pts{0};
+        pts.insert(pts.end(), hp.begin(), hp.end());
+        add_f(u, pts);
+        ok &= solve_scheme(u, {{0, 1}}, 6, c->dwydy);
+    }
+    {   // w_at_w: c(1) f''(wall) + c(2) f'(wall) = -sum_j c(j+2) f(j), c(9) = -1/18.
+        // The reference's first-moment row also carries -c(3), which moves
+        // only c(2); c(2) enters no formula of the path.
+        std::vector<Unknown> u{{+1, KD2, 0}, {+1, KD1, 0}};
+        std::vector<ld> pts{0};
+        pts.insert(pts.end(), hp.begin(), hp.end());
+        add_f(u, pts);
+        ok &= solve_scheme(u, {{8, -1.0L / 18.0L}}, 7, c->w_at_w);
+        ld c2 = c->w_at_w[2];
+        for (int n = 0; n < 6; ++n) c2 += static_cast<ld>(c->w_at_w[3 + n]) * hp[n];
+        c->w_at_w[1] = (fabsl(c2) < 1e-14L) ? 0.0 : static_cast<double>(c2);
+    }
     // Poisson schemes per level at stretch stf**(2**(l-1)) (coef_generation.f90:32-38)
     for (int l = 1; l <= levels && l <= TBL_MAX_LEVELS; ++l) {
         double sd = 1.0;
@@ -470,6 +491,14 @@ int read_coefs_bin(const char *path, int levels, tbl_coefs *c, std::string *err)
             return TBL_ERR_ARG;
         }
         std::memcpy(dst[i], rec[i].data(), want[i] * sizeof(double));
+    }
+    if (rec.size() >= 16) {     // the wall tables (coef_generation.f90:54-55)
+        if (rec[14].size() != 9 || rec[15].size() != 8) {
+            *err = "wall records 15-16 must hold 9 and 8 entries";
+            return TBL_ERR_ARG;
+        }
+        std::memcpy(c->w_at_w, rec[14].data(), 9 * sizeof(double));
+        std::memcpy(c->dwydy, rec[15].data(), 8 * sizeof(double));
     }
     const int per[4] = {7, 8, 4, 5};
     for (int t = 0; t < 4; ++t) {

@@ -54,6 +54,8 @@ cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
       coefs%cp_sd = cp_sd_coef
       coefs%pp_sd = pp_sd_coef
       coefs%lp_sd = lp_sd_coef
+      coefs%w_at_w = w_at_w_coef
+      coefs%dwydy = dwydy_coef
       coefs%sp_poi = 0.d0
       coefs%cp_poi = 0.d0
       coefs%pp_poi = 0.d0

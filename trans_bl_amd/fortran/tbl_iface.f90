@@ -24,6 +24,7 @@ module tbl_iface
     real(c_double) :: cp_poi(8, TBL_MAX_LEVELS)
     real(c_double) :: pp_poi(4, TBL_MAX_LEVELS)
     real(c_double) :: lp_poi(5, TBL_MAX_LEVELS)
+    real(c_double) :: w_at_w(9), dwydy(8)
   end type
 
   type, bind(C) :: tbl_solve_opts_t
