@@ -213,6 +213,51 @@ int tbl_residual_dev(tbl_ctx *ctx, const double *d_wx, const double *d_dwzdx,
 int tbl_residual_dev_noalloc(tbl_ctx *ctx, const double *d_wx, const double *d_dwzdx,
                              const double *d_uy, double *d_out, double *residual);
 
+/* ---- the rest of the RK-stage update: `loop` (main_par.f:1762-1785) --------
+   Device pointers, user layout (complex (nx, ny, nmodes), x fastest);
+   (nx, K) arrays are complex (ptsx, kfour).  my_form 1, or 2 with a th
+   field (NULL th: my_form 1).  Outputs keep column 1 (x inflow), as the
+   reference's routines at rank 0 do.  Needs nx >= 8, ny >= 8. */
+
+/* poi_ux (loop_par.f:428-556): duydy = deryfv(uy) (out); ux of mode 1 by
+   continuity on rows 2..ny-1, of modes >= 2 by the x-direction compact
+   Poisson equation (lhs5 258-299, ludecomp/lusolver penta_par.f:6-208). */
+int tbl_poi_ux_dev(tbl_ctx *ctx, const double *d_uy, const double *d_wy, double *d_ux,
+                   double *d_duydy);
+/* poi_uz (loop_par.f:559-688): modes >= 2, duydy from tbl_poi_ux_dev. */
+int tbl_poi_uz_dev(tbl_ctx *ctx, const double *d_wy, const double *d_duydy, double *d_uz);
+/* wx_wall + der2wydxdy (loop_par.f:174-255, 300-361): wall row of wx for
+   modes >= 2; lapv (nx, K) out, d2uydx2 (nx, K) in (COMMON /derw/). */
+int tbl_wx_wall_dev(tbl_ctx *ctx, const double *d_uy, const double *d_wy,
+                    const double *d_d2uydx2, double *d_wx, double *d_lapv);
+/* wz_wall (loop_par.f:364-425): wall row of wz for every mode. */
+int tbl_wz_wall_dev(tbl_ctx *ctx, const double *d_wx, const double *d_lapv, double *d_wz);
+/* bzone (loop_par.f:6-45, rows 2..ny; wall_row 0) or bzonew (691-725, row 1;
+   wall_row 1): multiply by the buffer-zone table bdfc (real (2, nx), COMMON
+   /bd/); th (my_form 2) only on rows 2..ny. */
+int tbl_bzone_dev(tbl_ctx *ctx, const double *d_bdfc, int wall_row, double *d_wx, double *d_wy,
+                  double *d_wz, double *d_th);
+/* filter_trid (filter_par.f:144-332): compact x filter of wx, wy, wz. */
+int tbl_filter_trid_dev(tbl_ctx *ctx, double *d_wx, double *d_wy, double *d_wz);
+/* outuy (loop_par.f:48-119, my_form 1 and 2): dwzdx = derparx(wz) (out),
+   uy(nx, 2..ny, :) = 0. */
+int tbl_outuy_dev(tbl_ctx *ctx, const double *d_wz, double *d_uy, double *d_dwzdx);
+/* loop(erro) (main_par.f:1762-1785): filter_trid when erro < 1e-5, bzone,
+   outuy, poi_uy (to erro, with opts; cycles/residual/stats as
+   tbl_poi_uy_dev), poi_ux, poi_uz, wx_wall, wz_wall, bzonew. */
+int tbl_loop_dev(tbl_ctx *ctx, double erro, const double *d_bdfc, const double *d_d2uydx2,
+                 double *d_ux, double *d_uy, double *d_uz, double *d_wx, double *d_wy,
+                 double *d_wz, double *d_th, const tbl_solve_opts *opts, int *cycles,
+                 double *residual, tbl_solve_stats *stats);
+/* Same with host arrays (the Fortran drop-in's `loop`): every field is
+   copied to the GPU and back. */
+int tbl_loop(tbl_ctx *ctx, double erro, const double *bdfc, const double *d2uydx2, double *ux,
+             double *uy, double *uz, double *wx, double *wy, double *wz, double *th,
+             const tbl_solve_opts *opts, int *cycles, double *residual, tbl_solve_stats *stats);
+/* The buffer-zone table bdfc (2, nx) as init_gv builds it
+   (main_par.f:738-755): 1-based i0, i3, i4 from par.for. */
+int tbl_bdfc_generate(int nx, int i0, int i3, int i4, double *bdfc);
+
 /* Line-solve shape of one multigrid level's smoother: `lanes` (16, 32 or
    64) lanes of a wavefront share one y-line; a negative value selects the
    same shape with LDS-staged, coalesced loads and stores; 2000 + lanes

@@ -1,0 +1,196 @@
+"""GPU parity of the rest of the RK-stage update (`loop`, main_par.f:1762-1785)
+against the oracle (itself pinned to the reference's own routines in
+test_oracle_loop.py).
+
+Tolerances (relative to the largest magnitude of the compared field):
+* poi_ux / poi_uz / wx_wall / wz_wall: 1e-12 (measured at most 6e-15 on
+  these inputs, 5e-14 after a whole loop).  The GPU solves each x-line
+  recurrence as a partitioned lane scan and multiplies by reciprocal pivots,
+  the reference substitutes sequentially and divides; both are exact in
+  exact arithmetic and differ by rounding, which the conditioning of the
+  x-direction Laplacian (about nx^2) can amplify.
+* filter_trid, bzone, outuy: 1e-13 (explicit stencils and a well-conditioned
+  tridiagonal solve).
+* loop: the poi_uy inside it is the GPU's colour-ordered FAS cycle, so the
+  oracle runs with the same order (iterate parity, as in test_gpu_parity.py);
+  per-mode cycle counts equal, fields 1e-12.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import trans_bl_amd as tb
+from trans_bl_amd import workload as wl
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+SMALL = wl.Grid(129, 33, 4, 4, 0.03, 1e-3, 1.03, 0.2, 0.6)
+CFG1 = wl.Grid(385, 97, 8, 4, 0.02, 5e-4, 1.02, 0.0, 0.2)
+CFG2 = wl.Grid(1025, 193, 64, 4, 0.02, 2e-4, 1.015, 0.0, 0.1)
+LONG = wl.Grid(2049, 33, 4, 3, 0.01, 1e-3, 1.03, 0.0, 0.3)      # 33 rows per lane
+TOL = 1e-12
+
+
+def coefs_of(g):
+    return oracle.coefgen.generate(g.stf, g.levels)
+
+
+def oparams(g):
+    return oracle.make_params(g.nx, g.ny, g.nmodes, g.levels, g.dx, g.dy0, g.stf, g.alpha,
+                              g.beta, g.fac_y, coefs=coefs_of(g))
+
+
+def inputs(g):
+    F = {k: wl.wavepacket_rhs(g, tag="gpuloop-" + k) * (0.1 if k[0] == "w" else 1.0)
+         for k in ("ux", "uy", "uz", "wx", "wy", "wz")}
+    rng = np.random.default_rng(7)
+    d2 = rng.standard_normal((g.nmodes, g.nx)) + 1j * rng.standard_normal((g.nmodes, g.nx))
+    b = tb.bdfc_generate(g.nx, 10, int(g.nx * 0.785), int(g.nx * 0.88))
+    return F, d2, b
+
+
+@pytest.fixture(scope="module")
+def torch_mod(gpu_ready):
+    import torch
+    return torch
+
+
+def _dev(torch, a):
+    t = torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    torch.cuda.synchronize()
+    return t
+
+
+def _host(torch, t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.mark.parametrize("g", [SMALL, CFG1, LONG], ids=["129x33x4", "385x97x8", "2049x33x4"])
+def test_poi_ux_uz_match_oracle(torch_mod, g):
+    torch = torch_mod
+    print(f"poi_ux/poi_uz {g.nx}x{g.ny}x{g.nmodes}", flush=True)
+    F, _, _ = inputs(g)
+    p = oparams(g)
+    ux_ref, du_ref = oracle.poi_ux(p, F["uy"], F["wy"], F["ux"])
+    uz_ref = oracle.poi_uz(p, F["wy"], du_ref, F["uz"])
+    with tb.Context(g, coefs=coefs_of(g)) as ctx:
+        uy, wy, ux, uz = (_dev(torch, F[k]) for k in ("uy", "wy", "ux", "uz"))
+        du = torch.zeros_like(uy)
+        ctx.poi_ux_dev(uy.data_ptr(), wy.data_ptr(), ux.data_ptr(), du.data_ptr())
+        ctx.poi_uz_dev(wy.data_ptr(), du.data_ptr(), uz.data_ptr())
+        ux, uz, du = (_host(torch, t) for t in (ux, uz, du))
+    e = [rel(ux, ux_ref), rel(uz, uz_ref), rel(du, du_ref)]
+    print(f"  rel ux {e[0]:.2e} uz {e[1]:.2e} duydy {e[2]:.2e}")
+    assert e[2] < 1e-12 and e[0] < TOL and e[1] < TOL
+    # column 1 and the mode-1 top row are inputs, kept bit for bit
+    assert np.array_equal(ux[:, :, 0], F["ux"][:, :, 0])
+    assert np.array_equal(ux[0, -1, :], F["ux"][0, -1, :])
+    assert np.array_equal(uz[0], F["uz"][0])
+
+
+@pytest.mark.parametrize("g", [SMALL, CFG1, LONG], ids=["129x33x4", "385x97x8", "2049x33x4"])
+def test_wall_vorticity_matches_oracle(torch_mod, g):
+    torch = torch_mod
+    print(f"wx_wall/wz_wall {g.nx}x{g.ny}x{g.nmodes}", flush=True)
+    F, d2, _ = inputs(g)
+    p = oparams(g)
+    wx_ref, lapv_ref = oracle.wx_wall(p, F["uy"], F["wy"], d2, F["wx"])
+    wz_ref = oracle.wz_wall(p, wx_ref, lapv_ref, F["wz"])
+    with tb.Context(g, coefs=coefs_of(g)) as ctx:
+        uy, wy, wx, wz, dd = (_dev(torch, a) for a in (F["uy"], F["wy"], F["wx"], F["wz"], d2))
+        lapv = torch.zeros_like(dd)
+        ctx.wx_wall_dev(uy.data_ptr(), wy.data_ptr(), dd.data_ptr(), wx.data_ptr(),
+                        lapv.data_ptr())
+        ctx.wz_wall_dev(wx.data_ptr(), lapv.data_ptr(), wz.data_ptr())
+        wx, wz, lapv = (_host(torch, t) for t in (wx, wz, lapv))
+    e = [rel(wx, wx_ref), rel(wz, wz_ref), rel(lapv, lapv_ref)]
+    print(f"  rel wx {e[0]:.2e} wz {e[1]:.2e} lapv {e[2]:.2e}")
+    assert max(e) < TOL
+    assert np.array_equal(wx[:, 1:, :], F["wx"][:, 1:, :])      # only the wall row moves
+    assert np.array_equal(wz[:, 1:, :], F["wz"][:, 1:, :])
+
+
+def test_filter_bzone_outuy_match_oracle(torch_mod):
+    torch = torch_mod
+    g = SMALL
+    F, _, b = inputs(g)
+    p = oparams(g)
+    with tb.Context(g, coefs=coefs_of(g)) as ctx:
+        w = [_dev(torch, F[k]) for k in ("wx", "wy", "wz")]
+        ctx.filter_trid_dev(*[t.data_ptr() for t in w])
+        got = [_host(torch, t) for t in w]
+        ref = oracle.filter_trid(p, F["wx"], F["wy"], F["wz"])
+        assert max(rel(a, r) for a, r in zip(got, ref)) < 1e-13
+        for wall in (False, True):
+            th0 = wl.wavepacket_rhs(g, tag="gpuloop-th")
+            w = [_dev(torch, F[k]) for k in ("wx", "wy", "wz")]
+            th = _dev(torch, th0)
+            db = _dev(torch, b)
+            ctx.bzone_dev(db.data_ptr(), *[t.data_ptr() for t in w], th.data_ptr(),
+                          wall_row=wall)
+            ref = oracle.bzone(p, b, F["wx"], F["wy"], F["wz"], th0, wall_row=wall)
+            for a, r in zip([*w, th], ref):
+                assert np.array_equal(_host(torch, a), r)
+        wz, uy = _dev(torch, F["wz"]), _dev(torch, F["uy"])
+        d = torch.zeros_like(wz)
+        ctx.outuy_dev(wz.data_ptr(), uy.data_ptr(), d.data_ptr())
+        uy_ref, d_ref = oracle.outuy(p, F["wz"], F["uy"])
+        assert np.array_equal(_host(torch, uy), uy_ref)
+        assert rel(_host(torch, d), d_ref) < 1e-12
+
+
+@pytest.mark.parametrize("g, erro", [(SMALL, 1e-5), (SMALL, 1e-6), (CFG1, 1e-6)],
+                         ids=["129x33x4-1e-5", "129x33x4-1e-6", "385x97x8-1e-6"])
+def test_loop_matches_oracle(torch_mod, g, erro):
+    """The whole loop(erro), device entry point and host entry point."""
+    torch = torch_mod
+    print(f"loop {g.nx}x{g.ny}x{g.nmodes} erro {erro}", flush=True)
+    F, d2, b = inputs(g)
+    p = oparams(g)
+    ref, cyc_ref, _ = oracle.loop(p, erro, b, d2, F, order=1)
+    with tb.Context(g, coefs=coefs_of(g)) as ctx:
+        dv = {k: _dev(torch, F[k]) for k in F}
+        db, dd = _dev(torch, b), _dev(torch, d2)
+        cyc, res, _ = ctx.loop_dev(erro, db.data_ptr(), dd.data_ptr(),
+                                   *[dv[k].data_ptr() for k in ("ux", "uy", "uz", "wx", "wy",
+                                                               "wz")])
+        got = {k: _host(torch, t) for k, t in dv.items()}
+        host = {k: np.array(F[k], copy=True) for k in F}
+        cyc_h, _, _ = ctx.loop(erro, b, d2, host)
+    assert np.array_equal(cyc, cyc_ref) and np.array_equal(cyc_h, cyc_ref)
+    for k in ref:
+        e = rel(got[k], ref[k])
+        print(f"  {k}: rel {e:.2e}")
+        assert e < TOL, k
+        assert np.array_equal(host[k], got[k]), k
+
+
+def test_poi_and_wall_at_benchmark_size(torch_mod):
+    """1025x193x64: poi_ux, poi_uz, wx_wall, wz_wall against the oracle."""
+    torch = torch_mod
+    g = CFG2
+    print("poi + wall 1025x193x64", flush=True)
+    F, d2, _ = inputs(g)
+    p = oparams(g)
+    ux_ref, du_ref = oracle.poi_ux(p, F["uy"], F["wy"], F["ux"])
+    uz_ref = oracle.poi_uz(p, F["wy"], du_ref, F["uz"])
+    wx_ref, lapv_ref = oracle.wx_wall(p, F["uy"], F["wy"], d2, F["wx"])
+    wz_ref = oracle.wz_wall(p, wx_ref, lapv_ref, F["wz"])
+    with tb.Context(g, coefs=coefs_of(g)) as ctx:
+        d = {k: _dev(torch, F[k]) for k in F}
+        du = torch.zeros_like(d["uy"])
+        dd = _dev(torch, d2)
+        lapv = torch.zeros_like(dd)
+        ctx.poi_ux_dev(d["uy"].data_ptr(), d["wy"].data_ptr(), d["ux"].data_ptr(),
+                       du.data_ptr())
+        ctx.poi_uz_dev(d["wy"].data_ptr(), du.data_ptr(), d["uz"].data_ptr())
+        ctx.wx_wall_dev(d["uy"].data_ptr(), d["wy"].data_ptr(), dd.data_ptr(),
+                        d["wx"].data_ptr(), lapv.data_ptr())
+        ctx.wz_wall_dev(d["wx"].data_ptr(), lapv.data_ptr(), d["wz"].data_ptr())
+        got = {k: _host(torch, d[k]) for k in ("ux", "uz", "wx", "wz")}
+    for k, r in (("ux", ux_ref), ("uz", uz_ref), ("wx", wx_ref), ("wz", wz_ref)):
+        e = rel(got[k], r)
+        print(f"  {k}: rel {e:.2e}")
+        assert e < TOL, k

@@ -77,7 +77,9 @@ EXPORTS = [
     "tbl_net_synchronize", "tbl_mfma_gemm_bf16", "tbl_guess_attach", "tbl_guess_shape",
     "tbl_guess_pairs", "tbl_batch_create", "tbl_batch_destroy", "tbl_batch_context",
     "tbl_batch_stage", "tbl_batch_results", "tbl_checkpoint_write", "tbl_checkpoint_read",
-    "tbl_baseflow_write", "tbl_baseflow_read",
+    "tbl_baseflow_write", "tbl_baseflow_read", "tbl_poi_ux_dev", "tbl_poi_uz_dev",
+    "tbl_wx_wall_dev", "tbl_wz_wall_dev", "tbl_bzone_dev", "tbl_filter_trid_dev",
+    "tbl_outuy_dev", "tbl_loop_dev", "tbl_loop", "tbl_bdfc_generate",
 ]
 
 _lib = None
@@ -113,6 +115,18 @@ def lib():
         "tbl_set_line_shape": ([vp, ip, ip, ip], C.c_int),
         "tbl_get_line_shape": ([vp, ip, ip], C.c_int),
         "tbl_time_smoother": ([vp, ip, ip, ip, C.POINTER(dp), C.POINTER(ip)], C.c_int),
+        "tbl_poi_ux_dev": ([vp, vp, vp, vp, vp], C.c_int),
+        "tbl_poi_uz_dev": ([vp, vp, vp, vp], C.c_int),
+        "tbl_wx_wall_dev": ([vp, vp, vp, vp, vp, vp], C.c_int),
+        "tbl_wz_wall_dev": ([vp, vp, vp, vp], C.c_int),
+        "tbl_bzone_dev": ([vp, vp, ip, vp, vp, vp, vp], C.c_int),
+        "tbl_filter_trid_dev": ([vp, vp, vp, vp], C.c_int),
+        "tbl_outuy_dev": ([vp, vp, vp, vp], C.c_int),
+        "tbl_loop_dev": ([vp, dp, vp, vp] + [vp] * 7 + [C.POINTER(SolveOpts), vp, vp,
+                                                     C.POINTER(SolveStats)], C.c_int),
+        "tbl_loop": ([vp, dp, vp, vp] + [vp] * 7 + [C.POINTER(SolveOpts), vp, vp,
+                                                 C.POINTER(SolveStats)], C.c_int),
+        "tbl_bdfc_generate": ([ip, ip, ip, ip, vp], C.c_int),
     }
     for n in ("tbl_derparx", "tbl_derparxx", "tbl_dery", "tbl_deryy", "tbl_deryfv"):
         types[n] = ([vp, vp, vp], C.c_int)
@@ -171,6 +185,14 @@ def checkpoint_read(path: str, nx: int, ny: int, nmodes: int, nfields: int = 6):
     _check(L.tbl_checkpoint_read(path.encode(), nx, ny, nmodes, C.byref(t), nfields, ptrs),
            "tbl_checkpoint_read")
     return t.value, arrs
+
+
+def bdfc_generate(nx: int, i0: int, i3: int, i4: int):
+    """Buffer-zone table (main_par.f:738-755) as (nx, 2): the C view of the
+    Fortran bdfc(2, imax)."""
+    b = np.zeros((nx, 2), np.float64)
+    _check(lib().tbl_bdfc_generate(nx, i0, i3, i4, _ptr(b)), "tbl_bdfc_generate")
+    return b
 
 
 def baseflow_write(path: str, ux, uy, wz) -> None:
@@ -361,6 +383,73 @@ class Context:
                                               C.c_void_p(out_ptr), _ptr(res)),
                "tbl_residual_dev_noalloc")
         return res
+
+    # ---- the rest of the RK-stage loop (device pointers, user layout) ----
+    def poi_ux_dev(self, uy, wy, ux, duydy):
+        _check(lib().tbl_poi_ux_dev(self.h, *[C.c_void_p(x) for x in (uy, wy, ux, duydy)]),
+               "tbl_poi_ux_dev")
+
+    def poi_uz_dev(self, wy, duydy, uz):
+        _check(lib().tbl_poi_uz_dev(self.h, *[C.c_void_p(x) for x in (wy, duydy, uz)]),
+               "tbl_poi_uz_dev")
+
+    def wx_wall_dev(self, uy, wy, d2uydx2, wx, lapv):
+        _check(lib().tbl_wx_wall_dev(self.h, *[C.c_void_p(x)
+                                               for x in (uy, wy, d2uydx2, wx, lapv)]),
+               "tbl_wx_wall_dev")
+
+    def wz_wall_dev(self, wx, lapv, wz):
+        _check(lib().tbl_wz_wall_dev(self.h, *[C.c_void_p(x) for x in (wx, lapv, wz)]),
+               "tbl_wz_wall_dev")
+
+    def bzone_dev(self, bdfc, wx, wy, wz, th=0, wall_row=False):
+        _check(lib().tbl_bzone_dev(self.h, C.c_void_p(bdfc), int(wall_row),
+                                   *[C.c_void_p(x) if x else None for x in (wx, wy, wz, th)]),
+               "tbl_bzone_dev")
+
+    def filter_trid_dev(self, wx, wy, wz):
+        _check(lib().tbl_filter_trid_dev(self.h, *[C.c_void_p(x) for x in (wx, wy, wz)]),
+               "tbl_filter_trid_dev")
+
+    def outuy_dev(self, wz, uy, dwzdx):
+        _check(lib().tbl_outuy_dev(self.h, *[C.c_void_p(x) for x in (wz, uy, dwzdx)]),
+               "tbl_outuy_dev")
+
+    def loop_dev(self, erro, bdfc, d2uydx2, ux, uy, uz, wx, wy, wz, th=0, **kw):
+        """loop(erro) on device fields; returns (cycles, residuals, stats)
+        of its poi_uy call."""
+        cyc = np.zeros(self.grid.nmodes, np.int32)
+        res = np.zeros(self.grid.nmodes, np.float64)
+        st = SolveStats()
+        o = self.opts(**kw)
+        ptrs = [C.c_void_p(x) if x else None for x in (ux, uy, uz, wx, wy, wz, th)]
+        _check(lib().tbl_loop_dev(self.h, erro, C.c_void_p(bdfc), C.c_void_p(d2uydx2), *ptrs,
+                                  C.byref(o), _ptr(cyc), _ptr(res), C.byref(st)),
+               "tbl_loop_dev")
+        return cyc, res, st.as_dict()
+
+    def loop(self, erro, bdfc, d2uydx2, fields: dict, **kw):
+        """loop(erro) on host arrays: fields is a dict of complex (nk, ny, nx)
+        arrays ux, uy, uz, wx, wy, wz (th optional), updated in place."""
+        arrs = {}
+        for k in ("ux", "uy", "uz", "wx", "wy", "wz", "th"):
+            a = fields.get(k)
+            if a is None:
+                continue
+            if not (a.dtype == np.complex128 and a.flags.c_contiguous):
+                raise TblError(f"field {k} must be a C-contiguous complex128 array")
+            arrs[k] = a
+        b = np.ascontiguousarray(bdfc, dtype=np.float64)
+        d2 = np.ascontiguousarray(d2uydx2, dtype=np.complex128)
+        cyc = np.zeros(self.grid.nmodes, np.int32)
+        res = np.zeros(self.grid.nmodes, np.float64)
+        st = SolveStats()
+        o = self.opts(**kw)
+        ptrs = [_ptr(arrs[k]) if k in arrs else None
+                for k in ("ux", "uy", "uz", "wx", "wy", "wz", "th")]
+        _check(lib().tbl_loop(self.h, erro, _ptr(b), _ptr(d2), *ptrs, C.byref(o), _ptr(cyc),
+                              _ptr(res), C.byref(st)), "tbl_loop")
+        return cyc, res, st.as_dict()
 
     def set_line_shape(self, prec, level, lanes):
         _check(lib().tbl_set_line_shape(self.h, prec, level, lanes), "tbl_set_line_shape")

@@ -43,6 +43,8 @@ SOURCES = [
     "tbl_batch.cpp",
     "tbl_noalloc.hip",
     "tbl_files.cpp",
+    "tbl_xline.hip",
+    "tbl_loop.cpp",
 ]
 
 

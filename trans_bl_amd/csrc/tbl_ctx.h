@@ -56,6 +56,13 @@ struct tbl_ctx {
 
     // learned initial guess (optional)
     tbl::Guess *guess = nullptr;
+
+    // routines of the RK-stage loop (tbl_loop.cpp), set up on first use:
+    // lane-segmented x-line tables (64 * xR rows each) and work fields
+    int xR = 0;
+    double *d_xpenta = nullptr, *d_xinteg = nullptr, *d_xfilter = nullptr;
+    tbl::cx<double> *d_t1 = nullptr, *d_t2 = nullptr;               // (nx, ny, K)
+    tbl::cx<double> *d_lapv = nullptr, *d_dwydy = nullptr, *d_d2wy = nullptr;  // (nx, K)
 };
 
 namespace tbl {

@@ -302,6 +302,8 @@ extern "C" int tbl_destroy(tbl_ctx *c)
     fr(c->d_active); fr(c->d_cycles); fr(c->d_nactive); fr(c->d_cycle);
     fr(c->d_resmax); fr(c->d_lastres); fr(c->d_qdy); fr(c->d_thomas); fr(c->d_tseg);
     fr(c->d_u0); fr(c->d_u1); fr(c->d_u2);
+    fr(c->d_xpenta); fr(c->d_xinteg); fr(c->d_xfilter);
+    fr(c->d_t1); fr(c->d_t2); fr(c->d_lapv); fr(c->d_dwydy); fr(c->d_d2wy);
     if (c->h_nactive) (void)hipHostFree(c->h_nactive);
     for (auto &g : c->graph)
         if (g) (void)hipGraphExecDestroy(g);
