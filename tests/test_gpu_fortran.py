@@ -1,7 +1,8 @@
 """The Fortran drop-in layer (trans_bl_amd/fortran) driven by a Fortran
 RK-stage program: derparx, derparxx, dery, deryy of uy, then poi_uy, with
 the reference's subroutine names, COMMON blocks and PARAMETERs (the test's
-own par.for, 129 x 33 x 4 modes, 3 levels), against the CPU oracle."""
+own par.for, 129 x 33 x 4 modes, 3 levels), against the CPU oracle; and a
+Fortran program calling the drop-in `loop(erro)`, the whole RK-stage update."""
 import os
 import subprocess
 
@@ -47,5 +48,46 @@ def test_fortran_stage_matches_oracle(tmp_path, gpu_ready):
     ref, cyc, res = oracle.poi_uy(p, wx, -f, uy, erromax, 11, order=1)
     assert rel(out[4], ref) < 1e-12, rel(out[4], ref)
     # poi_uy prints (k, V-cycles, residual) per mode like the reference
+    lines = [l.split() for l in r.stdout.strip().splitlines() if l.strip()]
+    assert [int(l[1]) for l in lines[-G.nmodes:]] == cyc.tolist()
+
+
+LOOP_EXE = os.path.join(os.path.dirname(EXE), "rk_loop")
+
+
+@pytest.mark.parametrize("erro", [1e-5, 1e-6])
+def test_fortran_loop_matches_oracle(tmp_path, gpu_ready, erro):
+    """The drop-in `loop(erro)` (main_par.f:1762-1785) from a Fortran
+    program with the reference's COMMON blocks /var/, /coefs/, /bd/, /derw/:
+    every field against the oracle's loop (colour-ordered poi_uy, as on the
+    GPU), and poi_uy's printed cycle counts."""
+    import trans_bl_amd as tb
+    if not os.path.exists(LOOP_EXE):
+        pytest.fail(f"{LOOP_EXE} not built (run __graft_entry__.build())")
+    coefs = oracle.coefgen.generate(G.stf, G.levels)
+    F = {k: wl.wavepacket_rhs(G, tag="floop-" + k) * (0.1 if k[0] == "w" else 1.0)
+         for k in ("ux", "uy", "uz", "wx", "wy", "wz")}
+    rng = np.random.default_rng(11)
+    d2 = rng.standard_normal((G.nmodes, G.nx)) + 1j * rng.standard_normal((G.nmodes, G.nx))
+    b = tb.bdfc_generate(G.nx, 10, 101, 113)
+    with open(tmp_path / "loop_in.bin", "wb") as fh:
+        for name in ("fp_fd_e", "sp_fd", "cp_fd", "pp_fd", "lp_fd", "fp_sd", "sp_sd", "cp_sd",
+                     "pp_sd", "lp_sd", "sp_poi", "cp_poi", "pp_poi", "lp_poi", "w_at_w",
+                     "dwydy"):
+            np.ascontiguousarray(coefs[name], dtype="<f8").tofile(fh)
+        np.array([erro], dtype="<f8").tofile(fh)
+        for k in ("ux", "uy", "uz", "wx", "wy", "wz"):
+            np.ascontiguousarray(F[k], dtype="<c16").tofile(fh)
+        np.ascontiguousarray(d2, dtype="<c16").tofile(fh)
+        np.ascontiguousarray(b, dtype="<f8").tofile(fh)
+    r = subprocess.run([LOOP_EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    shp = (G.nmodes, G.ny, G.nx)
+    out = np.fromfile(tmp_path / "loop_out.bin", dtype="<c16").reshape((6,) + shp)
+    p = oracle.make_params(G.nx, G.ny, G.nmodes, G.levels, G.dx, G.dy0, G.stf, G.alpha, G.beta,
+                           coefs=coefs)
+    ref, cyc, _ = oracle.loop(p, erro, b, d2, F, order=1)
+    for i, k in enumerate(("ux", "uy", "uz", "wx", "wy", "wz")):
+        assert rel(out[i], ref[k]) < 1e-12, (k, rel(out[i], ref[k]))
     lines = [l.split() for l in r.stdout.strip().splitlines() if l.strip()]
     assert [int(l[1]) for l in lines[-G.nmodes:]] == cyc.tolist()

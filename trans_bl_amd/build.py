@@ -85,6 +85,11 @@ def build_fortran(verbose: bool = True) -> str:
         [FC, os.path.join(FBUILD, "rk_stage.o"), os.path.join(FBUILD, "tbl_dropin.o"),
          os.path.join(FBUILD, "tbl_iface.o"), "-L", HERE, "-ltrans_bl_amd",
          "-Wl,-rpath,$ORIGIN/../..", "-o", os.path.join(FBUILD, "rk_stage")],
+        [FC, "-O2", "-ffixed-form", "-I", inc, "-J", FBUILD, "-c",
+         os.path.join(inc, "rk_loop.f"), "-o", os.path.join(FBUILD, "rk_loop.o")],
+        [FC, os.path.join(FBUILD, "rk_loop.o"), os.path.join(FBUILD, "tbl_dropin.o"),
+         os.path.join(FBUILD, "tbl_iface.o"), "-L", HERE, "-ltrans_bl_amd",
+         "-Wl,-rpath,$ORIGIN/../..", "-o", os.path.join(FBUILD, "rk_loop")],
     ]
     for cmd in steps:
         r = subprocess.run(cmd, capture_output=True, text=True)

@@ -4,8 +4,9 @@ c     derivative subroutines, running on an MI355X through libtrans_bl_amd.
 c
 c     Same names and argument lists as the reference (poi_uy,
 c     poisson_par.f:6; derparx, derparxx, dery, deryy, deryfv,
-c     derivs_par.f:80-177), same hidden inputs: the PARAMETERs of par.for,
-c     COMMON /var/ (uy, wx) and COMMON /coefs/.  Compile this file instead
+c     derivs_par.f:80-177; loop, main_par.f:1762-1785), same hidden
+c     inputs: the PARAMETERs of par.for, COMMON /var/, /coefs/, /bd/
+c     (bdfc) and /derw/ (d2uydx2).  Compile this file instead
 c     of poisson_par.f's poi_uy and derivs_par.f's derivative routines,
 c     with the run's own par.for and comm.* on the include path, and link
 c     tbl_iface.o and libtrans_bl_amd.so.  One rank (np = 1).
@@ -181,6 +182,39 @@ cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
       call tbl_setup
       rc = tbl_deryfv(tbl_ctx_handle, fc, ddy)
       if (rc .ne. 0) call tbl_fail('deryfv', rc)
+
+      return
+      end
+
+cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
+      subroutine loop(erro)
+
+      ! the RK-stage update (main_par.f:1762-1785): filter_trid when
+      ! erro < 1e-5, bzone, outuy, poi_uy (to erro), poi_ux, poi_uz,
+      ! wx_wall, wz_wall, bzonew -- all on the GPU in one call; prints
+      ! poi_uy's (k, V-cycles, residual) lines.  th (my_form 2) is
+      ! handed over only for that form.
+      use tbl_iface
+      implicit none
+      include 'par.for'
+      include 'comm.var'
+      real*8 erro, bdfc(2,imax)
+      complex*16 d2uydx2(ptsx,kfour)
+      common/bd/ bdfc
+      common/derw/ d2uydx2
+      integer k
+      integer(c_int) rc, cycles(kfour)
+      real(c_double) residual(kfour)
+      type(tbl_solve_stats_t) stats
+
+      call tbl_setup
+      rc = tbl_loop(tbl_ctx_handle, erro, bdfc, d2uydx2, ux, uy, uz,
+     &              wx, wy, wz, tbl_th_ptr(my_form .eq. 2, th),
+     &              tbl_opts, cycles, residual, stats)
+      if (rc .ne. 0) call tbl_fail('loop', rc)
+      do k = 1, kfour
+        write(*,*) k, cycles(k), residual(k)
+      end do
 
       return
       end

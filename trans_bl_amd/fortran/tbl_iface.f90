@@ -111,6 +111,22 @@ module tbl_iface
       import :: c_ptr
       type(c_ptr) :: p
     end function
+    function tbl_loop(ctx, erro, bdfc, d2uydx2, ux, uy, uz, wx, wy, wz, th, opts, cycles, &
+        residual, stats) bind(C, name="tbl_loop") result(rc)
+      import :: c_int, c_ptr, c_double, c_double_complex, tbl_solve_opts_t, tbl_solve_stats_t
+      type(c_ptr), value :: ctx
+      real(c_double), value :: erro
+      real(c_double), intent(in) :: bdfc(*)
+      complex(c_double_complex), intent(in) :: d2uydx2(*)
+      complex(c_double_complex), intent(inout) :: ux(*), uy(*), uz(*), wx(*), wy(*), wz(*)
+      type(c_ptr), value :: th
+      type(tbl_solve_opts_t), intent(in) :: opts
+      integer(c_int), intent(out) :: cycles(*)
+      real(c_double), intent(out) :: residual(*)
+      type(tbl_solve_stats_t), intent(out) :: stats
+      integer(c_int) :: rc
+    end function
+
   end interface
 
   type(c_ptr), save :: tbl_ctx_handle = c_null_ptr
@@ -141,5 +157,16 @@ contains
     write(*,*) 'trans_bl_amd: ', what, ' failed, status ', rc, ' ', trim(text)
     stop 1
   end subroutine
+
+
+  ! address of the th field for tbl_loop: my_form 2 hands it over, any other
+  ! form passes a null pointer (th untouched)
+  function tbl_th_ptr(with_th, th) result(p)
+    logical, intent(in) :: with_th
+    complex(c_double_complex), target, intent(in) :: th(*)
+    type(c_ptr) :: p
+    p = c_null_ptr
+    if (with_th) p = c_loc(th(1))
+  end function
 
 end module tbl_iface
