@@ -36,6 +36,12 @@ class Params(C.Structure):
     ]
 
 
+class Base(C.Structure):
+    """or_base: COMMON /blas/ uxb, uyb, wzb and /vc/ varg, dvargdx."""
+    _fields_ = [("uxb", C.c_void_p), ("uyb", C.c_void_p), ("wzb", C.c_void_p),
+                ("varg", C.c_void_p), ("dvargdx", C.c_void_p)]
+
+
 class Metrics(C.Structure):
     """or_metrics: create_ctes (main_par.f:1828-1892) per level."""
     _fields_ = [
@@ -99,6 +105,18 @@ def lib():
         L.or_bdfc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, vp]
         L.or_loop.argtypes = [P, C.c_double, vp, vp] + [vp] * 7 + [C.c_int, vp, vp]
         L.or_loop.restype = C.c_int
+        B = C.POINTER(Base)
+        L.or_f_to_p.argtypes = [P, C.c_int, vp, vp]
+        L.or_f_to_p.restype = C.c_int
+        L.or_p_to_f.argtypes = [P, C.c_int, vp, vp]
+        L.or_p_to_f.restype = C.c_int
+        L.or_terms.argtypes = [P, C.c_int, B] + [vp] * 10
+        L.or_terms.restype = C.c_int
+        L.or_drv_gv.argtypes = [P, C.c_double, C.c_int, B] + [vp] * 9
+        L.or_drv_gv.restype = C.c_int
+        L.or_rk4_step.argtypes = [P, C.c_double, C.c_double, C.c_int, B, vp, vp, vp, vp,
+                                  C.c_int, C.c_int, vp, C.c_int, vp]
+        L.or_rk4_step.restype = C.c_int
         _lib = L
     return _lib
 
@@ -413,3 +431,73 @@ def loop(p, erro, bdfc_, d2uydx2, fields, order=0):
     _rc(lib().or_loop(C.byref(p), erro, _vp(b), _vp(d2), *[_vp(f.get(k)) for k in FIELDS],
                       order, _vp(cyc), _vp(res)), "or_loop")
     return f, cyc, res
+
+
+
+# ---- right-hand side and RK4 step (tbl_oracle_rhs.c) -----------------------
+BASE_FIELDS = ("uxb", "uyb", "wzb", "varg", "dvargdx")
+
+
+def _base(p, base):
+    arrs = [np.ascontiguousarray(base[k], dtype=np.float64) for k in BASE_FIELDS]
+    for a in arrs:
+        assert a.shape == (p.ny, p.nx)
+    return Base(*[a.ctypes.data_as(C.c_void_p) for a in arrs]), arrs
+
+
+def f_to_p(p, kphys, f):
+    f = _cf(f, _field(p))
+    out = np.zeros((kphys, p.ny, p.nx))
+    _rc(lib().or_f_to_p(C.byref(p), kphys, _vp(f), _vp(out)), "or_f_to_p")
+    return out
+
+
+def p_to_f(p, kphys, phys):
+    phys = np.ascontiguousarray(phys, dtype=np.float64)
+    out = np.zeros(_field(p), np.complex128)
+    _rc(lib().or_p_to_f(C.byref(p), kphys, _vp(phys), _vp(out)), "or_p_to_f")
+    return out
+
+
+def terms(p, kphys, base, F):
+    """lterms_gv (kphys 0) or nlterms_gv: returns (a, b, c, d)."""
+    shp = _field(p)
+    b, keep = _base(p, base)
+    f = [_cf(F[k], shp) for k in ("ux", "uy", "uz", "wx", "wy", "wz")]
+    out = [np.zeros(shp, np.complex128) for _ in range(4)]
+    _rc(lib().or_terms(C.byref(p), kphys, C.byref(b), *[_vp(a) for a in f + out]), "or_terms")
+    return tuple(out)
+
+
+def drv_gv(p, Re, kphys, base, F):
+    """drv_gv (main_par.f:1339-1403): returns (dvx, dvy, dvz); row 1 zero."""
+    shp = _field(p)
+    b, keep = _base(p, base)
+    f = [_cf(F[k], shp) for k in ("ux", "uy", "uz", "wx", "wy", "wz")]
+    out = [np.zeros(shp, np.complex128) for _ in range(3)]
+    _rc(lib().or_drv_gv(C.byref(p), Re, kphys, C.byref(b), *[_vp(a) for a in f + out]),
+        "or_drv_gv")
+    return tuple(out)
+
+
+def rk4_step(p, Re, dt, kphys, base, bdfc_, d2uydx2, F, pert=None, order=0):
+    """One RK4 time step (main_par.f:266-349).  pert = (pert_uy, pert_d2,
+    i0, i1) replaces uy(i,1,k) and d2uydx2(i,k), i0 <= i <= i1 (1-based),
+    k >= 2, before the loops of stages 1 and 3.  Returns (fields dict,
+    d2uydx2, cycles (4, nk))."""
+    shp = _field(p)
+    b, keep = _base(p, base)
+    f = {k: _cf(F[k], shp) for k in FIELDS if F.get(k) is not None}
+    ptrs = (C.c_void_p * 7)(*[_vp(f.get(k)) for k in FIELDS])
+    d2 = _cf(d2uydx2, (p.nk, p.nx))
+    bd = np.ascontiguousarray(bdfc_, dtype=np.float64)
+    cyc = np.zeros((4, p.nk), np.int32)
+    if pert is None:
+        pu = pd = None
+        i0 = i1 = 0
+    else:
+        pu, pd = _cf(pert[0], (p.nk, p.nx)), _cf(pert[1], (p.nk, p.nx))
+        i0, i1 = pert[2], pert[3]
+    _rc(lib().or_rk4_step(C.byref(p), Re, dt, kphys, C.byref(b), _vp(bd), _vp(d2), _vp(pu),
+                          _vp(pd), i0, i1, ptrs, order, _vp(cyc)), "or_rk4_step")
+    return f, d2, cyc

@@ -5,8 +5,9 @@ The recipe compiles the reference's sources where they lie
 amdflang and the image's MPICH, against a ``par.for`` configured for one
 workload -- the reference's own per-run configuration step (every size is a
 PARAMETER of par.for) -- and links them with ``oracle/ref_harness.f``
-(poi_uy and the derivatives) and ``oracle/ref_loop.f`` (the other routines
-of ``loop``).  The
+(poi_uy and the derivatives), ``oracle/ref_loop.f`` (the other routines
+of ``loop``) and ``oracle/ref_rhs.f`` (drv_gv, nlterms_gv and the RK4 step).
+The
 coefficient file comes from the reference's own generator
 (pre_processing/coef_generation.f90 and its modules, with the constants
 module written by the reference's convert.pl from the same par.for), run
@@ -100,12 +101,17 @@ def loop_exe_path(name: str) -> str:
     return os.path.join(OUT, name, "ref_loop")
 
 
+def rhs_exe_path(name: str) -> str:
+    return os.path.join(OUT, name, "ref_rhs")
+
+
 def coefs_path(name: str) -> str:
     return os.path.join(OUT, name, "coefs.bin")
 
 
 def available(name: str) -> bool:
-    return all(os.path.exists(f) for f in (exe_path(name), loop_exe_path(name), coefs_path(name)))
+    return all(os.path.exists(f) for f in (exe_path(name), loop_exe_path(name),
+                                            rhs_exe_path(name), coefs_path(name)))
 
 
 def buildable() -> bool:
@@ -141,7 +147,9 @@ def build(name: str, verbose: bool = True) -> str:
         # after the Fortran runtime's own data, which is built for the small
         # code model; no relaxation of the GOT loads of those blocks
         _run([FC, *flags, "-c", os.path.join(HERE, "ref_loop.f"), "-o", "loop.o"], tmp)
-        for main, out in (("harness.o", exe), ("loop.o", loop_exe_path(name))):
+        _run([FC, *flags, "-c", os.path.join(HERE, "ref_rhs.f"), "-o", "rhs.o"], tmp)
+        for main, out in (("harness.o", exe), ("loop.o", loop_exe_path(name)),
+                          ("rhs.o", rhs_exe_path(name))):
             _run([FC, "-fuse-ld=bfd", "-mcmodel=medium", "-o", out, main, *objs, "-L", MPI_LIB,
                   "-lmpifort", "-lmpi", f"-Wl,-rpath,{MPI_LIB}", "-Wl,--no-relax"], tmp)
         # coefficient file: the reference's generator for this par.for
@@ -342,6 +350,62 @@ def run_loop(name: str, fields: dict, d2uydx2, bdfc, erro: float, what: str = "l
                 res[int(t[0]) - 1] = float(t[2].replace("D", "E"))
         out["cycles"] = cycles
         out["residual"] = res
+        return out
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+BASE_FIELDS = ("uxb", "uyb", "wzb", "varg", "dvargdx")
+
+
+def run_rhs(name: str, fields: dict, base: dict, d2uydx2, bdfc, fcx2, fcllx2, dt: float,
+            what: str, t: int = 1, timeout: float = 600.0):
+    """drv_gv ("drv": dvx, dvy, dvz), nlterms_gv ("nlterms": a, b, c, d) or
+    one RK4 time step at step t ("rk4": ux .. wz and d2uydx2) by the
+    reference's code (ref_rhs.f).  fields: complex (nk, ny, nx) ux .. wz;
+    base: real (ny, nx) uxb, uyb, wzb, varg, dvargdx; fcx2, fcllx2: the
+    disturbance shapes of gv_pert (length i2).  Also returns "params" (Re,
+    i1, i2, stpp of the configured par.for) and poi_uy's cycle lines."""
+    if not available(name):
+        raise RuntimeError(f"reference driver {name} is not built")
+    nx, ny, nk = CONFIGS[name][0], CONFIGS[name][1], CONFIGS[name][2]
+    shape = (nk, ny, nx)
+    tmp = tempfile.mkdtemp(prefix="tbl_refrhs_")
+    try:
+        fin = os.path.join(tmp, "in.bin")
+        fout = os.path.join(tmp, "out.bin")
+        with open(fin, "wb") as f:
+            for k in ("ux", "uy", "uz", "wx", "wy", "wz"):
+                f.write(np.ascontiguousarray(fields[k], dtype="<c16").tobytes())
+            for k in BASE_FIELDS:
+                f.write(np.ascontiguousarray(base[k], dtype="<f8").tobytes())
+            f.write(np.ascontiguousarray(d2uydx2, dtype="<c16").tobytes())
+            f.write(np.ascontiguousarray(bdfc, dtype="<f8").tobytes())
+            f.write(np.ascontiguousarray(fcx2, dtype="<f8").tobytes())
+            f.write(np.ascontiguousarray(fcllx2, dtype="<f8").tobytes())
+            f.write(np.array([dt], dtype="<f8").tobytes())
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        r = subprocess.run(["bash", "-c", 'ulimit -s unlimited && exec "$@"', "ref",
+                            rhs_exe_path(name), coefs_path(name), fin, fout, what, str(t)],
+                           capture_output=True, text=True, timeout=timeout, env=env, cwd=tmp)
+        if r.returncode != 0:
+            raise RuntimeError(f"ref_rhs failed:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        b = np.fromfile(fout, dtype="<c16")
+        n3 = nk * ny * nx
+        names = {"drv": ("dvx", "dvy", "dvz"), "nlterms": ("a", "b", "c", "d"),
+                 "rk4": ("ux", "uy", "uz", "wx", "wy", "wz")}[what]
+        out = {k: b[q * n3:(q + 1) * n3].reshape(shape).copy() for q, k in enumerate(names)}
+        if what == "rk4":
+            out["d2uydx2"] = b[len(names) * n3:].reshape(nk, nx).copy()
+        cycles = []
+        for line in r.stdout.splitlines():
+            tk = line.split()
+            if tk and tk[0] == "params":
+                out["params"] = {"Re": float(tk[1].replace("D", "E")), "i1": int(tk[2]),
+                                 "i2": int(tk[3]), "stpp": int(tk[4])}
+            elif len(tk) == 3 and tk[0].isdigit():
+                cycles.append(int(tk[1]))
+        out["cycles"] = np.array(cycles, np.int32)
         return out
     finally:
         shutil.rmtree(tmp, ignore_errors=True)

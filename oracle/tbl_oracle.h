@@ -134,4 +134,27 @@ int or_loop(const or_params *p, double erro, const double *bdfc, const or_cplx *
             or_cplx *ux, or_cplx *uy, or_cplx *uz, or_cplx *wx, or_cplx *wy, or_cplx *wz,
             or_cplx *th, int order, int *cycles, double *res);
 
+/* ---- right-hand side and RK4 step (tbl_oracle_rhs.c) --------------------- */
+/* base flow and curvature: COMMON /blas/ uxb, uyb, wzb and /vc/ varg,
+   dvargdx, each real (nx, ny) */
+typedef struct or_base {
+    const double *uxb, *uyb, *wzb, *varg, *dvargdx;
+} or_base;
+/* f_to_p / p_to_f (nlterms_par.f:303-428): phys is (nx, ny, kphys) reals */
+int or_f_to_p(const or_params *p, int kphys, const or_cplx *f, double *phys);
+int or_p_to_f(const or_params *p, int kphys, const double *phys, or_cplx *f);
+/* kphys 0: lterms_gv (nlterms_par.f:89-112); else nlterms_gv (115-180) */
+int or_terms(const or_params *p, int kphys, const or_base *b, const or_cplx *ux,
+             const or_cplx *uy, const or_cplx *uz, const or_cplx *wx, const or_cplx *wy,
+             const or_cplx *wz, or_cplx *a, or_cplx *bb, or_cplx *c, or_cplx *d);
+/* drv_gv (main_par.f:1339-1403): rows j >= 2 of dvx, dvy, dvz written */
+int or_drv_gv(const or_params *p, double Re, int kphys, const or_base *b, const or_cplx *ux,
+              const or_cplx *uy, const or_cplx *uz, const or_cplx *wx, const or_cplx *wy,
+              const or_cplx *wz, or_cplx *dvx, or_cplx *dvy, or_cplx *dvz);
+/* one RK4 time step (main_par.f:266-349); cycles: 4 x nk (per stage) */
+int or_rk4_step(const or_params *p, double Re, double dt, int kphys, const or_base *b,
+                const double *bdfc, or_cplx *d2uydx2, const or_cplx *pert_uy,
+                const or_cplx *pert_d2, int pert_i0, int pert_i1, or_cplx *const *f, int order,
+                int *cycles);
+
 #endif

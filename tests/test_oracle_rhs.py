@@ -1,0 +1,188 @@
+"""The oracle's restatement of the right-hand side and the RK4 step of the
+reference's Goertler case: lterms_gv / nlterms_gv with the z transforms
+f_to_p, p_to_f, four1 (nlterms_par.f), drv_gv (main_par.f:1339-1403) and
+the four-stage update around `loop` (main_par.f:266-349).
+
+Pinned against the reference's own routines (tests/golden/ref_rhs_vectors.npz,
+written by tests/golden/make_ref_vectors.py --rhs; inputs regenerated here),
+and by what the mathematics fixes: the z transforms are the real inverse and
+forward DFT of the packed spectrum (mode 0 real, the odd-even mode in its
+imaginary part), a product of single modes gives the expected two modes, and
+drv_gv with no base flow is the viscous term (Laplacian of the vorticity over
+Re), exact on quadratics.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import reference as R
+from trans_bl_amd import workload as wl
+from conftest import rel
+
+GOLD = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                            "ref_rhs_vectors.npz"))
+LOOPGOLD = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                "ref_loop_vectors.npz"))
+SUB = {"129x33x4": (slice(None), slice(None, None, 4), slice(None, None, 8)),
+       "385x97x8": (slice(None), slice(None, None, 12), slice(None, None, 32))}
+DT, T = 1e-3, 3
+
+
+def grid_of(name):
+    nx, ny, nk, msh, dx, dy, stf, alpha, beta = R.CONFIGS[name]
+    return wl.Grid(nx, ny, nk, msh, dx, dy, stf, alpha, beta)
+
+
+def params(name):
+    g = grid_of(name)
+    v = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "ref_vectors.npz"))
+    keys = ("fp_fd_e", "sp_fd", "cp_fd", "pp_fd", "lp_fd", "fp_sd", "sp_sd", "cp_sd",
+            "pp_sd", "lp_sd", "sp_poi", "cp_poi", "pp_poi", "lp_poi")
+    c = {k: v[f"{name}/coefs/{k}"] for k in keys}
+    for k in ("w_at_w", "dwydy"):
+        c[k] = LOOPGOLD[f"{name}/coefs/{k}"]
+    return g, oracle.make_params(g.nx, g.ny, g.nmodes, g.levels, g.dx, g.dy0, g.stf, g.alpha,
+                                 g.beta, coefs=c)
+
+
+def rhs_inputs(g):
+    """Same seeds and formulas as tests/golden/make_ref_vectors.py:rhs_inputs."""
+    F = {k: wl.wavepacket_rhs(g, tag="rhsvec-" + k) * (0.1 if k[0] == "w" else 1.0)
+         for k in ("ux", "uy", "uz", "wx", "wy", "wz")}
+    X, Y = np.meshgrid(g.x(), g.y())
+    d = 0.002 * np.sqrt(X + 1)
+    base = {"uxb": np.tanh(Y / d), "uyb": 0.01 * (1 - np.exp(-Y / d)),
+            "wzb": -(1 - np.tanh(Y / d) ** 2) / d, "varg": 0.05 + 0.01 * X,
+            "dvargdx": 0.01 + 0 * X}
+    rng = np.random.default_rng(1)
+    d2 = rng.standard_normal((g.nmodes, g.nx)) + 1j * rng.standard_normal((g.nmodes, g.nx))
+    b = oracle.bdfc(g.nx, 10, int(g.nx * 0.785), int(g.nx * 0.88))
+    return F, base, d2, b, rng.standard_normal(36), rng.standard_normal(36)
+
+
+def gv_pert_arrays(g, t, i1, i2, stpp, fcx2, fcll, fac_y=1.0):
+    """gv_pert (main_par.f:1577-1610): uy(i,1,k) = A fcx2(i), d2uydx2(i,k) =
+    A fcllx2(i) for i1 < i < i2, k >= 2, A = 1e-6 sqrt(fac_y) ramped in over
+    the first stpp steps."""
+    A = 1e-6 * np.sqrt(fac_y)
+    if t < stpp:
+        ep = t / stpp
+        A = A * ((6 * ep - 15) * ep + 10) * ep**3
+    pu = np.zeros((g.nmodes, g.nx), np.complex128)
+    pd = np.zeros((g.nmodes, g.nx), np.complex128)
+    pu[:, i1:i2 - 1] = A * fcx2[i1:i2 - 1]
+    pd[:, i1:i2 - 1] = A * fcll[i1:i2 - 1]
+    return pu, pd, i1 + 1, i2 - 1
+
+
+@pytest.mark.parametrize("name", ["129x33x4", "385x97x8"])
+def test_drv_gv_and_nlterms_match_reference(name):
+    g, p = params(name)
+    F, base, d2, b, fcx2, fcll = rhs_inputs(g)
+    Re = float(GOLD[f"{name}/params"][0])
+    sub = SUB[name]
+    dv = oracle.drv_gv(p, Re, 0, base, F)
+    for a, k in zip(dv, ("dvx", "dvy", "dvz")):
+        assert rel(a[sub][:, 1:], GOLD[f"{name}/drv/{k}"]) < 1e-13, k
+    nl = oracle.terms(p, 2 * g.nmodes, base, F)
+    for a, k in zip(nl, "abcd"):
+        assert rel(a[sub], GOLD[f"{name}/nlterms/{k}"]) < 1e-13, k
+
+
+def test_rk4_step_matches_reference():
+    name = "129x33x4"
+    g, p = params(name)
+    F, base, d2, b, fcx2, fcll = rhs_inputs(g)
+    Re, i1, i2, stpp = GOLD[f"{name}/params"]
+    pert = gv_pert_arrays(g, T, int(i1), int(i2), int(stpp), fcx2, fcll)
+    f, d2o, cyc = oracle.rk4_step(p, float(Re), DT, 0, base, b, d2, F, pert=pert, order=0)
+    assert np.array_equal(cyc.ravel(), GOLD[f"{name}/rk4/cycles"])
+    assert rel(d2o, GOLD[f"{name}/rk4/d2uydx2"]) == 0.0
+    for k in ("ux", "uy", "uz", "wx", "wy", "wz"):
+        assert rel(f[k][SUB[name]], GOLD[f"{name}/rk4/{k}"]) < 1e-12, k
+
+
+# ---- what the mathematics fixes -------------------------------------------------
+def _tiny(nk=8):
+    g = wl.Grid(9, 8, nk, 1, 0.1, 1e-3, 1.0, 0.0, 0.5)
+    return g, oracle.make_params(g.nx, g.ny, g.nmodes, g.levels, g.dx, g.dy0, g.stf, g.alpha,
+                                 g.beta)
+
+
+@pytest.mark.parametrize("N", [16, 32])
+def test_z_transforms_are_the_real_dft(N):
+    """f_to_p: p_n = Re c_0 + (Im c_0 / 2)(-1)^n + sum_m Re(c_m e^{-2 pi i m n / N});
+    p_to_f inverts it (mode 0 comes back real)."""
+    g, p = _tiny()
+    rng = np.random.default_rng(N)
+    c = rng.standard_normal((g.nmodes, g.ny, g.nx)) + 1j * rng.standard_normal((g.nmodes, g.ny,
+                                                                                 g.nx))
+    ph = oracle.f_to_p(p, N, c)
+    n = np.arange(N)
+    m = np.arange(1, g.nmodes)
+    E = np.exp(-2j * np.pi * np.outer(n, m) / N)                   # (N, K-1)
+    want = (c[0].real[None] + 0.5 * c[0].imag[None] * ((-1.0) ** n)[:, None, None]
+            + np.einsum("nm,mji->nji", E, c[1:]).real)
+    assert rel(ph, want) < 1e-14
+    c[0] = c[0].real
+    back = oracle.p_to_f(p, N, oracle.f_to_p(p, N, c))
+    assert rel(back, c) < 1e-14
+
+
+def test_nlterms_product_of_single_modes():
+    """u = cos-wave in mode 1 only: ux = 1 in mode 1 gives ux(z) = cos(z)
+    and d = ux^2 = 1/2 + cos(2z)/2: mode 0 gets 1/2, mode 2 gets 1/2, with
+    no base flow."""
+    g, p = _tiny()
+    shp = (g.nmodes, g.ny, g.nx)
+    F = {k: np.zeros(shp, np.complex128) for k in ("ux", "uy", "uz", "wx", "wy", "wz")}
+    F["ux"][1] = 1.0
+    zero = np.zeros((g.ny, g.nx))
+    base = {k: zero for k in ("uxb", "uyb", "wzb", "varg", "dvargdx")}
+    a, b, c, d = oracle.terms(p, 2 * g.nmodes, base, F)
+    want = np.zeros(shp, np.complex128)
+    want[0] = 0.5
+    want[2] = 0.5
+    assert rel(d, want) < 1e-14
+    assert np.all(a == 0) and np.all(b == 0) and np.all(c == 0)
+
+
+def test_drv_gv_viscous_term_on_quadratics():
+    """No base flow and no curvature: dv = (w_xx + fac_y w_yy + k2b2 w) / Re,
+    exact for w = x^2 / 100 + y^2 (compact schemes are exact on quadratics;
+    the x term is kept small because d2/dy2 of it, zero in exact
+    arithmetic, carries its rounding times 1/dy^2)."""
+    g = wl.Grid(65, 33, 3, 3, 0.05, 1e-3, 1.03, 0.0, 0.7)
+    p = oracle.make_params(g.nx, g.ny, g.nmodes, g.levels, g.dx, g.dy0, g.stf, g.alpha, g.beta)
+    X, Y = np.meshgrid(g.x(), g.y())
+    q = 0.01 * X**2 + Y**2
+    shp = (g.nmodes, g.ny, g.nx)
+    F = {k: np.zeros(shp, np.complex128) for k in ("ux", "uy", "uz")}
+    for k, s in (("wx", 1.0), ("wy", 2.0), ("wz", 1j)):
+        F[k] = np.broadcast_to(s * q, shp).astype(np.complex128)
+    zero = np.zeros((g.ny, g.nx))
+    base = {k: zero for k in ("uxb", "uyb", "wzb", "varg", "dvargdx")}
+    Re = 250.0
+    dvx, dvy, dvz = oracle.drv_gv(p, Re, 0, base, F)
+    for dv, s in ((dvx, 1.0), (dvy, 2.0), (dvz, 1j)):
+        for k in range(g.nmodes):
+            k2b2 = -(k * g.beta) ** 2
+            want = s * (0.02 + 2.0 * g.fac_y + k2b2 * q) / Re
+            assert rel(dv[k, 1:], want[1:]) < 1e-9, k
+
+
+@pytest.mark.skipif(not R.available("129x33x4"), reason="reference drivers not built")
+def test_rk4_step_against_reference_driver():
+    name = "129x33x4"
+    g, p = params(name)
+    F, base, d2, b, fcx2, fcll = rhs_inputs(g)
+    r = R.run_rhs(name, F, base, d2, b, fcx2, fcll, DT, "rk4", t=T)
+    pr = r["params"]
+    pert = gv_pert_arrays(g, T, pr["i1"], pr["i2"], pr["stpp"], fcx2, fcll)
+    f, d2o, cyc = oracle.rk4_step(p, pr["Re"], DT, 0, base, b, d2, F, pert=pert, order=0)
+    assert np.array_equal(cyc.ravel(), r["cycles"])
+    for k in ("ux", "uy", "uz", "wx", "wy", "wz"):
+        assert rel(f[k], r[k]) < 1e-12, k
