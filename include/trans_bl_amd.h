@@ -258,6 +258,39 @@ int tbl_loop(tbl_ctx *ctx, double erro, const double *bdfc, const double *d2uydx
    (main_par.f:738-755): 1-based i0, i3, i4 from par.for. */
 int tbl_bdfc_generate(int nx, int i0, int i3, int i4, double *bdfc);
 
+/* ---- right-hand side and RK4 step (Goertler case, my_form 1) ---------------
+   Base flow and curvature term: COMMON /blas/ uxb, uyb, wzb and /vc/ varg,
+   dvargdx, each a real (nx, ny) device array. */
+typedef struct tbl_base {
+    const double *uxb, *uyb, *wzb, *varg, *dvargdx;
+} tbl_base;
+
+/* kphys 0: lterms_gv (nlterms_par.f:89-112, the reference's active call);
+   otherwise nlterms_gv (115-180) with kphys points in z (a power of two,
+   >= 2 nmodes): a, b, c, d out. */
+int tbl_terms_dev(tbl_ctx *ctx, int kphys, const tbl_base *base, const double *d_ux,
+                  const double *d_uy, const double *d_uz, const double *d_wx,
+                  const double *d_wy, const double *d_wz, double *d_a, double *d_b, double *d_c,
+                  double *d_d);
+/* drv_gv (main_par.f:1339-1403): dvx, dvy, dvz on rows 2..ny (row 1 not
+   written); Re the Reynolds number of par.for. */
+int tbl_drv_gv_dev(tbl_ctx *ctx, double Re, int kphys, const tbl_base *base,
+                   const double *d_ux, const double *d_uy, const double *d_uz,
+                   const double *d_wx, const double *d_wy, const double *d_wz, double *d_dvx,
+                   double *d_dvy, double *d_dvz);
+/* One RK4 time step (main_par.f:266-349): four drv_gv + update stages
+   (rows 2..ny-1, columns 3..nx), each followed by loop (erro 1e-5, 1e-5,
+   1e-5, 1e-6).  The disturbance of gv_pert (main_par.f:1577-1610) before
+   the loops of stages 1 and 3: for pert_i0 <= i <= pert_i1 (1-based) and
+   modes >= 2, uy(i,1,k) = pert_uy(i,k) and d2uydx2(i,k) = pert_d2(i,k)
+   ((nx, K) arrays; NULL pert_uy: none).  cycles / residual: 4 x nmodes
+   (per stage), or NULL. */
+int tbl_rk4_step_dev(tbl_ctx *ctx, double Re, double dt, int kphys, const tbl_base *base,
+                     const double *d_bdfc, double *d_d2uydx2, const double *d_pert_uy,
+                     const double *d_pert_d2, int pert_i0, int pert_i1, double *d_ux,
+                     double *d_uy, double *d_uz, double *d_wx, double *d_wy, double *d_wz,
+                     double *d_th, const tbl_solve_opts *opts, int *cycles, double *residual);
+
 /* Line-solve shape of one multigrid level's smoother: `lanes` (16, 32 or
    64) lanes of a wavefront share one y-line; a negative value selects the
    same shape with LDS-staged, coalesced loads and stores; 2000 + lanes

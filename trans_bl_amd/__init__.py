@@ -33,6 +33,12 @@ class TblError(RuntimeError):
     pass
 
 
+class CBase(C.Structure):
+    """tbl_base: device pointers of uxb, uyb, wzb, varg, dvargdx."""
+    _fields_ = [("uxb", C.c_void_p), ("uyb", C.c_void_p), ("wzb", C.c_void_p),
+                ("varg", C.c_void_p), ("dvargdx", C.c_void_p)]
+
+
 class CGrid(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nmodes", C.c_int), ("levels", C.c_int),
                 ("dx", C.c_double), ("dy0", C.c_double), ("stf", C.c_double),
@@ -79,7 +85,8 @@ EXPORTS = [
     "tbl_batch_stage", "tbl_batch_results", "tbl_checkpoint_write", "tbl_checkpoint_read",
     "tbl_baseflow_write", "tbl_baseflow_read", "tbl_poi_ux_dev", "tbl_poi_uz_dev",
     "tbl_wx_wall_dev", "tbl_wz_wall_dev", "tbl_bzone_dev", "tbl_filter_trid_dev",
-    "tbl_outuy_dev", "tbl_loop_dev", "tbl_loop", "tbl_bdfc_generate",
+    "tbl_outuy_dev", "tbl_loop_dev", "tbl_loop", "tbl_bdfc_generate", "tbl_terms_dev",
+    "tbl_drv_gv_dev", "tbl_rk4_step_dev",
 ]
 
 _lib = None
@@ -127,6 +134,10 @@ def lib():
         "tbl_loop": ([vp, dp, vp, vp] + [vp] * 7 + [C.POINTER(SolveOpts), vp, vp,
                                                  C.POINTER(SolveStats)], C.c_int),
         "tbl_bdfc_generate": ([ip, ip, ip, ip, vp], C.c_int),
+        "tbl_terms_dev": ([vp, ip, C.POINTER(CBase)] + [vp] * 10, C.c_int),
+        "tbl_drv_gv_dev": ([vp, dp, ip, C.POINTER(CBase)] + [vp] * 9, C.c_int),
+        "tbl_rk4_step_dev": ([vp, dp, dp, ip, C.POINTER(CBase), vp, vp, vp, vp, ip, ip]
+                             + [vp] * 7 + [C.POINTER(SolveOpts), vp, vp], C.c_int),
     }
     for n in ("tbl_derparx", "tbl_derparxx", "tbl_dery", "tbl_deryy", "tbl_deryfv"):
         types[n] = ([vp, vp, vp], C.c_int)
@@ -450,6 +461,37 @@ class Context:
         _check(lib().tbl_loop(self.h, erro, _ptr(b), _ptr(d2), *ptrs, C.byref(o), _ptr(cyc),
                               _ptr(res), C.byref(st)), "tbl_loop")
         return cyc, res, st.as_dict()
+
+    # ---- right-hand side and RK4 step (device pointers) ----
+    def terms_dev(self, kphys, base, ux, uy, uz, wx, wy, wz, a, b, c, d):
+        """base: dict of device pointers uxb, uyb, wzb, varg, dvargdx."""
+        cb = CBase(*[C.c_void_p(base[k]) for k in ("uxb", "uyb", "wzb", "varg", "dvargdx")])
+        _check(lib().tbl_terms_dev(self.h, kphys, C.byref(cb),
+                                   *[C.c_void_p(x) for x in (ux, uy, uz, wx, wy, wz, a, b, c, d)]),
+               "tbl_terms_dev")
+
+    def drv_gv_dev(self, Re, kphys, base, ux, uy, uz, wx, wy, wz, dvx, dvy, dvz):
+        cb = CBase(*[C.c_void_p(base[k]) for k in ("uxb", "uyb", "wzb", "varg", "dvargdx")])
+        _check(lib().tbl_drv_gv_dev(self.h, Re, kphys, C.byref(cb),
+                                    *[C.c_void_p(x) for x in (ux, uy, uz, wx, wy, wz, dvx, dvy,
+                                                              dvz)]), "tbl_drv_gv_dev")
+
+    def rk4_step_dev(self, Re, dt, kphys, base, bdfc, d2uydx2, fields, pert=None, **kw):
+        """One RK4 time step on device fields (dict of pointers ux .. wz, th
+        optional); pert = (pert_uy_ptr, pert_d2_ptr, i0, i1) or None.
+        Returns cycles (4, nmodes) and residuals (4, nmodes)."""
+        cb = CBase(*[C.c_void_p(base[k]) for k in ("uxb", "uyb", "wzb", "varg", "dvargdx")])
+        cyc = np.zeros((4, self.grid.nmodes), np.int32)
+        res = np.zeros((4, self.grid.nmodes), np.float64)
+        pu, pd, i0, i1 = pert if pert is not None else (0, 0, 0, 0)
+        o = self.opts(**kw)
+        ptrs = [C.c_void_p(fields[k]) if fields.get(k) else None
+                for k in ("ux", "uy", "uz", "wx", "wy", "wz", "th")]
+        _check(lib().tbl_rk4_step_dev(self.h, Re, dt, kphys, C.byref(cb), C.c_void_p(bdfc),
+                                      C.c_void_p(d2uydx2), C.c_void_p(pu) if pu else None,
+                                      C.c_void_p(pd) if pd else None, i0, i1, *ptrs,
+                                      C.byref(o), _ptr(cyc), _ptr(res)), "tbl_rk4_step_dev")
+        return cyc, res
 
     def set_line_shape(self, prec, level, lanes):
         _check(lib().tbl_set_line_shape(self.h, prec, level, lanes), "tbl_set_line_shape")

@@ -45,6 +45,8 @@ SOURCES = [
     "tbl_files.cpp",
     "tbl_xline.hip",
     "tbl_loop.cpp",
+    "tbl_rhs.hip",
+    "tbl_rk4.cpp",
 ]
 
 

@@ -63,6 +63,12 @@ struct tbl_ctx {
     double *d_xpenta = nullptr, *d_xinteg = nullptr, *d_xfilter = nullptr;
     tbl::cx<double> *d_t1 = nullptr, *d_t2 = nullptr;               // (nx, ny, K)
     tbl::cx<double> *d_lapv = nullptr, *d_dwydy = nullptr, *d_d2wy = nullptr;  // (nx, K)
+
+    // right-hand side and RK4 step (tbl_rk4.cpp): 15 work fields of drv_gv,
+    // 9 of the RK4 update (dv1, dv2, w1 of wx, wy, wz), the z-DFT twiddles
+    tbl::cx<double> *d_rhs = nullptr, *d_rk = nullptr;
+    double *d_tw = nullptr;
+    int tw_n = 0;
 };
 
 namespace tbl {

@@ -1,6 +1,7 @@
 // Multigrid transfer, defect, layout and bookkeeping kernels of the
 // v-Poisson solve (gfx950), and the smoother dispatch.  The smoothers live in
 // tbl_smooth_phase.hip, tbl_smooth_strip.hip and tbl_smooth_coarse.hip.
+#include "tbl_rhs.h"
 #include "tbl_xline.h"
 #include "tbl_relax.h"
 
@@ -470,6 +471,7 @@ int set_kernel_attributes()
     if (rc == TBL_OK) rc = sweep_kernel_attributes();
     if (rc == TBL_OK) rc = deriv_kernel_attributes();
     if (rc == TBL_OK) rc = xline_kernel_attributes();
+    if (rc == TBL_OK) rc = rhs_kernel_attributes();
     return rc;
 }
 

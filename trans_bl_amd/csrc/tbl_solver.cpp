@@ -304,6 +304,7 @@ extern "C" int tbl_destroy(tbl_ctx *c)
     fr(c->d_u0); fr(c->d_u1); fr(c->d_u2);
     fr(c->d_xpenta); fr(c->d_xinteg); fr(c->d_xfilter);
     fr(c->d_t1); fr(c->d_t2); fr(c->d_lapv); fr(c->d_dwydy); fr(c->d_d2wy);
+    fr(c->d_rhs); fr(c->d_rk); fr(c->d_tw);
     if (c->h_nactive) (void)hipHostFree(c->h_nactive);
     for (auto &g : c->graph)
         if (g) (void)hipGraphExecDestroy(g);
