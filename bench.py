@@ -89,6 +89,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-modes", type=int, default=64, help="modes in the CPU sample")
     ap.add_argument("--no-guess", action="store_true", help="ignore a trained network")
+    ap.add_argument("--no-rk4", action="store_true", help="skip the RK4-step measurement")
     return ap.parse_args()
 
 
@@ -152,6 +153,76 @@ def cpu_baseline_port(grid, f, nmodes, workers):
         out = pool.map(cpu_worker, jobs)
     wall = time.perf_counter() - t0
     return wall, out
+
+
+def rk4_step_bench(torch, ctx, grid, steps=2):
+    """The whole RK-stage update on the bench grid: one RK4 time step of the
+    Goertler case (tbl_rk4_step_dev: four drv_gv with the linear terms +
+    stage update + loop, whose poi_uy runs to 1e-5 / 1e-6 as the reference
+    calls it), on synthetic fields (GPU wavepackets, vorticities scaled by
+    0.1), an analytic boundary-layer base flow, the reference's Re, the
+    buffer-zone table with the reference's proportions; FP64 and the mixed
+    cycle.  Also drv_gv and loop alone.  Timed with hipEvents on the
+    library's stream, fields re-copied outside the timed region."""
+    import trans_bl_amd as tb
+    from trans_bl_amd.train_guess import packet_params, rhs_torch
+    names = ("ux", "uy", "uz", "wx", "wy", "wz")
+    F0 = {k: rhs_torch(torch, grid, packet_params(grid, 500 + q), 0.0) * (0.1 if k[0] == "w" else 1.0)
+          for q, k in enumerate(names)}
+    x = torch.as_tensor(grid.x(), device="cuda")
+    y = torch.as_tensor(grid.y(), device="cuda")
+    X, Y = torch.meshgrid(x, y, indexing="xy")
+    d = 0.002 * torch.sqrt(X + 1)
+    base = {"uxb": torch.tanh(Y / d), "uyb": 0.01 * (1 - torch.exp(-Y / d)),
+            "wzb": -(1 - torch.tanh(Y / d) ** 2) / d, "varg": 0.05 + 0.01 * X,
+            "dvargdx": 0.01 + 0 * X}
+    base = {k: v.contiguous() for k, v in base.items()}
+    bp = {k: v.data_ptr() for k, v in base.items()}
+    bd = torch.from_numpy(tb.bdfc_generate(grid.nx, 10, int(grid.nx * 0.785),
+                                           int(grid.nx * 0.88))).to("cuda")
+    d2 = torch.zeros((grid.nmodes, grid.nx), dtype=torch.complex128, device="cuda")
+    F = {k: torch.empty_like(v) for k, v in F0.items()}
+    dv = [torch.empty_like(F0["ux"]) for _ in range(3)]
+    Re, dt = 33124.00001004055, 1e-4
+    st = torch.cuda.Stream()
+    out = {}
+
+    def timed(fn):
+        for k in names:
+            F[k].copy_(F0[k])
+        torch.cuda.synchronize()
+        with torch.cuda.stream(st):
+            ctx.set_stream(st.cuda_stream)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            r = fn()
+            e1.record(st)
+        torch.cuda.synchronize()
+        ctx.set_stream(None)
+        return e0.elapsed_time(e1), r
+
+    fp = {k: F[k].data_ptr() for k in names}
+    for prec in (0, 1):
+        kw = dict(precision=prec, use_graph=1, max_cycles=11)
+        timed(lambda: ctx.rk4_step_dev(Re, dt, 0, bp, bd.data_ptr(), d2.data_ptr(), fp, **kw))
+        ms, cyc = [], None
+        for _ in range(steps):
+            t, (cyc, res) = timed(lambda: ctx.rk4_step_dev(Re, dt, 0, bp, bd.data_ptr(),
+                                                            d2.data_ptr(), fp, **kw))
+            ms.append(t)
+        out["fp64" if prec == 0 else "mixed"] = {
+            "ms_per_step": round(float(np.mean(ms)), 3),
+            "poi_uy_cycles_mean_per_stage": [round(float(c.mean()), 2) for c in cyc]}
+    t_drv, _ = timed(lambda: ctx.drv_gv_dev(Re, 0, bp, *[fp[k] for k in names],
+                                             *[v.data_ptr() for v in dv]))
+    t_loop, _ = timed(lambda: ctx.loop_dev(1e-5, bd.data_ptr(), d2.data_ptr(),
+                                           *[fp[k] for k in names], precision=1, use_graph=1))
+    out.update({"drv_gv_ms": round(t_drv, 3), "loop_ms_mixed": round(t_loop, 3),
+                "grid": f"{grid.nx}x{grid.ny}x{grid.nmodes}", "Re": Re, "dt": dt,
+                "terms": "linear (lterms_gv, the reference's drv_gv)",
+                "data": "synthetic: GPU wavepackets, analytic base flow"})
+    return out
 
 
 def main():
@@ -326,6 +397,10 @@ def main():
                       "bytes_per_launch": int(bytes_per_launch),
                       "lanes_per_line": ctx.line_shape(prec, 1)}
 
+    rk4 = None
+    if not a.no_rk4:
+        rk4 = rk4_step_bench(torch, ctx, grid)
+
     line = {
         "metric": METRIC,
         "value": round(ratem, 2),
@@ -359,6 +434,7 @@ def main():
         "roofline": roof[1],
         "roofline_fp64_smoother": roof[0],
         "roofline_derivs": roof_d,
+        "rk4_step": rk4,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

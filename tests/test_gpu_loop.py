@@ -194,3 +194,29 @@ def test_poi_and_wall_at_benchmark_size(torch_mod):
         e = rel(got[k], r)
         print(f"  {k}: rel {e:.2e}")
         assert e < TOL, k
+
+
+def test_graph_cache_with_alternating_tolerances(torch_mod):
+    """The RK loop calls poi_uy at 1e-5 and 1e-6 in turn: the two captured
+    V-cycle graphs (one per tolerance) are reused, and every replay gives the
+    bits of the uncaptured cycle."""
+    torch = torch_mod
+    g = SMALL
+    F, d2, b = inputs(g)
+    res = {}
+    for use_graph in (0, 1):
+        with tb.Context(g, coefs=coefs_of(g)) as ctx:
+            dv = {k: _dev(torch, F[k]) for k in F}
+            db, dd = _dev(torch, b), _dev(torch, d2)
+            cyc = []
+            for erro in (1e-5, 1e-6, 1e-5, 1e-6):
+                c, _, _ = ctx.loop_dev(erro, db.data_ptr(), dd.data_ptr(),
+                                       *[dv[k].data_ptr() for k in ("ux", "uy", "uz", "wx",
+                                                                   "wy", "wz")],
+                                       precision=1, use_graph=use_graph)
+                cyc.append(c)
+            res[use_graph] = ({k: _host(torch, t) for k, t in dv.items()}, cyc)
+    for k in F:
+        assert np.array_equal(res[0][0][k], res[1][0][k]), k
+    for a, c in zip(res[0][1], res[1][1]):
+        assert np.array_equal(a, c)

@@ -48,9 +48,12 @@ struct tbl_ctx {
     double *d_u0 = nullptr, *d_u1 = nullptr, *d_u2 = nullptr;
 
     // captured V-cycle graphs (precision 0 / 1) and their parameters
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int graph_key[2][5] = {{0}};
-    double graph_tol[2] = {0, 0};
+    // two cached graphs per precision (the RK loop alternates two
+    // tolerances); the least recently used one is replaced
+    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    int graph_key[2][2][5] = {};
+    double graph_tol[2][2] = {{0, 0}, {0, 0}};
+    int graph_last[2] = {0, 0};
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 

@@ -306,8 +306,9 @@ extern "C" int tbl_destroy(tbl_ctx *c)
     fr(c->d_t1); fr(c->d_t2); fr(c->d_lapv); fr(c->d_dwydy); fr(c->d_d2wy);
     fr(c->d_rhs); fr(c->d_rk); fr(c->d_tw);
     if (c->h_nactive) (void)hipHostFree(c->h_nactive);
-    for (auto &g : c->graph)
-        if (g) (void)hipGraphExecDestroy(g);
+    for (auto &gp : c->graph)
+        for (auto &g : gp)
+            if (g) (void)hipGraphExecDestroy(g);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -510,11 +511,16 @@ int run_cycle(tbl_ctx *c, int prec, const Sched &s, double tol, bool use_graph)
 {
     if (!use_graph) return prec ? mixed_cycle(c, s, tol) : fas_cycle(c, s, tol);
     const int key[5] = {s.nu1, s.nu2, s.nc, shapes_key(c->G64), shapes_key(c->G32)};
-    if (!c->graph[prec] || std::memcmp(key, c->graph_key[prec], sizeof(key)) != 0 ||
-        c->graph_tol[prec] != tol) {
-        if (c->graph[prec]) {
-            TBL_HIP(hipGraphExecDestroy(c->graph[prec]));
-            c->graph[prec] = nullptr;
+    int slot = -1;
+    for (int q = 0; q < 2; ++q)
+        if (c->graph[prec][q] && std::memcmp(key, c->graph_key[prec][q], sizeof(key)) == 0 &&
+            c->graph_tol[prec][q] == tol)
+            slot = q;
+    if (slot < 0) {
+        slot = 1 - c->graph_last[prec];
+        if (c->graph[prec][slot]) {
+            TBL_HIP(hipGraphExecDestroy(c->graph[prec][slot]));
+            c->graph[prec][slot] = nullptr;
         }
         if (!c->stream) return TBL_ERR_STATE;   // capture needs a non-null stream
         hipGraph_t g;
@@ -523,12 +529,13 @@ int run_cycle(tbl_ctx *c, int prec, const Sched &s, double tol, bool use_graph)
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc != TBL_OK) return rc;
         TBL_HIP(e);
-        TBL_HIP(hipGraphInstantiate(&c->graph[prec], g, nullptr, nullptr, 0));
+        TBL_HIP(hipGraphInstantiate(&c->graph[prec][slot], g, nullptr, nullptr, 0));
         TBL_HIP(hipGraphDestroy(g));
-        std::memcpy(c->graph_key[prec], key, sizeof(key));
-        c->graph_tol[prec] = tol;
+        std::memcpy(c->graph_key[prec][slot], key, sizeof(key));
+        c->graph_tol[prec][slot] = tol;
     }
-    TBL_HIP(hipGraphLaunch(c->graph[prec], c->stream));
+    c->graph_last[prec] = slot;
+    TBL_HIP(hipGraphLaunch(c->graph[prec][slot], c->stream));
     return TBL_OK;
 }
 
