@@ -370,9 +370,10 @@ def main():
     n1 = grid.nx * grid.ny * K
     # algorithmic: each direction reads v once and writes two derivatives
     d_bytes = 6 * 16 * n1
+    tx, ty = measured_traffic("deriv_x_kernel<17>"), measured_traffic("deriv_y_kernel<32, 7>")
     roof_d = {"bound": "hbm", "achieved": round(d_bytes / d_us / 1e3, 1), "peak": HBM_PEAK_GBS,
               "unit": "GB/s", "frac": round(d_bytes / d_us / 1e3 / HBM_PEAK_GBS, 4),
-              "traffic": None,
+              "traffic": tx + ty if tx is not None and ty is not None else None,
               "kernel": "deriv_x_kernel<17> + deriv_y_kernel<32, 7> (d/dx, d2/dx2, d/dy, d2/dy2 "
                         "of one field)",
               "us_per_call": round(d_us, 1), "bytes_per_call": d_bytes}

@@ -96,9 +96,10 @@ def test_staged_conv_taps_stay_in_the_lds_stage(W, MT):
 # ---- derivative kernels (tbl_deriv_kernels.hip) ------------------------------
 def _x_kernel_reach(nx, ny, K):
     """deriv_x_kernel<R>: every wavefront (one x-line) stages the line in its
-    LDS slice of ldx = nx|1 points and reads clamped windows; global loads
-    and stores stay on its line.  Returns (R, max LDS index, LDS elements,
-    max global index, elements)."""
+    LDS slice of ldx = nx|1 points, reads clamped windows, writes its solved
+    points into its output slice (after the 4 input slices) and stores from
+    there; global loads and stores stay on its line.  Returns (R, max LDS
+    index, LDS elements, max global index, elements)."""
     R = next(r for r in (1, 2, 3, 5, 7, 9, 17) if 64 * r >= nx)
     ldx = nx | 1
     lines = ny * K
@@ -112,9 +113,12 @@ def _x_kernel_reach(nx, ny, K):
     i0 = lane * R
     q = np.arange(R + 4)
     win = np.clip(i0[:, None] + q[None, :] - 2, 0, nx - 1)        # LDS window
-    lds_max = 3 * ldx + max(gi.max(), win.max())
+    seg = (lane[:, None] * R + np.arange(R)[None, :])             # lane-segment writes
+    seg = seg[seg < nx]
+    lds_in = 3 * ldx + max(gi.max(), win.max())
+    lds_out = (4 + 3) * ldx + max(seg.max(), gi.max())
     gmax = (line[live].max() * nx + nx - 1)
-    return R, lds_max, 4 * ldx, gmax, lines * nx
+    return R, max(lds_in, lds_out), 8 * ldx, gmax, lines * nx
 
 
 def _y_kernel_reach(nx, ny, K):
@@ -139,7 +143,7 @@ def test_derivative_kernels_stay_in_bounds(nx, ny, K):
     if (nx + 63) // 64 <= 17:
         R, lds_max, lds_n, gmax, n = _x_kernel_reach(nx, ny, K)
         assert lds_max < lds_n and gmax < n and 64 * R >= nx
-        assert 4 * (nx | 1) * 16 <= 160 * 1024
+        assert 8 * (nx | 1) * 16 <= 160 * 1024
     if ny <= 64 * 9:
         G, R, lds_max, lds_n, gmax, n = _y_kernel_reach(nx, ny, K)
         assert lds_max < lds_n and gmax < n and G * R >= ny

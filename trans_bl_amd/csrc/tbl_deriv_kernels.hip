@@ -157,7 +157,9 @@ __device__ __forceinline__ C rhs_x_centre(C m2, C m1, C c0, C p1, C p2, double s
 }
 
 // x-lines: o1 = d/dx, o2 = d2/dx2 of f (either may be null).  LDS: the
-// lines of the workgroup's kDerivXLines wavefronts, ldx complex each.
+// input lines of the workgroup's kDerivXLines wavefronts, ldx complex each,
+// then as many output lines: a lane's solved points go there and leave as
+// coalesced rows.
 constexpr int kDerivXLines = 4;
 
 template <int R>
@@ -180,6 +182,7 @@ __global__ void __launch_bounds__(64 * kDerivXLines) deriv_x_kernel(DerivArgs A,
     }
     lds_wave_sync();
     const int i0 = lane * R;
+    C *Os = xline + static_cast<size_t>(kDerivXLines + w) * ldx;   // the wave's output line
 #pragma unroll 1
     for (int second = 0; second < 2; ++second) {
         C *o = second ? o2 : o1;
@@ -202,12 +205,20 @@ __global__ void __launch_bounds__(64 * kDerivXLines) deriv_x_kernel(DerivArgs A,
             u[r] = sel(i < nx, v, C{0, 0});
         }
         thomas_scan<64, R>(thomas_seg(A, second), lane, u);
-        C *ol = o + line * nx;
+        // the lane's points through the output line, out as coalesced stores
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = i0 + r;
-            if (i < nx) ol[i] = u[r];
+            if (i < nx) Os[i] = u[r];
         }
+        lds_wave_sync();
+        C *ol = o + line * nx;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = lane + 64 * r;
+            if (i < nx) ol[i] = Os[i];
+        }
+        lds_wave_sync();
     }
 }
 
@@ -260,7 +271,15 @@ __global__ void __launch_bounds__(256) deriv_y_kernel(DerivArgs A, int ldy, cons
     constexpr int TX = 256 / G;
     extern __shared__ C ytile[];
     const int nx = A.nx, ny = A.ny;
-    const int x0 = blockIdx.x * TX, k = blockIdx.y;
+    // workgroup n takes tile (n mod 8) * ceil(B/8) + n / 8 of the B tiles
+    // (x fastest, then mode): the hardware deals workgroups to the 8 XCDs
+    // round-robin, so neighbouring tiles -- whose rows share the cache line
+    // at their boundary -- run on one XCD and share its L2
+    const int tiles_x = (nx + TX - 1) / TX;
+    const int B = tiles_x * A.K, per = (B + 7) / 8;
+    const int m = static_cast<int>(blockIdx.x % 8) * per + static_cast<int>(blockIdx.x / 8);
+    if (m >= B) return;
+    const int x0 = (m % tiles_x) * TX, k = m / tiles_x;
     const size_t mb = static_cast<size_t>(k) * nx * ny;
     // tile load: TX consecutive x of each row (one contiguous run per row)
     for (int e = threadIdx.x; e < TX * ny; e += 256) {
@@ -564,7 +583,7 @@ static int deriv_x(const DerivArgs &A, const C *f, C *o1, C *o2, hipStream_t st)
     }
     const dim3 grid(static_cast<unsigned>((lines + kDerivXLines - 1) / kDerivXLines));
     const int ldx = A.nx | 1;
-    const size_t lds = static_cast<size_t>(kDerivXLines) * ldx * sizeof(C);
+    const size_t lds = 2 * static_cast<size_t>(kDerivXLines) * ldx * sizeof(C);
 #define X(RR)                                                                             \
     if (A.xR == RR) {                                                                     \
         hipLaunchKernelGGL(deriv_x_kernel<RR>, grid, dim3(64 * kDerivXLines), lds, st, A, \
@@ -586,8 +605,8 @@ static int deriv_y(const DerivArgs &A, const C *f, C *o1, int fv, C *o2, hipStre
         return TBL_OK;
     }
     const int G = A.yG;
-    const dim3 grid(static_cast<unsigned>((A.nx + 256 / G - 1) / (256 / G)),
-                    static_cast<unsigned>(A.K));
+    const int tiles = (A.nx + 256 / G - 1) / (256 / G) * A.K;
+    const dim3 grid(static_cast<unsigned>((tiles + 7) / 8 * 8));
     const size_t lds = static_cast<size_t>(256 / G) * ldy * sizeof(C);
 #define X(RR)                                                                                \
     if (A.yR == RR) {                                                                        \

@@ -32,15 +32,18 @@ __device__ __forceinline__ cxd mul_i(double s, cxd a) { return {-s * a.im, s * a
 // z(i) = g(i) - P(i) z(i-1) - Q(i) z(i-2) (FWD) or with i+1, i+2 (backward),
 // g = Z * S (S null: g = Z); Z in LDS holds g on entry, z on exit.  Rows
 // past the line must hold g = 0 with P = Q = 0 (they lead a backward pass).
-template <bool FWD>
-__device__ void recur2(cxd *Z, int R, const double *__restrict__ P, const double *__restrict__ Q,
+// RT > 0: R = RT rows per lane known at compile time (the row loops unroll,
+// so a lane's table loads issue together); RT = 0: R at run time, the row
+// loops unrolled by 4 (longer segments would hold too many registers).
+template <bool FWD, int RT>
+__device__ void recur2(cxd *Z, int Rrt, const double *__restrict__ P, const double *__restrict__ Q,
                        const double *__restrict__ S, int lane)
 {
+    const int R = RT > 0 ? RT : Rrt;
     const int base = lane * R;
     cxd z1 = cz(), z2 = cz();
     double m11 = 1.0, m12 = 0.0, m21 = 0.0, m22 = 1.0;
-#pragma unroll 1
-    for (int t = 0; t < R; ++t) {
+    auto step = [&](int t) {
         const int r = FWD ? t : R - 1 - t;
         const int e = r * 64 + lane;
         const double p = P[e], q = Q ? Q[e] : 0.0;
@@ -55,6 +58,13 @@ __device__ void recur2(cxd *Z, int R, const double *__restrict__ P, const double
         m11 = n1;
         m12 = n2;
         Z[base + r] = z;
+    };
+    if constexpr (RT > 0) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t) step(t);
+    } else {
+#pragma unroll 4
+        for (int t = 0; t < R; ++t) step(t);
     }
     // inclusive scan of the lanes' affine maps (state out = M state in + c)
     cxd c1 = z1, c2 = z2;
@@ -88,8 +98,7 @@ __device__ void recur2(cxd *Z, int R, const double *__restrict__ P, const double
         h1 = cz();
         h2 = cz();
     }
-#pragma unroll 1
-    for (int t = 0; t < R; ++t) {
+    auto fix = [&](int t) {
         const int r = FWD ? t : R - 1 - t;
         const int e = r * 64 + lane;
         const double p = P[e], q = Q ? Q[e] : 0.0;
@@ -97,6 +106,13 @@ __device__ void recur2(cxd *Z, int R, const double *__restrict__ P, const double
         h2 = h1;
         h1 = h;
         Z[base + r] = Z[base + r] + h;
+    };
+    if constexpr (RT > 0) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t) fix(t);
+    } else {
+#pragma unroll 4
+        for (int t = 0; t < R; ++t) fix(t);
     }
 }
 
@@ -104,6 +120,7 @@ __device__ void recur2(cxd *Z, int R, const double *__restrict__ P, const double
 // F = cA A + i cb B; the right-hand side rows are row 1: the line's own
 // boundary value, rows 2 and nx-1: sB (F + 10 F + F), rows 3..nx-2:
 // sI (F + 5.5 F + F), row nx: sL F.
+template <int RT>
 __global__ void __launch_bounds__(64) xpenta_kernel(XPenta a)
 {
     extern __shared__ cxd lds[];
@@ -145,14 +162,15 @@ __global__ void __launch_bounds__(64) xpenta_kernel(XPenta a)
     }
     __syncthreads();
     const double *T = a.tab + static_cast<size_t>(k - 1) * 5 * NP;
-    recur2<true>(Z, R, T, T + NP, nullptr, lane);
+    recur2<true, RT>(Z, R, T, T + NP, nullptr, lane);
     __syncthreads();
-    recur2<false>(Z, R, T + 3 * NP, T + 4 * NP, T + 2 * NP, lane);
+    recur2<false, RT>(Z, R, T + 3 * NP, T + 4 * NP, T + 2 * NP, lane);
     __syncthreads();
     for (int i = lane + 1; i < nx; i += 64) X[i] = Z[i];
 }
 
 // ---- k = 1 continuity integration (poi_ux) and wz_wall ---------------------
+template <int RT>
 __global__ void __launch_bounds__(64) xinteg_kernel(XInteg a)
 {
     extern __shared__ cxd lds[];
@@ -195,12 +213,13 @@ __global__ void __launch_bounds__(64) xinteg_kernel(XInteg a)
         Z[i] = g;
     }
     __syncthreads();
-    recur2<true>(Z, R, a.tab, a.tab + NP, nullptr, lane);
+    recur2<true, RT>(Z, R, a.tab, a.tab + NP, nullptr, lane);
     __syncthreads();
     for (int i = lane + 1; i < nx; i += 64) X[i] = Z[i];
 }
 
 // ---- filter_trid: explicit 7-point right-hand side + tridiagonal solve -----
+template <int RT>
 __global__ void __launch_bounds__(64) xfilter_kernel(XFilter a)
 {
     extern __shared__ cxd lds[];
@@ -236,9 +255,9 @@ __global__ void __launch_bounds__(64) xfilter_kernel(XFilter a)
         Z[i] = g;
     }
     __syncthreads();
-    recur2<true>(Z, R, a.tab, nullptr, a.tab + NP, lane);
+    recur2<true, RT>(Z, R, a.tab, nullptr, a.tab + NP, lane);
     __syncthreads();
-    recur2<false>(Z, R, a.tab + 2 * NP, nullptr, nullptr, lane);
+    recur2<false, RT>(Z, R, a.tab + 2 * NP, nullptr, nullptr, lane);
     __syncthreads();
     for (int i = lane + 1; i < nx; i += 64) Wl[i] = Z[i];
 }
@@ -322,11 +341,19 @@ __global__ void outflow_row_kernel(cxd *uy, int nx, int ny, int K)
 
 size_t xline_lds(int R) { return 2 * sizeof(cxd) * 64 * static_cast<size_t>(R); }
 
+// rows per lane compiled for (up to 576-point lines); longer lines run the
+// run-time R form
+#define TBL_XLR(X) X(1) X(2) X(3) X(5) X(9)
+
 int xpenta(const XPenta &a, hipStream_t st)
 {
     const int lines = a.kind == kXWall ? a.K - 1 : (a.ny - 1) * (a.K - 1);
     if (lines <= 0) return TBL_OK;
-    hipLaunchKernelGGL(xpenta_kernel, dim3(lines), dim3(64), xline_lds(a.R), st, a);
+#define X(RR) \
+    if (a.R == RR) { hipLaunchKernelGGL(xpenta_kernel<RR>, dim3(lines), dim3(64), xline_lds(a.R), st, a); return TBL_OK; }
+    TBL_XLR(X)
+#undef X
+    hipLaunchKernelGGL(xpenta_kernel<0>, dim3(lines), dim3(64), xline_lds(a.R), st, a);
     return TBL_OK;
 }
 
@@ -334,14 +361,22 @@ int xinteg(const XInteg &a, hipStream_t st)
 {
     const int lines = a.kind == kXIntUx ? a.ny - 2 : a.K;
     if (lines <= 0) return TBL_OK;
-    hipLaunchKernelGGL(xinteg_kernel, dim3(lines), dim3(64), xline_lds(a.R), st, a);
+#define X(RR) \
+    if (a.R == RR) { hipLaunchKernelGGL(xinteg_kernel<RR>, dim3(lines), dim3(64), xline_lds(a.R), st, a); return TBL_OK; }
+    TBL_XLR(X)
+#undef X
+    hipLaunchKernelGGL(xinteg_kernel<0>, dim3(lines), dim3(64), xline_lds(a.R), st, a);
     return TBL_OK;
 }
 
 int xfilter(const XFilter &a, hipStream_t st)
 {
     const int lines = 3 * a.K * (a.ny - 1);
-    hipLaunchKernelGGL(xfilter_kernel, dim3(lines), dim3(64), xline_lds(a.R), st, a);
+#define X(RR) \
+    if (a.R == RR) { hipLaunchKernelGGL(xfilter_kernel<RR>, dim3(lines), dim3(64), xline_lds(a.R), st, a); return TBL_OK; }
+    TBL_XLR(X)
+#undef X
+    hipLaunchKernelGGL(xfilter_kernel<0>, dim3(lines), dim3(64), xline_lds(a.R), st, a);
     return TBL_OK;
 }
 
@@ -370,10 +405,14 @@ int outflow_row(cx<double> *uy, int nx, int ny, int K, hipStream_t st)
 int xline_kernel_attributes()
 {
     const int lds = static_cast<int>(xline_lds(kXMaxR));
-    for (const void *f : {reinterpret_cast<const void *>(&xpenta_kernel),
-                          reinterpret_cast<const void *>(&xinteg_kernel),
-                          reinterpret_cast<const void *>(&xfilter_kernel)})
+#define X(RR)                                                                             \
+    for (const void *f : {reinterpret_cast<const void *>(&xpenta_kernel<RR>),             \
+                          reinterpret_cast<const void *>(&xinteg_kernel<RR>),             \
+                          reinterpret_cast<const void *>(&xfilter_kernel<RR>)})           \
         TBL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    TBL_XLR(X)
+    X(0)
+#undef X
     return TBL_OK;
 }
 

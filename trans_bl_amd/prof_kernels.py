@@ -9,7 +9,9 @@ Runs, on the 1025x193x64 v-Poisson workload:
      256 MiB written);
   2. two sweeps of the finest-level line smoother, FP64 then FP32;
   3. one flagship solve (FP32 smoothing under FP64 defect correction) and one
-     plain FP64 solve, each from zero to 1e-9.
+     plain FP64 solve, each from zero to 1e-9;
+  4. the four derivatives of the solution (tbl_derivs_dev: one x and one y
+     launch), twice.
 Prints one JSON line per stage so the counter rows can be matched to stages
 by dispatch order.
 
@@ -67,6 +69,11 @@ def main():
                                          max_cycles=40, guess=1, use_graph=0)
         print(json.dumps({"stage": "solve", "precision": prec, "cycles_max": int(cyc.max()),
                           "residual_max": float(res.max())}), flush=True)
+    outs = [torch.empty_like(u) for _ in range(4)]
+    for _ in range(2):
+        ctx.derivs_dev(u.data_ptr(), *[o.data_ptr() for o in outs])
+    ctx.synchronize()
+    print(json.dumps({"stage": "derivs", "launches": 4}), flush=True)
     ctx.close()
 
 
@@ -74,7 +81,8 @@ def _counter_rows(directory, counter):
     import csv
     import glob
     rows = []
-    for path in glob.glob(os.path.join(directory, "*counter_collection.csv")):
+    for path in glob.glob(os.path.join(directory, "**", "*counter_collection.csv"),
+                          recursive=True):
         with open(path) as f:
             for r in csv.DictReader(f):
                 if r["Counter_Name"] == counter:
