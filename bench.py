@@ -73,9 +73,11 @@ def measured_traffic(kernel):
         return None
     for name, e in table.items():
         if kernel + "(" in name:
-            # the plain sweeps' bytes (the launches the roofline times) when
+            # the bytes of the launches the roofline times (a smoother's plain
+            # sweeps, a derivative kernel's two-derivative launches) when
             # recorded, else the average over all of the kernel's launches
-            return e.get("sweep_bytes_per_launch", e["hbm_bytes_per_launch"])
+            return e.get("sweep_bytes_per_launch",
+                         e.get("derivs_bytes_per_launch", e["hbm_bytes_per_launch"]))
     return None
 METRIC = "v-Poisson mode-solves/s to 1e-9 residual; learned+MG vs plain MG on 1 GPU"
 

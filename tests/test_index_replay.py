@@ -94,12 +94,20 @@ def test_staged_conv_taps_stay_in_the_lds_stage(W, MT):
 
 
 # ---- derivative kernels (tbl_deriv_kernels.hip) ------------------------------
+def _x_chunk(nx):
+    """deriv_x_chunk: multiple of 64, workgroup LDS (4 lines + 4 chunks of
+    16-byte points) within 80 KB, at most the line rounded up to 64."""
+    room = 80 * 1024 // 16 // 4 - (nx | 1)
+    return max(64, min((nx + 63) // 64 * 64, room // 64 * 64))
+
+
 def _x_kernel_reach(nx, ny, K):
     """deriv_x_kernel<R>: every wavefront (one x-line) stages the line in its
-    LDS slice of ldx = nx|1 points, reads clamped windows, writes its solved
-    points into its output slice (after the 4 input slices) and stores from
-    there; global loads and stores stay on its line.  Returns (R, max LDS
-    index, LDS elements, max global index, elements)."""
+    LDS slice of ldx = nx|1 points, reads clamped windows, writes the first
+    derivative's solved points chunk by chunk into its CH-point output chunk
+    (after the 4 input slices) and the second's into its own input slice,
+    and stores from there; global loads and stores stay on its line.
+    Returns (R, max LDS index, LDS elements, max global index, elements)."""
     R = next(r for r in (1, 2, 3, 5, 7, 9, 17) if 64 * r >= nx)
     ldx = nx | 1
     lines = ny * K
@@ -113,12 +121,21 @@ def _x_kernel_reach(nx, ny, K):
     i0 = lane * R
     q = np.arange(R + 4)
     win = np.clip(i0[:, None] + q[None, :] - 2, 0, nx - 1)        # LDS window
-    seg = (lane[:, None] * R + np.arange(R)[None, :])             # lane-segment writes
+    seg = (lane[:, None] * R + np.arange(R)[None, :])             # lane-segment points
     seg = seg[seg < nx]
-    lds_in = 3 * ldx + max(gi.max(), win.max())
-    lds_out = (4 + 3) * ldx + max(seg.max(), gi.max())
+    CH = _x_chunk(nx)
+    assert CH % 64 == 0 and CH >= 64
+    e_max = -1
+    for c0 in range(0, nx, CH):
+        e = seg - c0                                              # chunk writes
+        e = e[(e >= 0) & (e < CH)]
+        st = c0 + np.arange(CH)                                   # chunk reads, all lanes
+        assert np.all(np.isin(st[st < nx], seg))                  # every stored point written
+        e_max = max(e_max, e.max() if e.size else -1, (st[st < nx] - c0).max())
+    lds_in = 3 * ldx + max(gi.max(), win.max(), seg.max())      # incl. last-derivative line
+    lds_out = 4 * ldx + 3 * CH + e_max
     gmax = (line[live].max() * nx + nx - 1)
-    return R, max(lds_in, lds_out), 8 * ldx, gmax, lines * nx
+    return R, max(lds_in, lds_out), 4 * (ldx + CH), gmax, lines * nx
 
 
 def _y_kernel_reach(nx, ny, K):
@@ -143,7 +160,8 @@ def test_derivative_kernels_stay_in_bounds(nx, ny, K):
     if (nx + 63) // 64 <= 17:
         R, lds_max, lds_n, gmax, n = _x_kernel_reach(nx, ny, K)
         assert lds_max < lds_n and gmax < n and 64 * R >= nx
-        assert 8 * (nx | 1) * 16 <= 160 * 1024
+        # two workgroups per CU up to the 1088-point lines of R = 17
+        assert 2 * lds_n * 16 <= 160 * 1024 and _x_chunk(nx) >= 128
     if ny <= 64 * 9:
         G, R, lds_max, lds_n, gmax, n = _y_kernel_reach(nx, ny, K)
         assert lds_max < lds_n and gmax < n and G * R >= ny

@@ -158,12 +158,18 @@ __device__ __forceinline__ C rhs_x_centre(C m2, C m1, C c0, C p1, C p2, double s
 
 // x-lines: o1 = d/dx, o2 = d2/dx2 of f (either may be null).  LDS: the
 // input lines of the workgroup's kDerivXLines wavefronts, ldx complex each,
-// then as many output lines: a lane's solved points go there and leave as
-// coalesced rows.
+// then one output chunk of ch points (a multiple of 64) per wavefront.  A
+// solved line goes from the lanes' segments back to coalesced rows through
+// LDS: the last derivative of the call through its own input line (no
+// longer read once its right-hand side is formed), in one pass; the first
+// of two through the chunk buffer, ch points a pass.  The launch sizes ch
+// so that a workgroup stays within kDerivXWgLds, two workgroups a CU (two
+// wavefronts per SIMD, the occupancy the kernel's registers allow).
 constexpr int kDerivXLines = 4;
+constexpr int kDerivXWgLds = 80 * 1024;
 
 template <int R>
-__global__ void __launch_bounds__(64 * kDerivXLines) deriv_x_kernel(DerivArgs A, int ldx,
+__global__ void __launch_bounds__(64 * kDerivXLines) deriv_x_kernel(DerivArgs A, int ldx, int ch,
                                                                     const C *__restrict__ f,
                                                                     C *o1, C *o2)
 {
@@ -175,14 +181,19 @@ __global__ void __launch_bounds__(64 * kDerivXLines) deriv_x_kernel(DerivArgs A,
     if (line >= static_cast<size_t>(A.ny) * A.K) return;   // whole wavefront
     C *Ls = xline + static_cast<size_t>(w) * ldx;
     const C *fl = f + line * nx;
+    {
+        // every load issued before the first is waited for (indices clamped
+        // into the line; a load under a lane mask would wait per point)
+        C v[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int i = lane + 64 * r;
-        if (i < nx) Ls[i] = fl[i];
+        for (int r = 0; r < R; ++r) v[r] = fl[min(lane + 64 * r, nx - 1)];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (lane + 64 * r < nx) Ls[lane + 64 * r] = v[r];
     }
     lds_wave_sync();
     const int i0 = lane * R;
-    C *Os = xline + static_cast<size_t>(kDerivXLines + w) * ldx;   // the wave's output line
+    C *Os = xline + static_cast<size_t>(kDerivXLines) * ldx + w * ch;   // output chunk
 #pragma unroll 1
     for (int second = 0; second < 2; ++second) {
         C *o = second ? o2 : o1;
@@ -205,20 +216,33 @@ __global__ void __launch_bounds__(64 * kDerivXLines) deriv_x_kernel(DerivArgs A,
             u[r] = sel(i < nx, v, C{0, 0});
         }
         thomas_scan<64, R>(thomas_seg(A, second), lane, u);
-        // the lane's points through the output line, out as coalesced stores
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int i = i0 + r;
-            if (i < nx) Os[i] = u[r];
-        }
-        lds_wave_sync();
         C *ol = o + line * nx;
+        if (second || !o2) {
+            // the call's last derivative: out through the input line
+            lds_wave_sync();
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int i = lane + 64 * r;
-            if (i < nx) ol[i] = Os[i];
+            for (int r = 0; r < R; ++r)
+                if (i0 + r < nx) Ls[i0 + r] = u[r];
+            lds_wave_sync();
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (lane + 64 * r < nx) ol[lane + 64 * r] = Ls[lane + 64 * r];
+            break;
         }
-        lds_wave_sync();
+        // the lane's points through the output chunk, out as coalesced stores
+#pragma unroll 1
+        for (int c0 = 0; c0 < nx; c0 += ch) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int e = i0 + r - c0;
+                if (e >= 0 && e < ch && i0 + r < nx) Os[e] = u[r];
+            }
+            lds_wave_sync();
+#pragma unroll 1
+            for (int e = lane; e < ch; e += 64)
+                if (c0 + e < nx) ol[c0 + e] = Os[e];
+            lds_wave_sync();
+        }
     }
 }
 
@@ -281,13 +305,26 @@ __global__ void __launch_bounds__(256) deriv_y_kernel(DerivArgs A, int ldy, cons
     if (m >= B) return;
     const int x0 = (m % tiles_x) * TX, k = m / tiles_x;
     const size_t mb = static_cast<size_t>(k) * nx * ny;
-    // tile load: TX consecutive x of each row (one contiguous run per row)
-    for (int e = threadIdx.x; e < TX * ny; e += 256) {
-        const int x = e % TX, j = e / TX;
-        const bool in = x0 + x < nx;
-        const size_t g = mb + static_cast<size_t>(j) * nx + (in ? x0 + x : 0);
-        const C v = f[g];
-        ytile[x * ldy + j] = in ? v : C{0, 0};
+    // tile load: TX consecutive x of each row (one contiguous run per row);
+    // TX * ny <= 256 R, so R steps of the workgroup cover the tile, and all
+    // of a thread's loads are issued before the first is waited for
+    {
+        C v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int e = min(static_cast<int>(threadIdx.x) + 256 * q, TX * ny - 1);
+            const int x = e % TX, j = e / TX;
+            const int xi = x0 + x < nx ? x0 + x : nx - 1;
+            v[q] = f[mb + static_cast<size_t>(j) * nx + xi];
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int e = static_cast<int>(threadIdx.x) + 256 * q;
+            if (e < TX * ny) {
+                const int x = e % TX, j = e / TX;
+                ytile[x * ldy + j] = x0 + x < nx ? v[q] : C{0, 0};
+            }
+        }
     }
     __syncthreads();
     const int col = threadIdx.x / G, lane = threadIdx.x % G;
@@ -570,6 +607,16 @@ void deriv_shapes(int nx, int ny, int *xR, int *yG, int *yR)
             if (needy <= r) { *yR = r; break; }
 }
 
+// output chunk of deriv_x_kernel: the largest multiple of 64 points (at
+// least 64, at most the line rounded up) that keeps a workgroup's LDS within
+// kDerivXWgLds; lines up to 1088 points leave at least 128
+int deriv_x_chunk(int nx)
+{
+    const int ldx = nx | 1;
+    const int room = kDerivXWgLds / static_cast<int>(sizeof(C)) / kDerivXLines - ldx;
+    return std::max(64, std::min((nx + 63) / 64 * 64, room / 64 * 64));
+}
+
 static int deriv_x(const DerivArgs &A, const C *f, C *o1, C *o2, hipStream_t st)
 {
     const int need = (A.nx + 63) / 64;
@@ -583,11 +630,12 @@ static int deriv_x(const DerivArgs &A, const C *f, C *o1, C *o2, hipStream_t st)
     }
     const dim3 grid(static_cast<unsigned>((lines + kDerivXLines - 1) / kDerivXLines));
     const int ldx = A.nx | 1;
-    const size_t lds = 2 * static_cast<size_t>(kDerivXLines) * ldx * sizeof(C);
+    const int ch = deriv_x_chunk(A.nx);
+    const size_t lds = static_cast<size_t>(kDerivXLines) * (ldx + ch) * sizeof(C);
 #define X(RR)                                                                             \
     if (A.xR == RR) {                                                                     \
         hipLaunchKernelGGL(deriv_x_kernel<RR>, grid, dim3(64 * kDerivXLines), lds, st, A, \
-                           ldx, f, o1, o2);                                               \
+                           ldx, ch, f, o1, o2);                                           \
         return TBL_OK;                                                                    \
     }
     TBL_XR(X)

@@ -7,10 +7,12 @@
 // der2wydxdy, the buffer-zone damping bzone/bzonew and outuy's outflow row.
 //
 // Every x-line (one (j, k) row of a user-layout field, x contiguous) is one
-// wavefront.  The line is staged in LDS, its right-hand side formed from
-// the staged values, and each linear recurrence along x -- the L and U
-// substitutions of the banded LU, the two Thomas passes, the integration --
-// runs as a partitioned scan: lane l owns rows [l R, l R + R), solves them
+// wavefront.  Its right-hand side is formed from coalesced global loads (a
+// point's stencil neighbours are the next lanes' points, served by the
+// cache) into one LDS line of 64 R points, and each linear recurrence along
+// x -- the L and U substitutions of the banded LU, the two Thomas passes,
+// the integration -- runs as a partitioned scan: lane l owns rows
+// [l R, l R + R), solves them
 // from a zero incoming state while tracking the 2x2 response of its last
 // two values to the incoming ones, the 64 lanes combine those affine maps
 // in a 6-step shuffle scan, and a second pass adds each lane's homogeneous
@@ -30,12 +32,13 @@ __device__ __forceinline__ cxd cz() { return {0.0, 0.0}; }
 __device__ __forceinline__ cxd mul_i(double s, cxd a) { return {-s * a.im, s * a.re}; }
 
 // z(i) = g(i) - P(i) z(i-1) - Q(i) z(i-2) (FWD) or with i+1, i+2 (backward),
-// g = Z * S (S null: g = Z); Z in LDS holds g on entry, z on exit.  Rows
-// past the line must hold g = 0 with P = Q = 0 (they lead a backward pass).
-// RT > 0: R = RT rows per lane known at compile time (the row loops unroll,
-// so a lane's table loads issue together); RT = 0: R at run time, the row
-// loops unrolled by 4 (longer segments would hold too many registers).
-template <bool FWD, int RT>
+// g = Z * S (HS; else g = Z, and HQ false: Q = 0); Z in LDS holds g on
+// entry, z on exit.  Rows past the line must hold g = 0 with P = Q = 0 (they
+// lead a backward pass).  RT > 0: R = RT rows per lane known at compile time
+// (the row loops unroll, so a lane's table loads issue together); RT = 0: R
+// at run time, the row loops unrolled by 4 (longer segments would hold too
+// many registers; loading blocks of 4-16 rows ahead measured no faster).
+template <bool FWD, int RT, bool HQ, bool HS>
 __device__ void recur2(cxd *Z, int Rrt, const double *__restrict__ P, const double *__restrict__ Q,
                        const double *__restrict__ S, int lane)
 {
@@ -46,9 +49,9 @@ __device__ void recur2(cxd *Z, int Rrt, const double *__restrict__ P, const doub
     auto step = [&](int t) {
         const int r = FWD ? t : R - 1 - t;
         const int e = r * 64 + lane;
-        const double p = P[e], q = Q ? Q[e] : 0.0;
+        const double p = P[e], q = HQ ? Q[e] : 0.0;
         cxd g = Z[base + r];
-        if (S) g = g * S[e];
+        if constexpr (HS) g = g * S[e];
         const cxd z = g - p * z1 - q * z2;
         const double n1 = -p * m11 - q * m21, n2 = -p * m12 - q * m22;
         z2 = z1;
@@ -101,7 +104,7 @@ __device__ void recur2(cxd *Z, int Rrt, const double *__restrict__ P, const doub
     auto fix = [&](int t) {
         const int r = FWD ? t : R - 1 - t;
         const int e = r * 64 + lane;
-        const double p = P[e], q = Q ? Q[e] : 0.0;
+        const double p = P[e], q = HQ ? Q[e] : 0.0;
         const cxd h = -p * h1 - q * h2;
         h2 = h1;
         h1 = h;
@@ -146,25 +149,26 @@ __global__ void __launch_bounds__(64) xpenta_kernel(XPenta a)
     const double sB = a.kind == kXUx ? -a.dxx / 12.0 : a.dxx / 12.0;
     const double kk = static_cast<double>(k) * static_cast<double>(k) * a.beta * a.beta;
     const double sL = a.kind == kXUx ? 1.0 / kk : -1.0 / kk;
-    cxd *F = lds, *Z = lds + NP;
+    cxd *Z = lds;
     const cxd *A = a.A + ofA, *B = a.B + ofA;
     cxd *X = a.X + ofX;
-    for (int i = lane; i < nx; i += 64) F[i] = cA * A[i] + mul_i(cb, B[i]);
-    __syncthreads();
+    auto F = [&](int i) { return cA * A[i] + mul_i(cb, B[i]); };
+    // rows past the line hold 0; interior rows take the centred form (the
+    // boundary rows' branches diverge only in the first and last chunks)
     for (int i = lane; i < NP; i += 64) {
         cxd g = cz();
         if (i == 0) g = X[0];
-        else if (i == 1) g = sB * (F[0] + 10.0 * F[1] + F[2]);
-        else if (i < nx - 2) g = s8 * (F[i - 1] + 5.5 * F[i] + F[i + 1]);
-        else if (i == nx - 2) g = sB * (F[nx - 1] + 10.0 * F[nx - 2] + F[nx - 3]);
-        else if (i == nx - 1) g = sL * F[nx - 1];
+        else if (i == 1) g = sB * (F(0) + 10.0 * F(1) + F(2));
+        else if (i < nx - 2) g = s8 * (F(i - 1) + 5.5 * F(i) + F(i + 1));
+        else if (i == nx - 2) g = sB * (F(nx - 1) + 10.0 * F(nx - 2) + F(nx - 3));
+        else if (i == nx - 1) g = sL * F(nx - 1);
         Z[i] = g;
     }
     __syncthreads();
     const double *T = a.tab + static_cast<size_t>(k - 1) * 5 * NP;
-    recur2<true, RT>(Z, R, T, T + NP, nullptr, lane);
+    recur2<true, RT, true, false>(Z, R, T, T + NP, nullptr, lane);
     __syncthreads();
-    recur2<false, RT>(Z, R, T + 3 * NP, T + 4 * NP, T + 2 * NP, lane);
+    recur2<false, RT, true, true>(Z, R, T + 3 * NP, T + 4 * NP, T + 2 * NP, lane);
     __syncthreads();
     for (int i = lane + 1; i < nx; i += 64) X[i] = Z[i];
 }
@@ -177,43 +181,37 @@ __global__ void __launch_bounds__(64) xinteg_kernel(XInteg a)
     const int lane = threadIdx.x;
     const int line = blockIdx.x;
     const int nx = a.nx, R = a.R, NP = 64 * R;
-    cxd *D = lds, *Z = lds + NP;
-    cxd *X;
-    if (a.kind == kXIntUx) {
-        const size_t of = static_cast<size_t>(line + 1) * nx;     // mode 1, row j = line + 2
-        X = a.X + of;
-        for (int i = lane; i < nx; i += 64) D[i] = a.D[of + i];
-    } else {
-        const int k = line;
-        const double kb = -static_cast<double>(k) * a.beta;
-        const size_t of = static_cast<size_t>(k) * a.ny * nx;     // row 1 of mode k
-        X = a.X + of;
-        const cxd *W = a.D + of, *Lp = a.Lp + static_cast<size_t>(k) * nx;
-        for (int i = lane; i < nx; i += 64) D[i] = mul_i(kb, W[i]) - Lp[i];
-    }
-    __syncthreads();
+    cxd *Z = lds;
+    const bool ux = a.kind == kXIntUx;
+    // ux: mode 1, row j = line + 2; wz: row 1 of mode k = line
+    const size_t of = ux ? static_cast<size_t>(line + 1) * nx
+                         : static_cast<size_t>(line) * a.ny * nx;
+    cxd *X = a.X + of;
+    const double kb = -static_cast<double>(line) * a.beta;
+    const cxd *W = a.D + of, *Lp = ux ? nullptr : a.Lp + static_cast<size_t>(line) * nx;
+    auto D = [&](int i) { return ux ? W[i] : mul_i(kb, W[i]) - Lp[i]; };
     const double sdx = a.sdx;
     for (int i = lane; i < NP; i += 64) {
         cxd g = cz();
         if (i == 0) {
             g = X[0];
         } else if (i == 1) {
-            g = (sdx * (251.0 * D[0] + 646.0 * D[1] - 264.0 * D[2] + 106.0 * D[3] - 19.0 * D[4]))
+            g = (sdx * (251.0 * D(0) + 646.0 * D(1) - 264.0 * D(2) + 106.0 * D(3) - 19.0 * D(4)))
                 * (1.0 / 720.0);
         } else if (i < nx - 2) {
-            g = (sdx * (281.0 * D[i - 2] + 2056.0 * D[i - 1] + 1176.0 * D[i] - 104.0 * D[i + 1]
-                        + 11.0 * D[i + 2])) * (1.0 / 2430.0);
+            g = (sdx * (281.0 * D(i - 2) + 2056.0 * D(i - 1) + 1176.0 * D(i) - 104.0 * D(i + 1)
+                        + 11.0 * D(i + 2))) * (1.0 / 2430.0);
         } else if (i == nx - 2) {
-            g = (sdx * (10.0 * D[i - 2] + 57.0 * D[i - 1] + 24.0 * D[i] - D[i + 1]))
+            g = (sdx * (10.0 * D(i - 2) + 57.0 * D(i - 1) + 24.0 * D(i) - D(i + 1)))
                 * (1.0 / 57.0);
         } else if (i == nx - 1) {
-            g = (sdx * (251.0 * D[i] + 646.0 * D[i - 1] - 264.0 * D[i - 2] + 106.0 * D[i - 3]
-                        - 19.0 * D[i - 4])) * (1.0 / 720.0);
+            g = (sdx * (251.0 * D(i) + 646.0 * D(i - 1) - 264.0 * D(i - 2) + 106.0 * D(i - 3)
+                        - 19.0 * D(i - 4))) * (1.0 / 720.0);
         }
         Z[i] = g;
     }
     __syncthreads();
-    recur2<true, RT>(Z, R, a.tab, a.tab + NP, nullptr, lane);
+    recur2<true, RT, true, false>(Z, R, a.tab, a.tab + NP, nullptr, lane);
     __syncthreads();
     for (int i = lane + 1; i < nx; i += 64) X[i] = Z[i];
 }
@@ -230,13 +228,16 @@ __global__ void __launch_bounds__(64) xfilter_kernel(XFilter a)
     const int rem = line % (a.K * rows);
     const int k = rem / rows, j = rem % rows + 1;
     cxd *Wl = (q == 0 ? a.w0 : q == 1 ? a.w1 : a.w2) + (static_cast<size_t>(k) * a.ny + j) * nx;
-    cxd *W = lds, *Z = lds + NP;
-    for (int i = lane; i < nx; i += 64) W[i] = Wl[i];
-    __syncthreads();
+    cxd *Z = lds;
+    // the line is read here and written back after the solve, by this wavefront
+    const cxd *W = Wl;
     const double af = a.af, bf = a.bf, cf = a.cf, df = a.df;
+    const int n = nx;
+    // rows past the line hold 0; interior rows take the centred form (the
+    // boundary rows' branches diverge only in the first and last chunks;
+    // forming every row's candidate forms and selecting measured slower)
     for (int i = lane; i < NP; i += 64) {
         cxd g = cz();
-        const int n = nx;
         if (i == 0) g = (15.0 * W[0] + 4.0 * W[1] - 6.0 * W[2] + 4.0 * W[3] - W[4]) * (1.0 / 16.0);
         else if (i == 1) g = (W[0] + 12.0 * W[1] + 6.0 * W[2] - 4.0 * W[3] + W[4]) * (1.0 / 16.0);
         else if (i == 2) g = (-W[0] + 4.0 * W[1] + 10.0 * W[2] + 4.0 * W[3] - W[4]) * (1.0 / 16.0);
@@ -255,9 +256,9 @@ __global__ void __launch_bounds__(64) xfilter_kernel(XFilter a)
         Z[i] = g;
     }
     __syncthreads();
-    recur2<true, RT>(Z, R, a.tab, nullptr, a.tab + NP, lane);
+    recur2<true, RT, false, true>(Z, R, a.tab, nullptr, a.tab + NP, lane);
     __syncthreads();
-    recur2<false, RT>(Z, R, a.tab + 2 * NP, nullptr, nullptr, lane);
+    recur2<false, RT, false, false>(Z, R, a.tab + 2 * NP, nullptr, nullptr, lane);
     __syncthreads();
     for (int i = lane + 1; i < nx; i += 64) Wl[i] = Z[i];
 }
@@ -339,7 +340,7 @@ __global__ void outflow_row_kernel(cxd *uy, int nx, int ny, int K)
 
 }  // namespace
 
-size_t xline_lds(int R) { return 2 * sizeof(cxd) * 64 * static_cast<size_t>(R); }
+size_t xline_lds(int R) { return sizeof(cxd) * 64 * static_cast<size_t>(R); }
 
 // rows per lane compiled for (up to 576-point lines); longer lines run the
 // run-time R form

@@ -11,7 +11,11 @@ Runs, on the 1025x193x64 v-Poisson workload:
   3. one flagship solve (FP32 smoothing under FP64 defect correction) and one
      plain FP64 solve, each from zero to 1e-9;
   4. the four derivatives of the solution (tbl_derivs_dev: one x and one y
-     launch), twice.
+     launch), twice;
+  5. the x-line solves of loop on those fields, twice each: poi_ux
+     (tbl_poi_ux_dev: the k = 0 pentadiagonal solve and the k = 1
+     integration), poi_uz (tbl_poi_uz_dev) and filter_trid (tbl_filter_trid_dev,
+     three fields in place).
 Prints one JSON line per stage so the counter rows can be matched to stages
 by dispatch order.
 
@@ -26,6 +30,10 @@ also writes the average over their first SWEEP_LAUNCHES dispatches alone
 cached, nothing launches a smoother before them), the same launches that
 `tbl_time_smoother` times; the solves' sweeps also include first sweeps from
 zero (fewer reads) and sweeps that update the FP64 field (more writes).
+Likewise for the derivative kernels, the average over their first
+DERIV_LAUNCHES dispatches (`derivs_bytes_per_launch`): the two
+tbl_derivs_dev calls of step 4, which write two derivatives per launch (the
+x-line solves of step 5 launch the same kernels for one derivative).
 """
 from __future__ import annotations
 
@@ -39,6 +47,7 @@ if ROOT not in sys.path:
 
 COPY_BYTES = 256 << 20
 SWEEP_LAUNCHES = 8   # two sweeps of four launches (border + three colours)
+DERIV_LAUNCHES = 2   # two tbl_derivs_dev calls: one x and one y launch each
 
 
 def main():
@@ -74,6 +83,14 @@ def main():
         ctx.derivs_dev(u.data_ptr(), *[o.data_ptr() for o in outs])
     ctx.synchronize()
     print(json.dumps({"stage": "derivs", "launches": 4}), flush=True)
+    uy, duydy, wy, d2 = outs
+    ux, uz = torch.empty_like(u), torch.empty_like(u)
+    for _ in range(2):
+        ctx.poi_ux_dev(uy.data_ptr(), wy.data_ptr(), ux.data_ptr(), duydy.data_ptr())
+        ctx.poi_uz_dev(wy.data_ptr(), duydy.data_ptr(), uz.data_ptr())
+        ctx.filter_trid_dev(ux.data_ptr(), uz.data_ptr(), d2.data_ptr())
+    ctx.synchronize()
+    print(json.dumps({"stage": "xline", "calls": 6}), flush=True)
     ctx.close()
 
 
@@ -118,11 +135,12 @@ def summarize(fetch_dir, write_dir, out):
         res["kernels"][name] = {"dispatches": len(e["fetch_kib"]),
                                 "fetch_bytes": round(f * 1024), "write_bytes": round(w * 1024),
                                 "hbm_bytes_per_launch": round((f + w) * 1024)}
-        if "smooth_" in name and len(e["fetch_kib"]) >= SWEEP_LAUNCHES:
-            n = SWEEP_LAUNCHES
-            fs8 = sum(e["fetch_kib"][:n]) / n
-            ws8 = sum(e["write_kib"][:n]) / n
-            res["kernels"][name]["sweep_bytes_per_launch"] = round((fs8 + ws8) * 1024)
+        for tag, key, n in (("smooth_", "sweep_bytes_per_launch", SWEEP_LAUNCHES),
+                            ("deriv_", "derivs_bytes_per_launch", DERIV_LAUNCHES)):
+            if tag in name and len(e["fetch_kib"]) >= n:
+                fn = sum(e["fetch_kib"][:n]) / n
+                wn = sum(e["write_kib"][:n]) / n
+                res["kernels"][name][key] = round((fn + wn) * 1024)
     with open(out, "w") as fo:
         json.dump(res, fo, indent=1, sort_keys=True)
     print(json.dumps({"kernels": len(res["kernels"]), "fetch_scale": fs, "write_scale": ws}))
