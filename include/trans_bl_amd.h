@@ -278,6 +278,15 @@ int tbl_drv_gv_dev(tbl_ctx *ctx, double Re, int kphys, const tbl_base *base,
                    const double *d_ux, const double *d_uy, const double *d_uz,
                    const double *d_wx, const double *d_wy, const double *d_wz, double *d_dvx,
                    double *d_dvy, double *d_dvz);
+/* Same with host arrays (the Fortran drop-in's drv_gv, main_par.f:1339):
+   the base flow and curvature term of COMMON /blas/ and /vc/ as real
+   (nx, ny) arrays; every array is copied to the GPU and the three results
+   back (their row 1, which drv_gv does not write, keeps the caller's
+   values). */
+int tbl_drv_gv(tbl_ctx *ctx, double Re, int kphys, const double *uxb, const double *uyb,
+               const double *wzb, const double *varg, const double *dvargdx, const double *ux,
+               const double *uy, const double *uz, const double *wx, const double *wy,
+               const double *wz, double *dvx, double *dvy, double *dvz);
 /* One RK4 time step (main_par.f:266-349): four drv_gv + update stages
    (rows 2..ny-1, columns 3..nx), each followed by loop (erro 1e-5, 1e-5,
    1e-5, 1e-6).  The disturbance of gv_pert (main_par.f:1577-1610) before

@@ -1,8 +1,9 @@
 """The Fortran drop-in layer (trans_bl_amd/fortran) driven by a Fortran
 RK-stage program: derparx, derparxx, dery, deryy of uy, then poi_uy, with
 the reference's subroutine names, COMMON blocks and PARAMETERs (the test's
-own par.for, 129 x 33 x 4 modes, 3 levels), against the CPU oracle; and a
-Fortran program calling the drop-in `loop(erro)`, the whole RK-stage update."""
+own par.for, 129 x 33 x 4 modes, 3 levels), against the CPU oracle; a
+Fortran program calling the drop-in `loop(erro)`, the whole RK-stage update;
+and one calling the drop-in `drv_gv(dvx, dvy, dvz)`, the RK right-hand side."""
 import os
 import subprocess
 
@@ -91,3 +92,46 @@ def test_fortran_loop_matches_oracle(tmp_path, gpu_ready, erro):
         assert rel(out[i], ref[k]) < 1e-12, (k, rel(out[i], ref[k]))
     lines = [l.split() for l in r.stdout.strip().splitlines() if l.strip()]
     assert [int(l[1]) for l in lines[-G.nmodes:]] == cyc.tolist()
+
+
+DRV_EXE = os.path.join(os.path.dirname(EXE), "rk_drv")
+RE_PARFOR = 2.5e3           # Re of tests/fortran/par.for
+
+
+def test_fortran_drv_gv_matches_oracle(tmp_path, gpu_ready):
+    """The drop-in `drv_gv(dvx, dvy, dvz)` (main_par.f:1339-1403) from a
+    Fortran program with the reference's COMMON blocks /var/, /coefs/,
+    /blas/, /vc/ and par.for's Re: the three results on rows 2..jmax against
+    the oracle's drv_gv (lterms_gv) at 1e-12 of each field's largest value,
+    as the device entry point is checked; row 1 stays the caller's zero."""
+    if not os.path.exists(DRV_EXE):
+        pytest.fail(f"{DRV_EXE} not built (run __graft_entry__.build())")
+    coefs = oracle.coefgen.generate(G.stf, G.levels)
+    F = {k: wl.wavepacket_rhs(G, tag="fdrv-" + k) * (0.1 if k[0] == "w" else 1.0)
+         for k in ("ux", "uy", "uz", "wx", "wy", "wz")}
+    X, Y = np.meshgrid(G.x(), G.y())
+    d = 0.002 * np.sqrt(X + 1)
+    base = {"uxb": np.tanh(Y / d), "uyb": 0.01 * (1 - np.exp(-Y / d)),
+            "wzb": -(1 - np.tanh(Y / d) ** 2) / d, "varg": 0.05 + 0.01 * X,
+            "dvargdx": 0.01 + 0 * X}
+    with open(tmp_path / "drv_in.bin", "wb") as fh:
+        for name in ("fp_fd_e", "sp_fd", "cp_fd", "pp_fd", "lp_fd", "fp_sd", "sp_sd", "cp_sd",
+                     "pp_sd", "lp_sd", "sp_poi", "cp_poi", "pp_poi", "lp_poi", "w_at_w",
+                     "dwydy"):
+            np.ascontiguousarray(coefs[name], dtype="<f8").tofile(fh)
+        for k in ("ux", "uy", "uz", "wx", "wy", "wz"):
+            np.ascontiguousarray(F[k], dtype="<c16").tofile(fh)
+        for k in ("uxb", "uyb", "wzb", "varg", "dvargdx"):
+            np.ascontiguousarray(base[k], dtype="<f8").tofile(fh)
+    r = subprocess.run([DRV_EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    shp = (G.nmodes, G.ny, G.nx)
+    out = np.fromfile(tmp_path / "drv_out.bin", dtype="<c16").reshape((3,) + shp)
+    p = oracle.make_params(G.nx, G.ny, G.nmodes, G.levels, G.dx, G.dy0, G.stf, G.alpha, G.beta,
+                           coefs=coefs)
+    ref = oracle.drv_gv(p, RE_PARFOR, 0, base, F)
+    for q, name in enumerate(("dvx", "dvy", "dvz")):
+        e = rel(out[q][:, 1:], ref[q][:, 1:])
+        assert e < 1e-12, (name, e)
+        assert not np.any(out[q][:, 0]), name
+

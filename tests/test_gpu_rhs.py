@@ -116,6 +116,21 @@ def test_drv_gv_matches_oracle(torch_mod, g, kphys):
         assert e < 1e-12, name
 
 
+def test_drv_gv_host_arrays_match_oracle(gpu_ready):
+    """tbl_drv_gv (host arrays, the Fortran drop-in's entry point): the same
+    results as the oracle, row 1 untouched."""
+    g = SMALL
+    F, base, _ = inputs(g)
+    ref = oracle.drv_gv(oparams(g), RE, 0, base, F)
+    with tb.Context(g, coefs=coefs_of(g)) as ctx:
+        got = ctx.drv_gv(RE, 0, base, F)
+    for q, name in enumerate(("dvx", "dvy", "dvz")):
+        e = rel(got[q][:, 1:], ref[q][:, 1:])
+        print(f"  {name}: rel {e:.2e}")
+        assert e < 1e-12, name
+        assert not np.any(got[q][:, 0]), name
+
+
 @pytest.mark.parametrize("g, kphys", [(SMALL, 0), (SMALL, 8)], ids=["lin", "nl8"])
 def test_rk4_step_matches_oracle(torch_mod, g, kphys):
     torch = torch_mod

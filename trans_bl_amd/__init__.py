@@ -86,7 +86,7 @@ EXPORTS = [
     "tbl_baseflow_write", "tbl_baseflow_read", "tbl_poi_ux_dev", "tbl_poi_uz_dev",
     "tbl_wx_wall_dev", "tbl_wz_wall_dev", "tbl_bzone_dev", "tbl_filter_trid_dev",
     "tbl_outuy_dev", "tbl_loop_dev", "tbl_loop", "tbl_bdfc_generate", "tbl_terms_dev",
-    "tbl_drv_gv_dev", "tbl_rk4_step_dev",
+    "tbl_drv_gv_dev", "tbl_drv_gv", "tbl_rk4_step_dev",
 ]
 
 _lib = None
@@ -136,6 +136,7 @@ def lib():
         "tbl_bdfc_generate": ([ip, ip, ip, ip, vp], C.c_int),
         "tbl_terms_dev": ([vp, ip, C.POINTER(CBase)] + [vp] * 10, C.c_int),
         "tbl_drv_gv_dev": ([vp, dp, ip, C.POINTER(CBase)] + [vp] * 9, C.c_int),
+        "tbl_drv_gv": ([vp, dp, ip] + [vp] * 14, C.c_int),
         "tbl_rk4_step_dev": ([vp, dp, dp, ip, C.POINTER(CBase), vp, vp, vp, vp, ip, ip]
                              + [vp] * 7 + [C.POINTER(SolveOpts), vp, vp], C.c_int),
     }
@@ -475,6 +476,23 @@ class Context:
         _check(lib().tbl_drv_gv_dev(self.h, Re, kphys, C.byref(cb),
                                     *[C.c_void_p(x) for x in (ux, uy, uz, wx, wy, wz, dvx, dvy,
                                                               dvz)]), "tbl_drv_gv_dev")
+
+    def drv_gv(self, Re, kphys, base: dict, fields: dict):
+        """drv_gv on host arrays: base is a dict of real (ny, nx) arrays uxb,
+        uyb, wzb, varg, dvargdx; fields of complex (nk, ny, nx) arrays ux ..
+        wz.  Returns dvx, dvy, dvz (row 1 zero: drv_gv does not write it)."""
+        g = self.grid
+        shp = (g.nmodes, g.ny, g.nx)
+        bs = [np.ascontiguousarray(base[k], dtype=np.float64)
+              for k in ("uxb", "uyb", "wzb", "varg", "dvargdx")]
+        fs = [np.ascontiguousarray(fields[k], dtype=np.complex128)
+              for k in ("ux", "uy", "uz", "wx", "wy", "wz")]
+        if any(b.shape != shp[1:] for b in bs) or any(f.shape != shp for f in fs):
+            raise TblError(f"drv_gv: base (ny, nx) and fields {shp} expected")
+        out = [np.zeros(shp, np.complex128) for _ in range(3)]
+        _check(lib().tbl_drv_gv(self.h, Re, kphys, *[_ptr(a) for a in bs + fs + out]),
+               "tbl_drv_gv")
+        return tuple(out)
 
     def rk4_step_dev(self, Re, dt, kphys, base, bdfc, d2uydx2, fields, pert=None, **kw):
         """One RK4 time step on device fields (dict of pointers ux .. wz, th

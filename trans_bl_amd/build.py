@@ -74,7 +74,8 @@ FC = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
 
 def build_fortran(verbose: bool = True) -> str:
     """ISO_C_BINDING module, the drop-in subroutines and the Fortran
-    RK-stage test driver (include path: the test's own par.for/comm.*)."""
+    test drivers rk_stage, rk_loop and rk_drv (include path: the test's own
+    par.for/comm.*)."""
     os.makedirs(FBUILD, exist_ok=True)
     inc = os.path.join(os.path.dirname(HERE), "tests", "fortran")
     steps = [
@@ -92,6 +93,11 @@ def build_fortran(verbose: bool = True) -> str:
         [FC, os.path.join(FBUILD, "rk_loop.o"), os.path.join(FBUILD, "tbl_dropin.o"),
          os.path.join(FBUILD, "tbl_iface.o"), "-L", HERE, "-ltrans_bl_amd",
          "-Wl,-rpath,$ORIGIN/../..", "-o", os.path.join(FBUILD, "rk_loop")],
+        [FC, "-O2", "-ffixed-form", "-I", inc, "-J", FBUILD, "-c",
+         os.path.join(inc, "rk_drv.f"), "-o", os.path.join(FBUILD, "rk_drv.o")],
+        [FC, os.path.join(FBUILD, "rk_drv.o"), os.path.join(FBUILD, "tbl_dropin.o"),
+         os.path.join(FBUILD, "tbl_iface.o"), "-L", HERE, "-ltrans_bl_amd",
+         "-Wl,-rpath,$ORIGIN/../..", "-o", os.path.join(FBUILD, "rk_drv")],
     ]
     for cmd in steps:
         r = subprocess.run(cmd, capture_output=True, text=True)

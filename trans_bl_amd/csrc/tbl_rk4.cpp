@@ -177,6 +177,49 @@ extern "C" int tbl_drv_gv_dev(tbl_ctx *c, double Re, int kphys, const tbl_base *
     return drv_gv(c, Re, kphys, b, f, dv);
 }
 
+extern "C" int tbl_drv_gv(tbl_ctx *c, double Re, int kphys, const double *uxb,
+                          const double *uyb, const double *wzb, const double *varg,
+                          const double *dvargdx, const double *ux, const double *uy,
+                          const double *uz, const double *wx, const double *wy, const double *wz,
+                          double *dvx, double *dvy, double *dvz)
+{
+    if (!c || !uxb || !uyb || !wzb || !varg || !dvargdx || !ux || !uy || !uz || !wx || !wy ||
+        !wz || !dvx || !dvy || !dvz || !(Re > 0.0))
+        return TBL_ERR_ARG;
+    TRY(check_kphys(c, kphys));
+    const size_t n3 = field(c) * 2, n2 = static_cast<size_t>(c->g.nx) * c->g.ny;
+    double *d = nullptr;
+    TBL_HIP(hipMalloc(reinterpret_cast<void **>(&d), sizeof(double) * (9 * n3 + 5 * n2)));
+    const double *hin[6] = {ux, uy, uz, wx, wy, wz};
+    double *hout[3] = {dvx, dvy, dvz};
+    const double *hb[5] = {uxb, uyb, wzb, varg, dvargdx};
+    double *df[9], *db[5];
+    int rc = TBL_OK;
+    for (int f = 0; f < 9; ++f) df[f] = d + f * n3;
+    for (int q = 0; q < 5; ++q) db[q] = d + 9 * n3 + q * n2;
+    // dv in as well: row 1, which drv_gv does not write, keeps the caller's values
+    for (int f = 0; f < 9 && rc == TBL_OK; ++f)
+        if (hipMemcpy(df[f], f < 6 ? hin[f] : hout[f - 6], sizeof(double) * n3,
+                      hipMemcpyHostToDevice) != hipSuccess)
+            rc = TBL_ERR_HIP;
+    for (int q = 0; q < 5 && rc == TBL_OK; ++q)
+        if (hipMemcpy(db[q], hb[q], sizeof(double) * n2, hipMemcpyHostToDevice) != hipSuccess)
+            rc = TBL_ERR_HIP;
+    if (rc == TBL_OK) {
+        const tbl_base b{db[0], db[1], db[2], db[3], db[4]};
+        rc = tbl_drv_gv_dev(c, Re, kphys, &b, df[0], df[1], df[2], df[3], df[4], df[5], df[6],
+                            df[7], df[8]);
+    }
+    if (rc == TBL_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = TBL_ERR_HIP;
+    for (int f = 0; f < 3 && rc == TBL_OK; ++f)
+        if (hipMemcpy(hout[f], df[6 + f], sizeof(double) * n3, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+            rc = TBL_ERR_HIP;
+    if (rc == TBL_ERR_HIP) tbl_set_last_error("tbl_drv_gv: copy failed", __FILE__, __LINE__);
+    (void)hipFree(d);
+    return rc;
+}
+
 extern "C" int tbl_rk4_step_dev(tbl_ctx *c, double Re, double dt, int kphys, const tbl_base *b,
                                 const double *d_bdfc, double *d_d2uydx2,
                                 const double *d_pert_uy, const double *d_pert_d2, int pert_i0,

@@ -4,9 +4,11 @@ c     derivative subroutines, running on an MI355X through libtrans_bl_amd.
 c
 c     Same names and argument lists as the reference (poi_uy,
 c     poisson_par.f:6; derparx, derparxx, dery, deryy, deryfv,
-c     derivs_par.f:80-177; loop, main_par.f:1762-1785), same hidden
-c     inputs: the PARAMETERs of par.for, COMMON /var/, /coefs/, /bd/
-c     (bdfc) and /derw/ (d2uydx2).  Compile this file instead
+c     derivs_par.f:80-177; loop, main_par.f:1762-1785; drv_gv,
+c     main_par.f:1339-1403), same hidden inputs: the PARAMETERs of
+c     par.for, COMMON /var/, /coefs/, /bd/ (bdfc), /derw/ (d2uydx2),
+c     /blas/ (uxb, uyb, wzb) and /vc/ (varg, dvargdx).  Compile this file
+c     instead
 c     of poisson_par.f's poi_uy and derivs_par.f's derivative routines,
 c     with the run's own par.for and comm.* on the include path, and link
 c     tbl_iface.o and libtrans_bl_amd.so.  One rank (np = 1).
@@ -215,6 +217,32 @@ cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
       do k = 1, kfour
         write(*,*) k, cycles(k), residual(k)
       end do
+
+      return
+      end
+
+cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
+      subroutine drv_gv(dvx, dvy, dvz)
+
+      ! the RK right-hand side (main_par.f:1339-1403) with lterms_gv, the
+      ! reference's active call: dvx, dvy, dvz on rows 2..jmax from
+      ! COMMON /var/, the base flow /blas/ and curvature term /vc/
+      use tbl_iface
+      implicit none
+      include 'par.for'
+      include 'comm.var'
+      real*8 uxb(ptsx,jmax), uyb(ptsx,jmax), wzb(ptsx,jmax)
+      real*8 varg(ptsx,jmax), dvargdx(ptsx,jmax)
+      complex*16 dvx(ptsx,jmax,kfour), dvy(ptsx,jmax,kfour),
+     &           dvz(ptsx,jmax,kfour)
+      common/blas/ uxb, uyb, wzb
+      common/vc/ varg, dvargdx
+      integer(c_int) rc
+
+      call tbl_setup
+      rc = tbl_drv_gv(tbl_ctx_handle, Re, 0, uxb, uyb, wzb, varg,
+     &                dvargdx, ux, uy, uz, wx, wy, wz, dvx, dvy, dvz)
+      if (rc .ne. 0) call tbl_fail('drv_gv', rc)
 
       return
       end

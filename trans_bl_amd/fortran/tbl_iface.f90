@@ -127,6 +127,18 @@ module tbl_iface
       integer(c_int) :: rc
     end function
 
+    function tbl_drv_gv(ctx, Re, kphys, uxb, uyb, wzb, varg, dvargdx, ux, uy, uz, wx, wy, wz, &
+        dvx, dvy, dvz) bind(C, name="tbl_drv_gv") result(rc)
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      real(c_double), value :: Re
+      integer(c_int), value :: kphys
+      real(c_double), intent(in) :: uxb(*), uyb(*), wzb(*), varg(*), dvargdx(*)
+      complex(c_double_complex), intent(in) :: ux(*), uy(*), uz(*), wx(*), wy(*), wz(*)
+      complex(c_double_complex), intent(inout) :: dvx(*), dvy(*), dvz(*)
+      integer(c_int) :: rc
+    end function
+
   end interface
 
   type(c_ptr), save :: tbl_ctx_handle = c_null_ptr
