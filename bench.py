@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # (rocprofv3 --pmc ... -- python3 -m trans_bl_amd.prof_kernels, summarized by
 # trans_bl_amd.prof_kernels --summarize)
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_1025x193x64.json")
-ROWS = (1, 2, 3, 4, 5, 7, 9, 13, 17)
+ROWS = (1, 2, 3, 4, 5, 6, 7, 9, 13, 17)
 
 
 def smoother_kernel(prec, lanes, ny):
@@ -51,9 +51,9 @@ def smoother_kernel(prec, lanes, ny):
     rocprofv3 prints it): G direct, -G staged (x terms, v, S in LDS), 2000+G
     staged (x terms only), 1000+G fused-sweep strip.  The staged kernel's
     last template argument selects its updating form, which plain sweeps do
-    not use."""
+    not use.  A line's lanes hold its rows 1..ny-1 (row 0 is not relaxed)."""
     g = lanes - 2000 if lanes > 2000 else lanes - 1000 if lanes > 1000 else abs(lanes)
-    need = -(-ny // g)
+    need = -(-(ny - 1) // g)
     r = next(x for x in ROWS if x >= need)
     t = "float" if prec else "double"
     if lanes > 2000:

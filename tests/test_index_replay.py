@@ -166,3 +166,51 @@ def test_derivative_kernels_stay_in_bounds(nx, ny, K):
         G, R, lds_max, lds_n, gmax, n = _y_kernel_reach(nx, ny, K)
         assert lds_max < lds_n and gmax < n and G * R >= ny
         assert lds_n * 16 <= 160 * 1024
+
+
+# ---- line smoother rows (tbl_relax.h, tbl_smooth_phase.hip, tbl_mg_kernels.hip)
+K_RS = (1, 2, 3, 4, 5, 6, 7, 9, 13, 17)    # kRs
+
+
+def _line_rows_for(G, ny):
+    """line_rows_for: rows per lane for rows 1..ny-1 (row 0 is not relaxed)."""
+    need = (ny - 1 + G - 1) // G
+    for r in K_RS:
+        if r < need:
+            continue
+        return -1 if G == 32 and r > 9 else r
+    return -1
+
+
+def _lu_stride(ny):
+    s = ny
+    for G in (16, 32, 64):
+        R = _line_rows_for(G, ny)
+        if R > 0 and 1 + G * R > s:
+            s = 1 + G * R
+    return (s + 7) // 8 * 8
+
+
+@pytest.mark.parametrize("ny", [9, 13, 17, 25, 33, 49, 65, 97, 129, 193, 257, 385, 513, 577])
+@pytest.mark.parametrize("G", [16, 32, 64])
+def test_line_rows_cover_rows_1_to_ny_once(G, ny):
+    """relax_core's lanes own rows line_j0(lane, R) = 1 + lane R .. + R - 1:
+    together rows 1..ny-1, each once; the staged kernels' lane-strided
+    loads (rows 1 + lane + s G, s < R) reach the same rows; the LU-table
+    reads stay inside the padded table, the x-term reads inside the line."""
+    R = _line_rows_for(G, ny)
+    if R < 0:
+        return
+    lane = np.arange(G)
+    rows = (1 + lane[:, None] * R + np.arange(R)[None, :]).ravel()
+    rel_rows = rows[rows < ny]
+    assert np.array_equal(np.sort(rel_rows), np.arange(1, ny))
+    staged = (1 + lane[:, None] + G * np.arange(R)[None, :]).ravel()
+    assert np.array_equal(np.sort(staged[staged < ny]), np.arange(1, ny))
+    assert rows.max() < _lu_stride(ny)              # identity rows past ny
+    xr = (rows[:, None] + np.array([-1, 0, 1])[None, :]).ravel()
+    xr = xr[(xr >= 0) & (xr < ny)]
+    assert xr.min() >= 0 and xr.max() <= ny - 1
+    # 193 = 1 + 64 * 3: no idle lane slot at the benchmark's finest level
+    if ny == 193 and G == 64:
+        assert R == 3

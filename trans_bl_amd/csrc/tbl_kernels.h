@@ -37,8 +37,9 @@ inline Level make_level(int nx, int ny, double dx2)
 int line_rows_for(int G, int ny);
 
 // Row stride of a level's line-LU tables: at least ny and every line shape's
-// G*R, so a lane's R rows are always inside the table; rows past ny hold the
-// identity (zero factors, unit pivot and 1/pivot).
+// 1 + G*R (its lanes hold rows 1..G*R), so a lane's R rows are always inside
+// the table; rows past ny hold the identity (zero factors, unit pivot and
+// 1/pivot).
 int lu_stride(int ny);
 
 // Fused-sweep strip smoother: line shape kStripShape + G selects it (G lanes

@@ -452,7 +452,7 @@ __global__ void __launch_bounds__(256) convert_kernel(size_t n, const cx<TA> *a,
 // launchers
 // ---------------------------------------------------------------------------
 
-static const int kRs[] = {1, 2, 3, 4, 5, 7, 9, 13, 17};
+static const int kRs[] = {1, 2, 3, 4, 5, 6, 7, 9, 13, 17};
 
 // grid of the point-wise level kernels: 32 rows x 8 columns per block
 static dim3 grid_2d(const Level &l, int K)
@@ -477,7 +477,8 @@ int set_kernel_attributes()
 
 int line_rows_for(int G, int ny)
 {
-    const int need = (ny + G - 1) / G;
+    // rows 1..ny-1 (row 0 is not relaxed)
+    const int need = (ny - 1 + G - 1) / G;
     for (int r : kRs) {
         if (r < need) continue;
         if (G == 32 && r > 9) return -1;
@@ -491,7 +492,7 @@ int lu_stride(int ny)
     int s = ny;
     for (int G : {16, 32, 64}) {
         const int R = line_rows_for(G, ny);
-        if (R > 0 && G * R > s) s = G * R;
+        if (R > 0 && 1 + G * R > s) s = 1 + G * R;
     }
     return (s + 7) / 8 * 8;
 }

@@ -306,16 +306,27 @@ __device__ __forceinline__ void scan_down(Aff<T> &m, int lane)
     }
 }
 
+// Rows of a y-line held by its lanes: row 0 (the wall value, an identity
+// row of the line system) is not relaxed, so lane t owns rows
+// line_j0(t, R) .. + R - 1 of rows 1..ny-1 and a shape needs G R >= ny - 1
+// (ny = 1 + 2^k m leaves no idle lane slot where G divides 2^k m).  Loops
+// that stage a line lane-strided (row lane + s G) take R + 1 steps, which
+// reach row ny - 1.
+__host__ __device__ inline int line_j0(int lane, int R) { return 1 + lane * R; }
+
 // The line's data is reached through accessors: getX(j) = off-diagonal x
-// term of row j, getV(j) = current v, getS(j, r) = source of row j = lane*R + r,
-// putV(j, v) = store.
+// term of row j, getV(j) = current v, getS(j, r) = source of row
+// j = line_j0(lane, R) + r, putV(j, v) = store.
 template <typename T, int G, int R, class FX, class FV, class FS, class FW>
 __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const LineLU<T, R> &f, T rf,
                                            int lane, FX getX, FV getV, FS getS, FW putV)
 {
     const int ny = L.ny;
-    const int j0 = lane * R;
+    const int j0 = line_j0(lane, R);
     const T qx = L.qx;
+    // row 0's solution is its own value (identity row: y_0 = x_0 = v_0); it
+    // enters the rows above as the forward substitution's initial state
+    const cx<T> v0 = getV(0);
 
     cx<T> vo[R], y[R];
     // x off-diagonal terms for rows j0-1 .. j0+R
@@ -334,9 +345,7 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const LineLU<T
         cx<T> v = {T(0), T(0)}, rhs = {T(0), T(0)};
         if (j < ny) {
             v = getV(j);
-            if (j == 0) {
-                rhs = v;
-            } else if (j == 1) {
+            if (j == 1) {
                 // S - qx (sp2 X(j+1) + sp1 X(j))
                 rhs = mac(-qx, dot2(L.sp[2], X[r + 2], L.sp[1], X[r + 1]), getS(j, r));
             } else if (j <= ny - 3) {
@@ -363,11 +372,17 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const LineLU<T
         const cx<T> nc1 = mac(-a1, m.c1, mac(-a2, m.c0, y[r]));
         m.c0 = m.c1; m.c1 = nc1;
     }
+    // the line's initial state (y_{-1}, y_0) = (0, v_0) enters through the
+    // first lane's map: c += P (0, v_0)
+    if (lane == 0) {
+        m.c0 = mac(m.p01, v0, m.c0);
+        m.c1 = mac(m.p11, v0, m.c1);
+    }
     scan_up<T, G>(m, lane);
     {
         // state entering this lane's rows: the previous lane's inclusive map
         cx<T> a = lane_prev(m.c0), b = lane_prev(m.c1);
-        if (lane == 0) { a = {0, 0}; b = {0, 0}; }
+        if (lane == 0) { a = {0, 0}; b = v0; }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const T a1 = f.l1[r], a2 = f.l2[r];
@@ -411,7 +426,7 @@ __device__ __forceinline__ void relax_core(const LevelArgs<T> &L, const LineLU<T
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int j = j0 + r;
-        if (j >= 1 && j < ny) putV(j, mac(rf, y[r] - vo[r], vo[r]));
+        if (j < ny) putV(j, mac(rf, y[r] - vo[r], vo[r]));
     }
 }
 
@@ -422,7 +437,7 @@ __device__ __forceinline__ void relax_line(const LevelArgs<T> &L, cx<T> *col,
                                            int side, T rf, int lane)
 {
     const int ny = L.ny, ld = L.ld;
-    const LineLU<T, R> f = load_line_lu_tbl<T, R>(lu, L.lus, lane * R);
+    const LineLU<T, R> f = load_line_lu_tbl<T, R>(lu, L.lus, line_j0(lane, R));
     relax_core<T, G, R>(
         L, f, rf, lane,
         [&](int j) { return xoff<T>(col, ld, j, side, side == 0 && j >= ny - 2); },
@@ -447,9 +462,10 @@ __host__ __device__ inline int phase_lines(int phase, int nx)
 
 // (G, R) table shared by both smoother forms
 #define TBL_LINE_SHAPES(X) \
-    X(16, 1) X(16, 2) X(16, 3) X(16, 4) X(16, 5) X(16, 7) X(16, 9) X(16, 13) X(16, 17) \
-    X(32, 1) X(32, 2) X(32, 3) X(32, 4) X(32, 5) X(32, 7) X(32, 9) \
-    X(64, 1) X(64, 2) X(64, 3) X(64, 4) X(64, 5) X(64, 7) X(64, 9) X(64, 13) X(64, 17)
+    X(16, 1) X(16, 2) X(16, 3) X(16, 4) X(16, 5) X(16, 6) X(16, 7) X(16, 9) X(16, 13) \
+    X(16, 17) \
+    X(32, 1) X(32, 2) X(32, 3) X(32, 4) X(32, 5) X(32, 6) X(32, 7) X(32, 9) \
+    X(64, 1) X(64, 2) X(64, 3) X(64, 4) X(64, 5) X(64, 6) X(64, 7) X(64, 9) X(64, 13) X(64, 17)
 
 inline unsigned nblk(size_t n, unsigned b = 256) { return static_cast<unsigned>((n + b - 1) / b); }
 

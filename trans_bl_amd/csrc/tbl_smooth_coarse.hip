@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(512) smooth_coarsest_kernel(LevelArgs<T> L, cx
                 cx<T> *col = sv + static_cast<size_t>(i) * ls;
                 const cx<T> *scol = gs + static_cast<size_t>(i) * ld;
                 const LineLU<T, R> f =
-                    load_line_lu<T, R>(slu + (side ? kLuFields * ny : 0), ny, lane * R, ny);
+                    load_line_lu<T, R>(slu + (side ? kLuFields * ny : 0), ny, line_j0(lane, R), ny);
                 relax_core<T, G, R>(
                     Ls, f, rf, lane,
                     [&](int j) { return xoff<T>(col, ls, j, side, side == 0 && j >= ny - 2); },

@@ -33,6 +33,8 @@ constexpr int kXcds = 8;          // MI355X: 8 XCDs, one L2 each
 // of the LDS, so more lines in flight per CU); the scan then works on
 // lane-contiguous segments; with FULL the relaxed v goes back through LDS
 // and is stored lane-strided.  Arithmetic identical to smooth_phase_kernel.
+// The stage holds rows 1 + lane + s G (s < R: rows 1..G R, the rows the
+// lanes relax) and, with FULL, v of row 0 (the line's initial state).
 // UPD: vup (the FP64 field of the same layout) += the relaxed v on the
 // line's rows 1.., its values loaded with the line's other inputs
 template <typename T, int G, int R, bool FULL, bool UPD = false>
@@ -105,15 +107,17 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
         if constexpr (UPD) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const int j = lane + r * G;
-                if (j >= 1 && j < ny) up0[r] = vup[base + j];
+                const int j = 1 + lane + r * G;
+                if (j < ny) up0[r] = vup[base + j];
             }
         }
+        // row 0's v (lane 0's row-0 offset is lo = bv)
+        const cx<T> v0 = FULL && lane == 0 ? buf_load(vr, bv, T(0)) : cx<T>{T(0), T(0)};
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             // rows past ny (up to G*R) read zero, not the next column
-            const bool in = lane + r * G < ny;
-            const unsigned rs = r * G * ES;
+            const bool in = 1 + lane + r * G < ny;
+            const unsigned rs = (1 + r * G) * ES;
             auto at = [&](unsigned b) { return in ? b + rs : kBufOff; };
             a0[r] = buf_load(vr, at(b0), T(0));
             a1[r] = buf_load(vr, at(b1), T(0));
@@ -125,9 +129,13 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
                 ss[r] = buf_load(sr, at(lo), T(0));
             }
         }
+        if (lane == 0) {
+            if (FULL) Vs[0] = v0;
+            Xs[0] = cx<T>{T(0), T(0)};   // read by relax_core, used by no row
+        }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int j = lane + r * G;
+            const int j = 1 + lane + r * G;
             if (j < ny) {
                 cx<T> x;
                 // the sums of xoff in its order (multiply-adds in FP32)
@@ -152,7 +160,7 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
             buf_rsrc(lu, static_cast<size_t>(L.K) * 2 * kLuFields * L.lus * sizeof(T));
         const unsigned lo = static_cast<unsigned>((static_cast<size_t>(k) * 2 + (side != 0)) *
                                                   kLuFields * L.lus * sizeof(T));
-        const LineLU<T, R> f = load_line_lu_buf<T, R>(lr, lo, L.lus, lane * R);
+        const LineLU<T, R> f = load_line_lu_buf<T, R>(lr, lo, L.lus, line_j0(lane, R));
         if (FULL) {
             relax_core<T, G, R>(L, f, rf, lane, [&](int j) { return Xs[j]; },
                                 [&](int j) { return Vs[j]; }, [&](int j, int) { return Ss[j]; },
@@ -171,8 +179,8 @@ __global__ void __launch_bounds__(256) smooth_staged_kernel(LevelArgs<T> L, cx<T
             cx<T> *col = V + base;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const int j = lane + r * G;
-                if (j >= 1 && j < ny) {
+                const int j = 1 + lane + r * G;
+                if (j < ny) {
                     const cx<T> e = Vs[j];
                     // the updating sweep's last colour is read by no later
                     // phase, and the next correction solve starts from zero

@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(512) smooth_strip_kernel(LevelArgs<T> L, cx<T>
     }
     __syncthreads();
     const int grp = threadIdx.x / G, lane = threadIdx.x % G, ngrp = blockDim.x / G;
-    const int j0 = lane * R;
+    const int j0 = line_j0(lane, R);
     // lines of each phase: first column, count, column step
     int i0s[4], cnts[4], steps[4];
 #pragma unroll

@@ -60,7 +60,8 @@ __device__ __forceinline__ void wave_lds_sync()
 // i+1, i+2), its own old values and its source, R rows per lane.
 template <typename T, int R, bool UPD>
 struct SweepIn {
-    cx<T> a0[R], a1[R], a2[R], a3[R], a4[R], vv[R], ss[R];
+    // rows 1 + lane + r G (the rows the lanes relax); v0: row 0's v (lane 0)
+    cx<T> a0[R], a1[R], a2[R], a3[R], a4[R], vv[R], ss[R], v0;
     cx<double> up0[UPD ? R : 1];
 };
 
@@ -95,15 +96,17 @@ __device__ __forceinline__ void sweep_load(const LevelArgs<T> &L, __amdgpu_buffe
         const size_t col = static_cast<size_t>(i) * ld;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int j = lane + r * G;
-            if (own && j >= 1 && j < ny) in.up0[r] = upp[col + j];
+            const int j = 1 + lane + r * G;
+            if (own && j < ny) in.up0[r] = upp[col + j];
         }
     }
+    // row 0 (lane 0's offset bv): the line's initial state, copied to Vout
+    in.v0 = buf_load(vr, lane == 0 ? bv : kBufOff, T(0));
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         // rows past ny (up to G*R) read zero, not the next column
-        const bool inr = lane + r * G < ny;
-        const unsigned rs = r * G * ES;
+        const bool inr = 1 + lane + r * G < ny;
+        const unsigned rs = (1 + r * G) * ES;
         auto at = [&](unsigned b) { return inr ? b + rs : kBufOff; };
         in.a0[r] = buf_load(vr, at(b0), T(0));
         in.a1[r] = buf_load(vr, at(b1), T(0));
@@ -127,9 +130,13 @@ __device__ __forceinline__ void sweep_relax(const LevelArgs<T> &L, __amdgpu_buff
     const int ny = L.ny, ld = L.ld, nx = L.nx;
     const int ph = sweep_phase_of(i, nx);
     const int side = ph == 0 ? (i == 1 ? -1 : 1) : 0;
+    if (lane == 0) {
+        Vs[0] = in.v0;
+        Xs[0] = cx<T>{T(0), T(0)};   // read by relax_core, used by no row
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int j = lane + r * G;
+        const int j = 1 + lane + r * G;
         if (j < ny) {
             cx<T> x;
             // the sums of xoff in its order (multiply-adds in FP32)
@@ -150,21 +157,22 @@ __device__ __forceinline__ void sweep_relax(const LevelArgs<T> &L, __amdgpu_buff
     wave_lds_sync();
     const unsigned luo = static_cast<unsigned>((static_cast<size_t>(k) * 2 + (side != 0)) *
                                                kLuFields * L.lus * sizeof(T));
-    const LineLU<T, R> f = load_line_lu_buf<T, R>(lr, luo, L.lus, lane * R);
+    const LineLU<T, R> f = load_line_lu_buf<T, R>(lr, luo, L.lus, line_j0(lane, R));
     relax_core<T, G, R>(L, f, rf, lane, [&](int j) { return Xs[j]; }, [&](int j) { return Vs[j]; },
                         [&](int j, int) { return Ss[j]; }, [&](int j, cx<T> v) { Vs[j] = v; });
     wave_lds_sync();
     const size_t col = static_cast<size_t>(i) * ld;
     cx<T> *o = outp + col;
+    // row 0 (the wall value) is not relaxed: Vout takes Vin's
+    if (lane == 0 && !(UPD && ph == 3)) o[0] = in.v0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int j = lane + r * G;
+        const int j = 1 + lane + r * G;
         if (j < ny) {
-            // row 0 (the wall value) is not relaxed: Vout takes Vin's
-            const cx<T> e = j >= 1 ? Vs[j] : in.vv[r];
+            const cx<T> e = Vs[j];
             if (!(UPD && ph == 3)) o[j] = e;
             if constexpr (UPD) {
-                if (own && j >= 1)
+                if (own)
                     upp[col + j] = cx<double>{in.up0[r].re + static_cast<double>(e.re),
                                               in.up0[r].im + static_cast<double>(e.im)};
             }
