@@ -247,11 +247,36 @@ __device__ __forceinline__ cx<T> dpp(cx<T> id, cx<T> x)
 {
     return {dpp<CTRL>(id.re, x.re), dpp<CTRL>(id.im, x.im)};
 }
-template <int CTRL, typename T>
-__device__ __forceinline__ Aff<T> dpp(const Aff<T> &id, const Aff<T> &x)
+// Same with zero for lanes without a source (the move's bound-control
+// form: no register to preset, and the move can merge into its user)
+template <int CTRL>
+__device__ __forceinline__ float dpp0(float x)
 {
-    return {dpp<CTRL>(id.p00, x.p00), dpp<CTRL>(id.p01, x.p01), dpp<CTRL>(id.p10, x.p10),
-            dpp<CTRL>(id.p11, x.p11), dpp<CTRL>(id.c0, x.c0),   dpp<CTRL>(id.c1, x.c1)};
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x),
+                                                                 CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp0(double x)
+{
+    const unsigned long long v = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_update_dpp(
+        0, static_cast<int>(static_cast<unsigned>(v)), CTRL, 0xF, 0xF, true));
+    const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_update_dpp(
+        0, static_cast<int>(static_cast<unsigned>(v >> 32)), CTRL, 0xF, 0xF, true));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+template <int CTRL, typename T>
+__device__ __forceinline__ cx<T> dpp0(cx<T> x)
+{
+    return {dpp0<CTRL>(x.re), dpp0<CTRL>(x.im)};
+}
+// The map of the source lane, the identity where there is none: the zero
+// entries of the identity (p01, p10, c) through the zero-filling form
+template <int CTRL, typename T>
+__device__ __forceinline__ Aff<T> dpp_aff(const Aff<T> &x)
+{
+    return {dpp<CTRL>(T(1), x.p00), dpp0<CTRL>(x.p01), dpp0<CTRL>(x.p10),
+            dpp<CTRL>(T(1), x.p11), dpp0<CTRL>(x.c0),  dpp0<CTRL>(x.c1)};
 }
 template <typename T>
 __device__ __forceinline__ Aff<T> aff_shfl(const Aff<T> &x, int src, int width)
@@ -263,9 +288,9 @@ __device__ __forceinline__ Aff<T> aff_shfl(const Aff<T> &x, int src, int width)
 // value of lane - 1 / lane + 1 (the caller replaces it on a group's first /
 // last lane)
 template <typename T>
-__device__ __forceinline__ cx<T> lane_prev(cx<T> x) { return dpp<0x138>(cx<T>{T(0), T(0)}, x); }
+__device__ __forceinline__ cx<T> lane_prev(cx<T> x) { return dpp0<0x138>(x); }
 template <typename T>
-__device__ __forceinline__ cx<T> lane_next(cx<T> x) { return dpp<0x130>(cx<T>{T(0), T(0)}, x); }
+__device__ __forceinline__ cx<T> lane_next(cx<T> x) { return dpp0<0x130>(x); }
 
 // Inclusive scan of the lanes' maps in lane order (lane t ends with the map
 // of lanes 0..t): four DPP steps inside each 16-lane row, then the rows are
@@ -273,11 +298,10 @@ __device__ __forceinline__ cx<T> lane_next(cx<T> x) { return dpp<0x130>(cx<T>{T(
 template <typename T, int G>
 __device__ __forceinline__ void scan_up(Aff<T> &m, int lane)
 {
-    const Aff<T> id = aff_identity<T>();
-    aff_fold(m, dpp<0x111>(id, m));
-    aff_fold(m, dpp<0x112>(id, m));
-    aff_fold(m, dpp<0x114>(id, m));
-    aff_fold(m, dpp<0x118>(id, m));
+    aff_fold(m, dpp_aff<0x111>(m));
+    aff_fold(m, dpp_aff<0x112>(m));
+    aff_fold(m, dpp_aff<0x114>(m));
+    aff_fold(m, dpp_aff<0x118>(m));
     if (G >= 32) {
         const Aff<T> q = aff_shfl(m, (lane & 32) + 15, G);   // end of the row below
         if (lane & 16) aff_fold(m, q);
@@ -291,11 +315,10 @@ __device__ __forceinline__ void scan_up(Aff<T> &m, int lane)
 template <typename T, int G>
 __device__ __forceinline__ void scan_down(Aff<T> &m, int lane)
 {
-    const Aff<T> id = aff_identity<T>();
-    aff_fold(m, dpp<0x101>(id, m));
-    aff_fold(m, dpp<0x102>(id, m));
-    aff_fold(m, dpp<0x104>(id, m));
-    aff_fold(m, dpp<0x108>(id, m));
+    aff_fold(m, dpp_aff<0x101>(m));
+    aff_fold(m, dpp_aff<0x102>(m));
+    aff_fold(m, dpp_aff<0x104>(m));
+    aff_fold(m, dpp_aff<0x108>(m));
     if (G >= 32) {
         const Aff<T> q = aff_shfl(m, (lane & 32) + 16, G);   // start of the row above
         if (!(lane & 16)) aff_fold(m, q);
