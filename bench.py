@@ -79,6 +79,22 @@ def measured_traffic(kernel):
             return e.get("sweep_bytes_per_launch",
                          e.get("derivs_bytes_per_launch", e["hbm_bytes_per_launch"]))
     return None
+
+
+def traffic_source():
+    """Where `traffic` comes from: the counter summary and whether it was
+    read from this same library build (its sha256 recorded at profiling)."""
+    import hashlib
+    try:
+        with open(TRAFFIC_JSON) as f:
+            rec = json.load(f).get("library_sha256")
+        with open(os.path.join(ROOT, "trans_bl_amd", "libtrans_bl_amd.so"), "rb") as f:
+            cur = hashlib.sha256(f.read()).hexdigest()
+    except (OSError, ValueError):
+        return None
+    return {"file": os.path.relpath(TRAFFIC_JSON, ROOT),
+            "counters": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of trans_bl_amd.prof_kernels",
+            "same_library_build": rec is not None and rec == cur}
 METRIC = "v-Poisson mode-solves/s to 1e-9 residual; learned+MG vs plain MG on 1 GPU"
 
 
@@ -437,6 +453,7 @@ def main():
         "roofline": roof[1],
         "roofline_fp64_smoother": roof[0],
         "roofline_derivs": roof_d,
+        "traffic_source": traffic_source(),
         "rk4_step": rk4,
         "cpu_baseline": cpu,
     }

@@ -94,6 +94,17 @@ def main():
     ctx.close()
 
 
+def library_sha256():
+    """sha256 of the built HIP library (None when it is missing)."""
+    import hashlib
+    path = os.path.join(ROOT, "trans_bl_amd", "libtrans_bl_amd.so")
+    try:
+        with open(path, "rb") as f:
+            return hashlib.sha256(f.read()).hexdigest()
+    except OSError:
+        return None
+
+
 def _counter_rows(directory, counter):
     import csv
     import glob
@@ -126,7 +137,11 @@ def summarize(fetch_dir, write_dir, out):
         for _, name, v in rows:
             e = table.setdefault(name, {"fetch_kib": [], "write_kib": []})
             e[key + "_kib"].append(v * scale)
-    res = {"fetch_scale": fs, "write_scale": ws, "kernels": {}}
+    res = {"fetch_scale": fs, "write_scale": ws, "kernels": {},
+           # the library the counters were read from (run --summarize on the
+           # machine and tree of the counter runs): bench.py says whether its
+           # own library is the same build
+           "library_sha256": library_sha256()}
     for name, e in table.items():
         if not e["fetch_kib"] or not e["write_kib"]:
             continue
