@@ -287,6 +287,27 @@ int tbl_drv_gv(tbl_ctx *ctx, double Re, int kphys, const double *uxb, const doub
                const double *wzb, const double *varg, const double *dvargdx, const double *ux,
                const double *uy, const double *uz, const double *wx, const double *wy,
                const double *wz, double *dvx, double *dvy, double *dvz);
+/* Flat-plate form (my_form 0, main_par.f:155-230).  kphys 0: lterms_fi
+   (nlterms_par.f:6-38; base->uxb, uyb, wzb used); otherwise nlterms_fi
+   (41-86, the reference's active call; base may be NULL) with kphys points
+   in z (a power of two, >= 2 nmodes): a, b, c out. */
+int tbl_terms_fi_dev(tbl_ctx *ctx, int kphys, const tbl_base *base, const double *d_ux,
+                     const double *d_uy, const double *d_uz, const double *d_wx,
+                     const double *d_wy, const double *d_wz, double *d_a, double *d_b,
+                     double *d_c);
+/* drv_fi (main_par.f:1264-1336): dvx, dvy, dvz on rows 2..ny (row 1 not
+   written); base as tbl_terms_fi_dev. */
+int tbl_drv_fi_dev(tbl_ctx *ctx, double Re, int kphys, const tbl_base *base,
+                   const double *d_ux, const double *d_uy, const double *d_uz,
+                   const double *d_wx, const double *d_wy, const double *d_wz, double *d_dvx,
+                   double *d_dvy, double *d_dvz);
+/* Same with host arrays (the Fortran drop-in's drv_fi): uxb, uyb, wzb real
+   (nx, ny), used (and required) for kphys 0 only; row 1 of the results
+   keeps the caller's values. */
+int tbl_drv_fi(tbl_ctx *ctx, double Re, int kphys, const double *uxb, const double *uyb,
+               const double *wzb, const double *ux, const double *uy, const double *uz,
+               const double *wx, const double *wy, const double *wz, double *dvx, double *dvy,
+               double *dvz);
 /* One RK4 time step (main_par.f:266-349): four drv_gv + update stages
    (rows 2..ny-1, columns 3..nx), each followed by loop (erro 1e-5, 1e-5,
    1e-5, 1e-6).  The disturbance of gv_pert (main_par.f:1577-1610) before

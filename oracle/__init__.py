@@ -114,6 +114,10 @@ def lib():
         L.or_terms.restype = C.c_int
         L.or_drv_gv.argtypes = [P, C.c_double, C.c_int, B] + [vp] * 9
         L.or_drv_gv.restype = C.c_int
+        L.or_terms_fi.argtypes = [P, C.c_int, B] + [vp] * 9
+        L.or_terms_fi.restype = C.c_int
+        L.or_drv_fi.argtypes = [P, C.c_double, C.c_int, B] + [vp] * 9
+        L.or_drv_fi.restype = C.c_int
         L.or_rk4_step.argtypes = [P, C.c_double, C.c_double, C.c_int, B, vp, vp, vp, vp,
                                   C.c_int, C.c_int, vp, C.c_int, vp]
         L.or_rk4_step.restype = C.c_int
@@ -477,6 +481,29 @@ def drv_gv(p, Re, kphys, base, F):
     out = [np.zeros(shp, np.complex128) for _ in range(3)]
     _rc(lib().or_drv_gv(C.byref(p), Re, kphys, C.byref(b), *[_vp(a) for a in f + out]),
         "or_drv_gv")
+    return tuple(out)
+
+
+def terms_fi(p, kphys, base, F):
+    """lterms_fi (kphys 0, base needed) or nlterms_fi (base may be None):
+    returns (a, b, c)."""
+    shp = _field(p)
+    b, keep = _base(p, base) if base is not None else (None, None)
+    f = [_cf(F[k], shp) for k in ("ux", "uy", "uz", "wx", "wy", "wz")]
+    out = [np.zeros(shp, np.complex128) for _ in range(3)]
+    _rc(lib().or_terms_fi(C.byref(p), kphys, C.byref(b) if b is not None else None,
+                          *[_vp(a) for a in f + out]), "or_terms_fi")
+    return tuple(out)
+
+
+def drv_fi(p, Re, kphys, base, F):
+    """drv_fi (main_par.f:1264-1336): returns (dvx, dvy, dvz); row 1 zero."""
+    shp = _field(p)
+    b, keep = _base(p, base) if base is not None else (None, None)
+    f = [_cf(F[k], shp) for k in ("ux", "uy", "uz", "wx", "wy", "wz")]
+    out = [np.zeros(shp, np.complex128) for _ in range(3)]
+    _rc(lib().or_drv_fi(C.byref(p), Re, kphys, C.byref(b) if b is not None else None,
+                        *[_vp(a) for a in f + out]), "or_drv_fi")
     return tuple(out)
 
 

@@ -3,9 +3,10 @@ c     Driver of the reference's right-hand side and RK4 step (test
 c     infrastructure, built by oracle/reference.py into oracle/_ref/).
 c
 c     Linked with the reference objects compiled from their sources
-c     (drv_gv main_par.f:1339-1403, lterms_gv / nlterms_gv with f_to_p,
-c     p_to_f, four1 nlterms_par.f, gv_pert main_par.f:1577-1610, loop and
-c     everything it calls) and a par.for configured for one workload.
+c     (drv_gv main_par.f:1339-1403, drv_fi 1264-1336, lterms_gv /
+c     nlterms_gv / lterms_fi / nlterms_fi with f_to_p, p_to_f, four1
+c     nlterms_par.f, gv_pert main_par.f:1577-1610, loop and everything it
+c     calls) and a par.for configured for one workload.
 c     Set-up as init_gv does it for these routines at one rank (see
 c     ref_loop.f); the RK4 update loops are the four stages of the
 c     Goertler case, main_par.f:270-341, written out here around the
@@ -18,6 +19,9 @@ c            (ptsx,jmax)), d2uydx2 (complex*16 (ptsx,kfour)), bdfc
 c            (real*8 (2,imax)), fcx2, fcllx2 (real*8 (i2)), dt (real*8)
 c       WHAT drv     = drv_gv (dvx, dvy, dvz out)
 c            nlterms = nlterms_gv (a, b, c, d out)
+c            drvfi   = drv_fi (dvx, dvy, dvz out)
+c            nltfi   = nlterms_fi (a, b, c out)
+c            ltfi    = lterms_fi (a, b, c out)
 c            rk4     = one time step t = T (ux .. wz and d2uydx2 out)
 c     stdout: a line 'params Re i1 i2 stpp', then poi_uy's lines.
 cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
@@ -104,6 +108,15 @@ cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
       else if (trim(what) .eq. 'nlterms') then
         call nlterms_gv(dv1x, dv1y, dv1z, e4)
         write(3) dv1x, dv1y, dv1z, e4
+      else if (trim(what) .eq. 'drvfi') then
+        call drv_fi(dv1x, dv1y, dv1z)
+        write(3) dv1x, dv1y, dv1z
+      else if (trim(what) .eq. 'nltfi') then
+        call nlterms_fi(dv1x, dv1y, dv1z)
+        write(3) dv1x, dv1y, dv1z
+      else if (trim(what) .eq. 'ltfi') then
+        call lterms_fi(dv1x, dv1y, dv1z)
+        write(3) dv1x, dv1y, dv1z
       else if (trim(what) .eq. 'rk4') then
         i_ini = 3
         dt2 = 0.5d0 * dt_run

@@ -360,7 +360,8 @@ BASE_FIELDS = ("uxb", "uyb", "wzb", "varg", "dvargdx")
 
 def run_rhs(name: str, fields: dict, base: dict, d2uydx2, bdfc, fcx2, fcllx2, dt: float,
             what: str, t: int = 1, timeout: float = 600.0):
-    """drv_gv ("drv": dvx, dvy, dvz), nlterms_gv ("nlterms": a, b, c, d) or
+    """drv_gv ("drv": dvx, dvy, dvz), nlterms_gv ("nlterms": a, b, c, d),
+    drv_fi ("drvfi"), nlterms_fi ("nltfi": a, b, c), lterms_fi ("ltfi") or
     one RK4 time step at step t ("rk4": ux .. wz and d2uydx2) by the
     reference's code (ref_rhs.f).  fields: complex (nk, ny, nx) ux .. wz;
     base: real (ny, nx) uxb, uyb, wzb, varg, dvargdx; fcx2, fcllx2: the
@@ -393,6 +394,8 @@ def run_rhs(name: str, fields: dict, base: dict, d2uydx2, bdfc, fcx2, fcllx2, dt
         b = np.fromfile(fout, dtype="<c16")
         n3 = nk * ny * nx
         names = {"drv": ("dvx", "dvy", "dvz"), "nlterms": ("a", "b", "c", "d"),
+                 "drvfi": ("dvx", "dvy", "dvz"), "nltfi": ("a", "b", "c"),
+                 "ltfi": ("a", "b", "c"),
                  "rk4": ("ux", "uy", "uz", "wx", "wy", "wz")}[what]
         out = {k: b[q * n3:(q + 1) * n3].reshape(shape).copy() for q, k in enumerate(names)}
         if what == "rk4":

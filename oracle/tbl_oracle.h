@@ -151,6 +151,15 @@ int or_terms(const or_params *p, int kphys, const or_base *b, const or_cplx *ux,
 int or_drv_gv(const or_params *p, double Re, int kphys, const or_base *b, const or_cplx *ux,
               const or_cplx *uy, const or_cplx *uz, const or_cplx *wx, const or_cplx *wy,
               const or_cplx *wz, or_cplx *dvx, or_cplx *dvy, or_cplx *dvz);
+/* flat-plate form (my_form 0): kphys 0: lterms_fi (nlterms_par.f:6-38,
+   needs b); else nlterms_fi (41-86, b unused): a, b, c */
+int or_terms_fi(const or_params *p, int kphys, const or_base *b, const or_cplx *ux,
+                const or_cplx *uy, const or_cplx *uz, const or_cplx *wx, const or_cplx *wy,
+                const or_cplx *wz, or_cplx *a, or_cplx *bb, or_cplx *c);
+/* drv_fi (main_par.f:1264-1336): rows j >= 2 of dvx, dvy, dvz written */
+int or_drv_fi(const or_params *p, double Re, int kphys, const or_base *b, const or_cplx *ux,
+              const or_cplx *uy, const or_cplx *uz, const or_cplx *wx, const or_cplx *wy,
+              const or_cplx *wz, or_cplx *dvx, or_cplx *dvy, or_cplx *dvz);
 /* one RK4 time step (main_par.f:266-349); cycles: 4 x nk (per stage) */
 int or_rk4_step(const or_params *p, double Re, double dt, int kphys, const or_base *b,
                 const double *bdfc, or_cplx *d2uydx2, const or_cplx *pert_uy,

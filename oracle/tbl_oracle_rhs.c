@@ -3,7 +3,9 @@
  * the RK4 time step of the reference's Goertler case (my_form 1):
  * lterms_gv / nlterms_gv with their z transforms f_to_p, p_to_f and four1
  * (nlterms_par.f:89-182, 303-490), drv_gv (main_par.f:1339-1403) and the
- * four-stage update around `loop` (main_par.f:266-349).
+ * four-stage update around `loop` (main_par.f:266-349); and the flat-plate
+ * form (my_form 0) of the right-hand side: lterms_fi / nlterms_fi
+ * (nlterms_par.f:6-86) and drv_fi (main_par.f:1264-1336).
  *
  * TEST INFRASTRUCTURE ONLY (see tbl_oracle.h).  One MPI rank.  Expressions
  * keep the reference's evaluation order.
@@ -291,6 +293,125 @@ int or_drv_gv(const or_params *p, double Re, int kphys, const or_base *b, const 
                                       + k2 * F3(wy, i, j, k)) / Re;
                 F3(dvz, i, j, k) = -F3(dbdx, i, j, k) + F3(dcdy, i, j, k)
                                    + vg * F3(dddx, i, j, k) + F3(d, i, j, k) * dvg
+                                   + (F3(zx, i, j, k) + F3(zy, i, j, k) * fy
+                                      + k2 * F3(wz, i, j, k)) / Re;
+            }
+    }
+    free(buf);
+    return 0;
+}
+
+/* lterms_fi: nlterms_par.f:6-38 (mode 1 against the base flow's own
+   products, modes >= 2 as lterms_gv without d) */
+static void lterms_fi(const or_params *p, const or_base *b, const or_cplx *ux,
+                      const or_cplx *uy, const or_cplx *wx, const or_cplx *wy, const or_cplx *wz,
+                      or_cplx *a, or_cplx *bb, or_cplx *c)
+{
+    const int nx = p->nx, ny = p->ny, nk = p->nk;
+    for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i) {
+            double uxb = F2(b->uxb, i, j), uyb = F2(b->uyb, i, j), wzb = F2(b->wzb, i, j);
+            F3(a, i, j, 1) = uyb * F3(wx, i, j, 1) - F3(wy, i, j, 1) * uxb;
+            F3(bb, i, j, 1) = uxb * (F3(wz, i, j, 1) - wzb) + wzb * (F3(ux, i, j, 1) - uxb);
+            F3(c, i, j, 1) = -uyb * (F3(wz, i, j, 1) - wzb) - wzb * (F3(uy, i, j, 1) - uyb);
+        }
+    for (int k = 2; k <= nk; ++k)
+        for (int j = 1; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                double uxb = F2(b->uxb, i, j), uyb = F2(b->uyb, i, j), wzb = F2(b->wzb, i, j);
+                F3(a, i, j, k) = uyb * F3(wx, i, j, k) - F3(wy, i, j, k) * uxb;
+                F3(bb, i, j, k) = uxb * F3(wz, i, j, k) + F3(ux, i, j, k) * wzb;
+                F3(c, i, j, k) = -uyb * F3(wz, i, j, k) - F3(uy, i, j, k) * wzb;
+            }
+}
+
+/* nlterms_fi: nlterms_par.f:41-86 (the three products in physical space,
+   no base-flow terms) */
+static int nlterms_fi(const or_params *p, int kphys, const or_cplx *ux, const or_cplx *uy,
+                      const or_cplx *uz, const or_cplx *wx, const or_cplx *wy,
+                      const or_cplx *wz, or_cplx *a, or_cplx *bb, or_cplx *c)
+{
+    const int nx = p->nx, ny = p->ny, nk = p->nk;
+    if (kphys < 2 * nk || kphys < 4 || (kphys & (kphys - 1))) return -1;
+    const size_t plane = (size_t)nx * ny;
+    double *P = malloc(sizeof(double) * kphys * 9);
+    if (!P) return -2;
+    double *up = P, *vp = P + kphys, *zp = P + 2 * kphys, *xp = P + 3 * kphys,
+           *yp = P + 4 * kphys, *wp = P + 5 * kphys, *ap = P + 6 * kphys, *bp = P + 7 * kphys,
+           *cp = P + 8 * kphys;
+    for (size_t e = 0; e < plane; ++e) {
+        f_to_p_point(ux + e, plane, nk, kphys, up);
+        f_to_p_point(uy + e, plane, nk, kphys, vp);
+        f_to_p_point(uz + e, plane, nk, kphys, zp);
+        f_to_p_point(wx + e, plane, nk, kphys, xp);
+        f_to_p_point(wy + e, plane, nk, kphys, yp);
+        f_to_p_point(wz + e, plane, nk, kphys, wp);
+        for (int q = 0; q < kphys; ++q) {
+            ap[q] = vp[q] * xp[q] - up[q] * yp[q];
+            bp[q] = up[q] * wp[q] - zp[q] * xp[q];
+            cp[q] = zp[q] * yp[q] - vp[q] * wp[q];
+        }
+        p_to_f_point(ap, nk, kphys, a + e, plane);
+        p_to_f_point(bp, nk, kphys, bb + e, plane);
+        p_to_f_point(cp, nk, kphys, c + e, plane);
+    }
+    free(P);
+    return 0;
+}
+
+int or_terms_fi(const or_params *p, int kphys, const or_base *b, const or_cplx *ux,
+                const or_cplx *uy, const or_cplx *uz, const or_cplx *wx, const or_cplx *wy,
+                const or_cplx *wz, or_cplx *a, or_cplx *bb, or_cplx *c)
+{
+    if (kphys == 0) {
+        if (!b) return -1;
+        lterms_fi(p, b, ux, uy, wx, wy, wz, a, bb, c);
+        return 0;
+    }
+    return nlterms_fi(p, kphys, ux, uy, uz, wx, wy, wz, a, bb, c);
+}
+
+/* drv_fi: main_par.f:1264-1336 (kphys > 0: nlterms_fi, the reference's
+   active call, with that many physical points; 0: lterms_fi) */
+int or_drv_fi(const or_params *p, double Re, int kphys, const or_base *b, const or_cplx *ux,
+              const or_cplx *uy, const or_cplx *uz, const or_cplx *wx, const or_cplx *wy,
+              const or_cplx *wz, or_cplx *dvx, or_cplx *dvy, or_cplx *dvz)
+{
+    const int nx = p->nx, ny = p->ny, nk = p->nk;
+    const size_t n3 = (size_t)nx * ny * nk;
+    or_cplx *buf = malloc(sizeof(or_cplx) * n3 * 13);
+    if (!buf) return -2;
+    or_cplx *a = buf, *bb = a + n3, *c = bb + n3, *dadx = c + n3, *dbdx = dadx + n3,
+            *dady = dbdx + n3, *dcdy = dady + n3, *xx = dcdy + n3, *yx = xx + n3, *zx = yx + n3,
+            *xy = zx + n3, *yy = xy + n3, *zy = yy + n3;
+    int rc = or_terms_fi(p, kphys, b, ux, uy, uz, wx, wy, wz, a, bb, c);
+    if (!rc) rc = or_derparx(p, a, dadx);
+    if (!rc) rc = or_derparx(p, bb, dbdx);
+    if (!rc) rc = or_dery(p, a, dady);
+    if (!rc) rc = or_dery(p, c, dcdy);
+    if (!rc) rc = or_derparxx(p, wx, xx);
+    if (!rc) rc = or_derparxx(p, wy, yx);
+    if (!rc) rc = or_derparxx(p, wz, zx);
+    if (!rc) rc = or_deryy(p, wx, xy);
+    if (!rc) rc = or_deryy(p, wy, yy);
+    if (!rc) rc = or_deryy(p, wz, zy);
+    if (rc) {
+        free(buf);
+        return rc;
+    }
+    const double fy = p->fac_y;
+    for (int k = 1; k <= nk; ++k) {
+        const or_cplx kb = kb_of(p, k);
+        const double k2 = k2b2_of(p, k);
+        for (int j = 2; j <= ny; ++j)
+            for (int i = 1; i <= nx; ++i) {
+                F3(dvx, i, j, k) = -F3(dady, i, j, k) + kb * F3(bb, i, j, k)
+                                   + (F3(xx, i, j, k) + F3(xy, i, j, k) * fy
+                                      + k2 * F3(wx, i, j, k)) / Re;
+                F3(dvy, i, j, k) = -kb * F3(c, i, j, k) + F3(dadx, i, j, k)
+                                   + (F3(yx, i, j, k) + F3(yy, i, j, k) * fy
+                                      + k2 * F3(wy, i, j, k)) / Re;
+                F3(dvz, i, j, k) = -F3(dbdx, i, j, k) + F3(dcdy, i, j, k)
                                    + (F3(zx, i, j, k) + F3(zy, i, j, k) * fy
                                       + k2 * F3(wz, i, j, k)) / Re;
             }

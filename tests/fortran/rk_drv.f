@@ -1,8 +1,9 @@
-c     Fortran driver of the drop-in `drv_gv` for the drop-in test.  Reads
-c     the coefficient tables, the fields ux .. wz (COMMON /var/), the base
-c     flow uxb, uyb, wzb (/blas/) and varg, dvargdx (/vc/) from drv_in.bin
-c     (stream access, written by tests/test_gpu_fortran.py), calls
-c     drv_gv(dvx, dvy, dvz) and writes the three results to drv_out.bin.
+c     Fortran driver of the drop-in `drv_gv` (or, with the argument fi,
+c     `drv_fi`) for the drop-in test.  Reads the coefficient tables, the
+c     fields ux .. wz (COMMON /var/), the base flow uxb, uyb, wzb (/blas/)
+c     and varg, dvargdx (/vc/) from drv_in.bin (stream access, written by
+c     tests/test_gpu_fortran.py), calls drv_gv(dvx, dvy, dvz) or
+c     drv_fi(dvx, dvy, dvz) and writes the three results to drv_out.bin.
       program rk_drv
 
       implicit none
@@ -15,6 +16,12 @@ c     drv_gv(dvx, dvy, dvz) and writes the three results to drv_out.bin.
      &           dvz(ptsx,jmax,kfour)
       common/blas/ uxb, uyb, wzb
       common/vc/ varg, dvargdx
+      character*16 form
+
+      form = 'gv'
+      if (command_argument_count() .ge. 1) then
+        call get_command_argument(1, form)
+      end if
 
       open(1, file='drv_in.bin', access='stream', form='unformatted',
      &     status='old')
@@ -28,7 +35,11 @@ c     drv_gv(dvx, dvy, dvz) and writes the three results to drv_out.bin.
       dvx = (0.d0, 0.d0)
       dvy = (0.d0, 0.d0)
       dvz = (0.d0, 0.d0)
-      call drv_gv(dvx, dvy, dvz)
+      if (trim(form) .eq. 'fi') then
+        call drv_fi(dvx, dvy, dvz)
+      else
+        call drv_gv(dvx, dvy, dvz)
+      end if
 
       open(2, file='drv_out.bin', access='stream', form='unformatted',
      &     status='replace')

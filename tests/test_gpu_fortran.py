@@ -3,7 +3,8 @@ RK-stage program: derparx, derparxx, dery, deryy of uy, then poi_uy, with
 the reference's subroutine names, COMMON blocks and PARAMETERs (the test's
 own par.for, 129 x 33 x 4 modes, 3 levels), against the CPU oracle; a
 Fortran program calling the drop-in `loop(erro)`, the whole RK-stage update;
-and one calling the drop-in `drv_gv(dvx, dvy, dvz)`, the RK right-hand side."""
+and one calling the drop-in `drv_gv(dvx, dvy, dvz)` or `drv_fi(dvx, dvy,
+dvz)`, the RK right-hand side of the Goertler or the flat-plate form."""
 import os
 import subprocess
 
@@ -96,14 +97,18 @@ def test_fortran_loop_matches_oracle(tmp_path, gpu_ready, erro):
 
 DRV_EXE = os.path.join(os.path.dirname(EXE), "rk_drv")
 RE_PARFOR = 2.5e3           # Re of tests/fortran/par.for
+KPHYS_PARFOR = 8            # its kphys
 
 
-def test_fortran_drv_gv_matches_oracle(tmp_path, gpu_ready):
-    """The drop-in `drv_gv(dvx, dvy, dvz)` (main_par.f:1339-1403) from a
-    Fortran program with the reference's COMMON blocks /var/, /coefs/,
-    /blas/, /vc/ and par.for's Re: the three results on rows 2..jmax against
-    the oracle's drv_gv (lterms_gv) at 1e-12 of each field's largest value,
-    as the device entry point is checked; row 1 stays the caller's zero."""
+@pytest.mark.parametrize("form", ["gv", "fi"])
+def test_fortran_drv_matches_oracle(tmp_path, gpu_ready, form):
+    """The drop-in `drv_gv(dvx, dvy, dvz)` (main_par.f:1339-1403, with
+    lterms_gv) or `drv_fi(dvx, dvy, dvz)` (1264-1336, with nlterms_fi at
+    par.for's kphys) from a Fortran program with the reference's COMMON
+    blocks /var/, /coefs/, /blas/, /vc/ and par.for's Re: the three results
+    on rows 2..jmax against the oracle at 1e-12 of each field's largest
+    value, as the device entry points are checked; row 1 stays the caller's
+    zero."""
     if not os.path.exists(DRV_EXE):
         pytest.fail(f"{DRV_EXE} not built (run __graft_entry__.build())")
     coefs = oracle.coefgen.generate(G.stf, G.levels)
@@ -123,13 +128,15 @@ def test_fortran_drv_gv_matches_oracle(tmp_path, gpu_ready):
             np.ascontiguousarray(F[k], dtype="<c16").tofile(fh)
         for k in ("uxb", "uyb", "wzb", "varg", "dvargdx"):
             np.ascontiguousarray(base[k], dtype="<f8").tofile(fh)
-    r = subprocess.run([DRV_EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    r = subprocess.run([DRV_EXE, form], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     shp = (G.nmodes, G.ny, G.nx)
     out = np.fromfile(tmp_path / "drv_out.bin", dtype="<c16").reshape((3,) + shp)
     p = oracle.make_params(G.nx, G.ny, G.nmodes, G.levels, G.dx, G.dy0, G.stf, G.alpha, G.beta,
                            coefs=coefs)
-    ref = oracle.drv_gv(p, RE_PARFOR, 0, base, F)
+    ref = (oracle.drv_gv(p, RE_PARFOR, 0, base, F) if form == "gv"
+           else oracle.drv_fi(p, RE_PARFOR, KPHYS_PARFOR, None, F))
     for q, name in enumerate(("dvx", "dvy", "dvz")):
         e = rel(out[q][:, 1:], ref[q][:, 1:])
         assert e < 1e-12, (name, e)

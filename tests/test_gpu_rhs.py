@@ -12,6 +12,8 @@ Tolerances, relative to the largest magnitude of each compared field:
 * one RK4 step (four drv_gv + update + loop stages; the loops' poi_uy in the
   GPU's colour order, as the oracle is run): per-stage cycle counts equal,
   fields 1e-11 (errors of four stages compound).
+The flat-plate form (lterms_fi / nlterms_fi, drv_fi) at the same tolerances
+as its Goertler counterparts.
 """
 import numpy as np
 import pytest
@@ -114,6 +116,53 @@ def test_drv_gv_matches_oracle(torch_mod, g, kphys):
         e = rel(got[q][:, 1:], ref[q][:, 1:])
         print(f"  {name}: rel {e:.2e}")
         assert e < 1e-12, name
+
+
+@pytest.mark.parametrize("g, kphys", [(SMALL, 0), (SMALL, 8), (CFG1, 16), (CFG2, 128)],
+                         ids=["129x33x4-lin", "129x33x4-nl8", "385x97x8-nl16",
+                              "1025x193x64-nl128"])
+def test_terms_fi_match_oracle(torch_mod, g, kphys):
+    torch = torch_mod
+    print(f"terms_fi {g.nx}x{g.ny}x{g.nmodes} kphys {kphys}", flush=True)
+    F, base, _ = inputs(g)
+    ref = oracle.terms_fi(oparams(g), kphys, base if kphys == 0 else None, F)
+    with tb.Context(g, coefs=coefs_of(g)) as ctx:
+        d = {k: _dev(torch, F[k]) for k in F}
+        db = {k: _dev(torch, base[k]) for k in ("uxb", "uyb", "wzb")}
+        out = [torch.zeros_like(d["ux"]) for _ in range(3)]
+        ctx.terms_fi_dev(kphys, {k: db[k].data_ptr() for k in db} if kphys == 0 else None,
+                         *[d[k].data_ptr() for k in ("ux", "uy", "uz", "wx", "wy", "wz")],
+                         *[t.data_ptr() for t in out])
+        got = [_host(torch, t) for t in out]
+    tol = 1e-15 if kphys == 0 else 1e-13
+    for q, name in enumerate("abc"):
+        e = rel(got[q], ref[q])
+        print(f"  {name}: rel {e:.2e}")
+        assert e < tol, name
+
+
+@pytest.mark.parametrize("g, kphys", [(SMALL, 0), (SMALL, 8), (CFG1, 16)],
+                         ids=["129x33x4-lin", "129x33x4-nl8", "385x97x8-nl16"])
+def test_drv_fi_matches_oracle(torch_mod, g, kphys):
+    torch = torch_mod
+    print(f"drv_fi {g.nx}x{g.ny}x{g.nmodes} kphys {kphys}", flush=True)
+    F, base, _ = inputs(g)
+    ref = oracle.drv_fi(oparams(g), RE, kphys, base if kphys == 0 else None, F)
+    with tb.Context(g, coefs=coefs_of(g)) as ctx:
+        d = {k: _dev(torch, F[k]) for k in F}
+        db = {k: _dev(torch, base[k]) for k in ("uxb", "uyb", "wzb")}
+        out = [torch.zeros_like(d["ux"]) for _ in range(3)]
+        ctx.drv_fi_dev(RE, kphys, {k: db[k].data_ptr() for k in db} if kphys == 0 else None,
+                       *[d[k].data_ptr() for k in ("ux", "uy", "uz", "wx", "wy", "wz")],
+                       *[t.data_ptr() for t in out])
+        got = [_host(torch, t) for t in out]
+        # the host-array entry point (the Fortran drop-in's) gives the same
+        got_h = ctx.drv_fi(RE, kphys, base if kphys == 0 else None, F)
+    for q, name in enumerate(("dvx", "dvy", "dvz")):
+        e = rel(got[q][:, 1:], ref[q][:, 1:])
+        print(f"  {name}: rel {e:.2e}")
+        assert e < 1e-12, name
+        assert np.array_equal(got_h[q][:, 1:], got[q][:, 1:]) and not np.any(got_h[q][:, 0])
 
 
 def test_drv_gv_host_arrays_match_oracle(gpu_ready):

@@ -1,7 +1,9 @@
 """The oracle's restatement of the right-hand side and the RK4 step of the
 reference's Goertler case: lterms_gv / nlterms_gv with the z transforms
 f_to_p, p_to_f, four1 (nlterms_par.f), drv_gv (main_par.f:1339-1403) and
-the four-stage update around `loop` (main_par.f:266-349).
+the four-stage update around `loop` (main_par.f:266-349); and of the
+flat-plate form's lterms_fi / nlterms_fi (nlterms_par.f:6-86) and drv_fi
+(main_par.f:1264-1336).
 
 Pinned against the reference's own routines (tests/golden/ref_rhs_vectors.npz,
 written by tests/golden/make_ref_vectors.py --rhs; inputs regenerated here),
@@ -90,6 +92,57 @@ def test_drv_gv_and_nlterms_match_reference(name):
     nl = oracle.terms(p, 2 * g.nmodes, base, F)
     for a, k in zip(nl, "abcd"):
         assert rel(a[sub], GOLD[f"{name}/nlterms/{k}"]) < 1e-13, k
+
+
+@pytest.mark.parametrize("name", ["129x33x4", "385x97x8"])
+def test_drv_fi_matches_reference(name):
+    """drv_fi with its active nlterms_fi (kphys = 2 nmodes, the configured
+    par.for's), rows 2..jmax."""
+    g, p = params(name)
+    F, base, d2, b, fcx2, fcll = rhs_inputs(g)
+    Re = float(GOLD[f"{name}/params"][0])
+    sub = SUB[name]
+    dv = oracle.drv_fi(p, Re, 2 * g.nmodes, None, F)
+    for a, k in zip(dv, ("dvx", "dvy", "dvz")):
+        assert rel(a[sub][:, 1:], GOLD[f"{name}/drvfi/{k}"]) < 1e-13, k
+
+
+def test_terms_fi_match_reference():
+    """nlterms_fi (the three products through the z transforms) and
+    lterms_fi (mode 1 against the base flow's own products)."""
+    name = "129x33x4"
+    g, p = params(name)
+    F, base, d2, b, fcx2, fcll = rhs_inputs(g)
+    sub = SUB[name]
+    nl = oracle.terms_fi(p, 2 * g.nmodes, None, F)
+    lt = oracle.terms_fi(p, 0, base, F)
+    for q, k in enumerate("abc"):
+        assert rel(nl[q][sub], GOLD[f"{name}/nltfi/{k}"]) < 1e-13, ("nl", k)
+        assert rel(lt[q][sub], GOLD[f"{name}/ltfi/{k}"]) < 1e-14, ("l", k)
+
+
+def test_terms_fi_closed_forms():
+    """What the mathematics fixes: nlterms_fi of single modes is the cross
+    product of the physical fields (ux = cos, wy = const: a = -ux wy has
+    only mode 1), and lterms_fi's mode 1 vanishes on the base flow itself
+    (ux = uxb, uy = uyb, wz = wzb, wx = wy = 0)."""
+    g = wl.Grid(17, 9, 4, 2, 0.1, 0.05, 1.0, 0.0, 0.5)
+    p = oracle.make_params(g.nx, g.ny, g.nmodes, g.levels, g.dx, g.dy0, g.stf, g.alpha, g.beta)
+    shp = (g.nmodes, g.ny, g.nx)
+    F = {k: np.zeros(shp, np.complex128) for k in ("ux", "uy", "uz", "wx", "wy", "wz")}
+    F["ux"][1] = 1.0          # ux = cos(beta z) in the packed convention's mode 1
+    F["wy"][0] = 2.0          # wy = 2 (mode 0 real)
+    a, b_, c = oracle.terms_fi(p, 8, None, F)
+    ref = np.zeros(shp, np.complex128)
+    ref[1] = -2.0             # a = uy wx - ux wy = -2 cos(beta z)
+    assert np.abs(a - ref).max() < 1e-13 and np.abs(b_).max() < 1e-13
+    assert np.abs(c).max() < 1e-13
+    X, Y = np.meshgrid(np.arange(g.nx) * 0.1, np.arange(g.ny) * 0.05)
+    base = {"uxb": 1 + X * Y, "uyb": 0.1 * Y, "wzb": -X, "varg": 0 * X, "dvargdx": 0 * X}
+    G = {k: np.zeros(shp, np.complex128) for k in F}
+    G["ux"][0], G["uy"][0], G["wz"][0] = base["uxb"], base["uyb"], base["wzb"]
+    la, lb, lc = oracle.terms_fi(p, 0, base, G)
+    assert np.abs(lb[0]).max() == 0 and np.abs(lc[0]).max() == 0 and np.abs(la).max() == 0
 
 
 def test_rk4_step_matches_reference():

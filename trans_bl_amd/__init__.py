@@ -86,7 +86,8 @@ EXPORTS = [
     "tbl_baseflow_write", "tbl_baseflow_read", "tbl_poi_ux_dev", "tbl_poi_uz_dev",
     "tbl_wx_wall_dev", "tbl_wz_wall_dev", "tbl_bzone_dev", "tbl_filter_trid_dev",
     "tbl_outuy_dev", "tbl_loop_dev", "tbl_loop", "tbl_bdfc_generate", "tbl_terms_dev",
-    "tbl_drv_gv_dev", "tbl_drv_gv", "tbl_rk4_step_dev",
+    "tbl_drv_gv_dev", "tbl_drv_gv", "tbl_terms_fi_dev", "tbl_drv_fi_dev", "tbl_drv_fi",
+    "tbl_rk4_step_dev",
 ]
 
 _lib = None
@@ -137,6 +138,9 @@ def lib():
         "tbl_terms_dev": ([vp, ip, C.POINTER(CBase)] + [vp] * 10, C.c_int),
         "tbl_drv_gv_dev": ([vp, dp, ip, C.POINTER(CBase)] + [vp] * 9, C.c_int),
         "tbl_drv_gv": ([vp, dp, ip] + [vp] * 14, C.c_int),
+        "tbl_terms_fi_dev": ([vp, ip, C.POINTER(CBase)] + [vp] * 9, C.c_int),
+        "tbl_drv_fi_dev": ([vp, dp, ip, C.POINTER(CBase)] + [vp] * 9, C.c_int),
+        "tbl_drv_fi": ([vp, dp, ip] + [vp] * 12, C.c_int),
         "tbl_rk4_step_dev": ([vp, dp, dp, ip, C.POINTER(CBase), vp, vp, vp, vp, ip, ip]
                              + [vp] * 7 + [C.POINTER(SolveOpts), vp, vp], C.c_int),
     }
@@ -255,6 +259,14 @@ def coefs_from_dict(d: dict) -> CCoefs:
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def _cbase(base):
+    """tbl_base of device pointers (missing keys: NULL), or None."""
+    if base is None:
+        return None
+    return CBase(*[C.c_void_p(base[k]) if base.get(k) else None
+                   for k in ("uxb", "uyb", "wzb", "varg", "dvargdx")])
 
 
 class Context:
@@ -492,6 +504,37 @@ class Context:
         out = [np.zeros(shp, np.complex128) for _ in range(3)]
         _check(lib().tbl_drv_gv(self.h, Re, kphys, *[_ptr(a) for a in bs + fs + out]),
                "tbl_drv_gv")
+        return tuple(out)
+
+    def terms_fi_dev(self, kphys, base, ux, uy, uz, wx, wy, wz, a, b, c):
+        """Flat-plate terms (lterms_fi for kphys 0, else nlterms_fi) on device
+        pointers; base: dict of device pointers uxb, uyb, wzb, or None."""
+        cb = _cbase(base)
+        _check(lib().tbl_terms_fi_dev(self.h, kphys, C.byref(cb) if cb else None,
+                                      *[C.c_void_p(x) for x in (ux, uy, uz, wx, wy, wz, a, b,
+                                                                c)]), "tbl_terms_fi_dev")
+
+    def drv_fi_dev(self, Re, kphys, base, ux, uy, uz, wx, wy, wz, dvx, dvy, dvz):
+        cb = _cbase(base)
+        _check(lib().tbl_drv_fi_dev(self.h, Re, kphys, C.byref(cb) if cb else None,
+                                    *[C.c_void_p(x) for x in (ux, uy, uz, wx, wy, wz, dvx, dvy,
+                                                              dvz)]), "tbl_drv_fi_dev")
+
+    def drv_fi(self, Re, kphys, base, fields: dict):
+        """drv_fi on host arrays: base (for kphys 0) a dict of real (ny, nx)
+        arrays uxb, uyb, wzb, or None; returns dvx, dvy, dvz (row 1 zero)."""
+        g = self.grid
+        shp = (g.nmodes, g.ny, g.nx)
+        bs = [np.ascontiguousarray(base[k], dtype=np.float64) for k in ("uxb", "uyb", "wzb")] \
+            if base is not None else []
+        fs = [np.ascontiguousarray(fields[k], dtype=np.complex128)
+              for k in ("ux", "uy", "uz", "wx", "wy", "wz")]
+        if any(b.shape != shp[1:] for b in bs) or any(f.shape != shp for f in fs):
+            raise TblError(f"drv_fi: base (ny, nx) and fields {shp} expected")
+        out = [np.zeros(shp, np.complex128) for _ in range(3)]
+        bp = [_ptr(a) for a in bs] if bs else [None] * 3
+        _check(lib().tbl_drv_fi(self.h, Re, kphys, *bp, *[_ptr(a) for a in fs + out]),
+               "tbl_drv_fi")
         return tuple(out)
 
     def rk4_step_dev(self, Re, dt, kphys, base, bdfc, d2uydx2, fields, pert=None, **kw):

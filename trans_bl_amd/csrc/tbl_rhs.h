@@ -16,9 +16,10 @@ struct BaseArgs {
 struct RhsArgs {
     BaseArgs b;
     const cx<double> *ux, *uy, *uz, *wx, *wy, *wz;
-    cx<double> *a, *bb, *c, *d;
+    cx<double> *a, *bb, *c, *d;    // d: Goertler form only
     const double *tw;          // nonlinear: cos, sin of 2 pi t / kphys, t < kphys
     int nx, ny, K;
+    int fi;                    // flat-plate form (lterms_fi / nlterms_fi): a, b, c
 };
 
 struct DrvArgs {
@@ -29,6 +30,7 @@ struct DrvArgs {
     cx<double> *dvx, *dvy, *dvz;
     double rre, fac_y, beta;
     int nx, ny, K;
+    int fi;                    // drv_fi: no curvature terms (d, dddx, varg unused)
 };
 
 struct RkArgs {

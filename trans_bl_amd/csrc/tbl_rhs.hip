@@ -2,7 +2,9 @@
 // reference's Goertler case (my_form 1): lterms_gv / nlterms_gv
 // (nlterms_par.f:89-182) with the z transforms of f_to_p / p_to_f
 // (303-428), the combination of drv_gv (main_par.f:1339-1403), the four
-// stage updates (main_par.f:270-341) and the gv_pert disturbance rows.
+// stage updates (main_par.f:270-341) and the gv_pert disturbance rows; and
+// the flat-plate form (my_form 0): lterms_fi / nlterms_fi (nlterms_par.f:
+// 6-86) and drv_fi's combination (main_par.f:1264-1336).
 //
 // User layout (complex (nx, ny, K), x fastest); base-flow arrays real
 // (nx, ny).  All kernels but the nonlinear one are point-wise and
@@ -34,6 +36,17 @@ __device__ __forceinline__ void lin_terms(double uxb, double uyb, double wzb, cx
     d = (2.0 * uxb) * ux;
 }
 
+// lterms_fi's mode 1 (nlterms_par.f:19-24): the disturbance of b and c
+// taken against the base flow's own products
+__device__ __forceinline__ void lin_terms_fi1(double uxb, double uyb, double wzb, cxd ux, cxd uy,
+                                              cxd wx, cxd wy, cxd wz, cxd &a, cxd &b, cxd &c)
+{
+    const cxd dwz = {wz.re - wzb, wz.im}, dux = {ux.re - uxb, ux.im}, duy = {uy.re - uyb, uy.im};
+    a = uyb * wx - wy * uxb;
+    b = uxb * dwz + wzb * dux;
+    c = -uyb * dwz - wzb * duy;
+}
+
 __global__ void lterms_kernel(RhsArgs r)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -41,14 +54,28 @@ __global__ void lterms_kernel(RhsArgs r)
     if (i >= r.nx) return;
     const size_t e2 = static_cast<size_t>(j) * r.nx + i;
     const size_t e = static_cast<size_t>(k) * r.nx * r.ny + e2;
+    if (r.fi) {
+        // lterms_fi: modes >= 2 as lterms_gv without d (nlterms_par.f:28-34)
+        cxd d;
+        if (k == 0)
+            lin_terms_fi1(r.b.uxb[e2], r.b.uyb[e2], r.b.wzb[e2], r.ux[e], r.uy[e], r.wx[e],
+                          r.wy[e], r.wz[e], r.a[e], r.bb[e], r.c[e]);
+        else
+            lin_terms(r.b.uxb[e2], r.b.uyb[e2], r.b.wzb[e2], r.ux[e], r.uy[e], r.wx[e],
+                      r.wy[e], r.wz[e], r.a[e], r.bb[e], r.c[e], d);
+        return;
+    }
     lin_terms(r.b.uxb[e2], r.b.uyb[e2], r.b.wzb[e2], r.ux[e], r.uy[e], r.wx[e], r.wy[e],
               r.wz[e], r.a[e], r.bb[e], r.c[e], r.d[e]);
 }
 
-// nlterms_gv: kNlPts points per workgroup of 256 threads; LDS holds the six
+// nlterms_gv (FI: nlterms_fi, the first three products, no base-flow
+// terms): kNlPts points per workgroup of 256 threads; LDS holds the six
 // inputs' modes [f][m][p], the four products [q][n][p] and the twiddles
+template <bool FI>
 __global__ void __launch_bounds__(256) nlterms_kernel(RhsArgs r, int N)
 {
+    constexpr int NQ = FI ? 3 : 4;
     extern __shared__ double sm[];
     const int K = r.K, P = kNlPts;
     const size_t plane = static_cast<size_t>(r.nx) * r.ny;
@@ -93,7 +120,7 @@ __global__ void __launch_bounds__(256) nlterms_kernel(RhsArgs r, int N)
         Q[(0 * N + n) * P + p] = v[1] * v[3] - v[0] * v[4];
         Q[(1 * N + n) * P + p] = v[0] * v[5] - v[2] * v[3];
         Q[(2 * N + n) * P + p] = v[2] * v[4] - v[1] * v[5];
-        Q[(3 * N + n) * P + p] = v[0] * v[0];
+        if (!FI) Q[(3 * N + n) * P + p] = v[0] * v[0];
     }
     __syncthreads();
     // analysis: c_0 = (1/N) sum v_n (real), c_m = (2/N) sum v_n e^{+2 pi i m n / N};
@@ -108,7 +135,7 @@ __global__ void __launch_bounds__(256) nlterms_kernel(RhsArgs r, int N)
         for (int n = 0; n < N; ++n) {
             const double cm = cs[idx], sm_ = sn[idx];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < NQ; ++q) {
                 const double v = Q[(q * N + n) * P + p];
                 re[q] += v * cm;
                 im[q] += v * sm_;
@@ -121,11 +148,17 @@ __global__ void __launch_bounds__(256) nlterms_kernel(RhsArgs r, int N)
         const size_t e = static_cast<size_t>(m) * plane + e2;
         cxd nl[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             double x = re[q] * sc, y = m == 0 ? 0.0 : im[q] * sc;
             if (fabs(x) < 1e-14) x = 0.0;
             if (fabs(y) < 1e-14) y = 0.0;
             nl[q] = {x, y};
+        }
+        if (FI) {
+            out[0][e] = nl[0];
+            out[1][e] = nl[1];
+            out[2][e] = nl[2];
+            continue;
         }
         cxd la, lb, lc, ld;
         lin_terms(r.b.uxb[e2], r.b.uyb[e2], r.b.wzb[e2], M[(0 * K + m) * P + p],
@@ -148,7 +181,15 @@ __global__ void drv_combine_kernel(DrvArgs r)
     const size_t e = static_cast<size_t>(k) * r.nx * r.ny + e2;
     const double kb = -static_cast<double>(k) * r.beta;      // v_kb = i kb
     const double k2 = -static_cast<double>(k) * static_cast<double>(k) * r.beta * r.beta;
-    const double vg = r.b.varg[e2], dvg = r.b.dvargdx[e2], rre = r.rre, fy = r.fac_y;
+    const double rre = r.rre, fy = r.fac_y;
+    if (r.fi) {
+        // drv_fi's combination (main_par.f:1317-1331): no curvature terms
+        r.dvx[e] = -r.dady[e] + mul_i(kb, r.bb[e]) + rre * (r.xx[e] + fy * r.xy[e] + k2 * r.wx[e]);
+        r.dvy[e] = -mul_i(kb, r.c[e]) + r.dadx[e] + rre * (r.yx[e] + fy * r.yy[e] + k2 * r.wy[e]);
+        r.dvz[e] = -r.dbdx[e] + r.dcdy[e] + rre * (r.zx[e] + fy * r.zy[e] + k2 * r.wz[e]);
+        return;
+    }
+    const double vg = r.b.varg[e2], dvg = r.b.dvargdx[e2];
     const cxd d = r.d[e];
     r.dvx[e] = -r.dady[e] + mul_i(kb, r.bb[e]) - vg * mul_i(kb, d)
                + rre * (r.xx[e] + fy * r.xy[e] + k2 * r.wx[e]);
@@ -207,8 +248,12 @@ int terms(const RhsArgs &r, int kphys, hipStream_t st)
     } else {
         const size_t plane = static_cast<size_t>(r.nx) * r.ny;
         dim3 grid(static_cast<unsigned>((plane + kNlPts - 1) / kNlPts));
-        hipLaunchKernelGGL(nlterms_kernel, grid, dim3(256), nlterms_lds(r.K, kphys), st, r,
-                           kphys);
+        if (r.fi)
+            hipLaunchKernelGGL(nlterms_kernel<true>, grid, dim3(256), nlterms_lds(r.K, kphys), st,
+                               r, kphys);
+        else
+            hipLaunchKernelGGL(nlterms_kernel<false>, grid, dim3(256), nlterms_lds(r.K, kphys), st,
+                               r, kphys);
     }
     return TBL_OK;
 }
@@ -240,7 +285,9 @@ int pert_rows(cx<double> *uy, cx<double> *d2, const cx<double> *pu, const cx<dou
 
 int rhs_kernel_attributes()
 {
-    TBL_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&nlterms_kernel),
+    TBL_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&nlterms_kernel<false>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    TBL_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&nlterms_kernel<true>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     return TBL_OK;
 }

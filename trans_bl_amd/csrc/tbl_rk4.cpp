@@ -1,8 +1,10 @@
 // Right-hand side (drv_gv with lterms_gv or nlterms_gv) and the RK4 time
 // step of the reference's Goertler case on the GPU: host side -- work
 // fields, the z-DFT twiddle table, the launch sequence of drv_gv (terms,
-// the eleven derivatives as six fused derivative calls, the combination)
-// and of the four RK4 stages around tbl_loop_dev (main_par.f:266-349).
+// the eleven derivatives as seven derivative calls, the combination) and
+// of the four RK4 stages around tbl_loop_dev (main_par.f:266-349); and
+// drv_fi, the flat-plate form (main_par.f:1264-1336: lterms_fi /
+// nlterms_fi, ten derivatives, the combination without curvature terms).
 #include <cmath>
 #include <vector>
 
@@ -70,38 +72,41 @@ cx<double> *C(double *p) { return reinterpret_cast<cx<double> *>(p); }
 
 BaseArgs base_of(const tbl_base *b)
 {
-    return BaseArgs{b->uxb, b->uyb, b->wzb, b->varg, b->dvargdx};
+    return b ? BaseArgs{b->uxb, b->uyb, b->wzb, b->varg, b->dvargdx} : BaseArgs{};
 }
 
+// fi: the flat-plate form (a, b, c; out[3] unused, b unused when kphys > 0)
 int do_terms(tbl_ctx *c, int kphys, const tbl_base *b, const double *const *f,
-             cx<double> *const *out)
+             cx<double> *const *out, bool fi = false)
 {
     if (kphys) TRY(twiddles(c, kphys));
     RhsArgs r{base_of(b), C(f[0]), C(f[1]), C(f[2]), C(f[3]), C(f[4]), C(f[5]),
-              out[0], out[1], out[2], out[3], kphys ? c->d_tw : nullptr,
-              c->g.nx, c->g.ny, c->K};
+              out[0], out[1], out[2], fi ? nullptr : out[3], kphys ? c->d_tw : nullptr,
+              c->g.nx, c->g.ny, c->K, fi ? 1 : 0};
     TRY(terms(r, kphys, c->stream));
     TBL_HIP(hipGetLastError());
     return TBL_OK;
 }
 
-// drv_gv (main_par.f:1339-1403): f = ux, uy, uz, wx, wy, wz; dv = dvx, dvy, dvz
+// drv_gv (main_par.f:1339-1403), or with fi drv_fi (main_par.f:1264-1336):
+// f = ux, uy, uz, wx, wy, wz; dv = dvx, dvy, dvz
 int drv_gv(tbl_ctx *c, double Re, int kphys, const tbl_base *b, const double *const *f,
-           cx<double> *const *dv)
+           cx<double> *const *dv, bool fi = false)
 {
     TRY(ensure(c, &c->d_rhs, 15));
     const size_t n = field(c);
     cx<double> *w[15];
     for (int q = 0; q < 15; ++q) w[q] = c->d_rhs + q * n;
     cx<double> *a = w[0], *bb = w[1], *cc = w[2], *d = w[3];
-    TRY(do_terms(c, kphys, b, f, w));
+    TRY(do_terms(c, kphys, b, f, w, fi));
     auto D = [](cx<double> *p) { return reinterpret_cast<double *>(p); };
     const DerivArgs &A = c->dargs;
     hipStream_t st = c->stream;
-    // a: d/dx, d/dy; b, d: d/dx; c: d/dy; wx, wy, wz: d2/dx2, d2/dy2
+    // a: d/dx, d/dy; b, d (not in drv_fi): d/dx; c: d/dy; wx, wy, wz:
+    // d2/dx2, d2/dy2
     TRY(derivs4(A, D(a), D(w[4]), nullptr, D(w[7]), nullptr, st));
     TRY(derivs4(A, D(bb), D(w[5]), nullptr, nullptr, nullptr, st));
-    TRY(derivs4(A, D(d), D(w[6]), nullptr, nullptr, nullptr, st));
+    if (!fi) TRY(derivs4(A, D(d), D(w[6]), nullptr, nullptr, nullptr, st));
     TRY(derivs4(A, D(cc), nullptr, nullptr, D(w[8]), nullptr, st));
     TRY(derivs4(A, f[3], nullptr, D(w[9]), nullptr, D(w[12]), st));
     TRY(derivs4(A, f[4], nullptr, D(w[10]), nullptr, D(w[13]), st));
@@ -128,6 +133,7 @@ int drv_gv(tbl_ctx *c, double Re, int kphys, const tbl_base *b, const double *co
     r.dvx = dv[0];
     r.dvy = dv[1];
     r.dvz = dv[2];
+    r.fi = fi ? 1 : 0;
     r.rre = 1.0 / Re;
     r.fac_y = c->g.fac_y;
     r.beta = c->g.beta;
@@ -177,6 +183,80 @@ extern "C" int tbl_drv_gv_dev(tbl_ctx *c, double Re, int kphys, const tbl_base *
     return drv_gv(c, Re, kphys, b, f, dv);
 }
 
+// flat-plate form: base needs uxb, uyb, wzb for the linear terms only
+static bool fi_base_ok(const tbl_base *b, int kphys)
+{
+    return kphys > 0 || (b && b->uxb && b->uyb && b->wzb);
+}
+
+extern "C" int tbl_terms_fi_dev(tbl_ctx *c, int kphys, const tbl_base *b, const double *d_ux,
+                                const double *d_uy, const double *d_uz, const double *d_wx,
+                                const double *d_wy, const double *d_wz, double *d_a, double *d_b,
+                                double *d_c)
+{
+    if (!c || !fi_base_ok(b, kphys) || !d_ux || !d_uy || !d_uz || !d_wx || !d_wy || !d_wz ||
+        !d_a || !d_b || !d_c)
+        return TBL_ERR_ARG;
+    TRY(check_kphys(c, kphys));
+    const double *f[6] = {d_ux, d_uy, d_uz, d_wx, d_wy, d_wz};
+    cx<double> *o[4] = {C(d_a), C(d_b), C(d_c), nullptr};
+    for (auto *x : f)
+        for (int q = 0; q < 3; ++q)
+            if (reinterpret_cast<const void *>(x) == o[q])
+                return fail("tbl_terms_fi_dev: aliased fields");
+    return do_terms(c, kphys, kphys ? nullptr : b, f, o, true);
+}
+
+extern "C" int tbl_drv_fi_dev(tbl_ctx *c, double Re, int kphys, const tbl_base *b,
+                              const double *d_ux, const double *d_uy, const double *d_uz,
+                              const double *d_wx, const double *d_wy, const double *d_wz,
+                              double *d_dvx, double *d_dvy, double *d_dvz)
+{
+    if (!c || !fi_base_ok(b, kphys) || !d_ux || !d_uy || !d_uz || !d_wx || !d_wy || !d_wz ||
+        !d_dvx || !d_dvy || !d_dvz || !(Re > 0.0))
+        return TBL_ERR_ARG;
+    TRY(check_kphys(c, kphys));
+    const double *f[6] = {d_ux, d_uy, d_uz, d_wx, d_wy, d_wz};
+    cx<double> *dv[3] = {C(d_dvx), C(d_dvy), C(d_dvz)};
+    return drv_gv(c, Re, kphys, kphys ? nullptr : b, f, dv, true);
+}
+
+// host arrays of the RHS entry points: fields in (dv in as well, so row 1
+// keeps the caller's values), base-flow arrays in when given, dv out
+static int drv_host(tbl_ctx *c, double Re, int kphys, bool fi, const double *const *hb, int nb,
+             const double *const *hin, double *const *hout)
+{
+    const size_t n3 = field(c) * 2, n2 = static_cast<size_t>(c->g.nx) * c->g.ny;
+    double *d = nullptr;
+    TBL_HIP(hipMalloc(reinterpret_cast<void **>(&d), sizeof(double) * (9 * n3 + 5 * n2)));
+    double *df[9], *db[5] = {};
+    int rc = TBL_OK;
+    for (int q = 0; q < 9; ++q) df[q] = d + q * n3;
+    for (int q = 0; q < nb; ++q) db[q] = d + 9 * n3 + q * n2;
+    for (int q = 0; q < 9 && rc == TBL_OK; ++q)
+        if (hipMemcpy(df[q], q < 6 ? hin[q] : hout[q - 6], sizeof(double) * n3,
+                      hipMemcpyHostToDevice) != hipSuccess)
+            rc = TBL_ERR_HIP;
+    for (int q = 0; q < nb && rc == TBL_OK; ++q)
+        if (hipMemcpy(db[q], hb[q], sizeof(double) * n2, hipMemcpyHostToDevice) != hipSuccess)
+            rc = TBL_ERR_HIP;
+    if (rc == TBL_OK) {
+        const tbl_base b{db[0], db[1], db[2], db[3], db[4]};
+        rc = fi ? tbl_drv_fi_dev(c, Re, kphys, nb ? &b : nullptr, df[0], df[1], df[2], df[3],
+                                 df[4], df[5], df[6], df[7], df[8])
+                : tbl_drv_gv_dev(c, Re, kphys, &b, df[0], df[1], df[2], df[3], df[4], df[5],
+                                 df[6], df[7], df[8]);
+    }
+    if (rc == TBL_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = TBL_ERR_HIP;
+    for (int q = 0; q < 3 && rc == TBL_OK; ++q)
+        if (hipMemcpy(hout[q], df[6 + q], sizeof(double) * n3, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+            rc = TBL_ERR_HIP;
+    if (rc == TBL_ERR_HIP) tbl_set_last_error("drv_gv / drv_fi: copy failed", __FILE__, __LINE__);
+    (void)hipFree(d);
+    return rc;
+}
+
 extern "C" int tbl_drv_gv(tbl_ctx *c, double Re, int kphys, const double *uxb,
                           const double *uyb, const double *wzb, const double *varg,
                           const double *dvargdx, const double *ux, const double *uy,
@@ -187,37 +267,26 @@ extern "C" int tbl_drv_gv(tbl_ctx *c, double Re, int kphys, const double *uxb,
         !wz || !dvx || !dvy || !dvz || !(Re > 0.0))
         return TBL_ERR_ARG;
     TRY(check_kphys(c, kphys));
-    const size_t n3 = field(c) * 2, n2 = static_cast<size_t>(c->g.nx) * c->g.ny;
-    double *d = nullptr;
-    TBL_HIP(hipMalloc(reinterpret_cast<void **>(&d), sizeof(double) * (9 * n3 + 5 * n2)));
+    const double *hb[5] = {uxb, uyb, wzb, varg, dvargdx};
     const double *hin[6] = {ux, uy, uz, wx, wy, wz};
     double *hout[3] = {dvx, dvy, dvz};
-    const double *hb[5] = {uxb, uyb, wzb, varg, dvargdx};
-    double *df[9], *db[5];
-    int rc = TBL_OK;
-    for (int f = 0; f < 9; ++f) df[f] = d + f * n3;
-    for (int q = 0; q < 5; ++q) db[q] = d + 9 * n3 + q * n2;
-    // dv in as well: row 1, which drv_gv does not write, keeps the caller's values
-    for (int f = 0; f < 9 && rc == TBL_OK; ++f)
-        if (hipMemcpy(df[f], f < 6 ? hin[f] : hout[f - 6], sizeof(double) * n3,
-                      hipMemcpyHostToDevice) != hipSuccess)
-            rc = TBL_ERR_HIP;
-    for (int q = 0; q < 5 && rc == TBL_OK; ++q)
-        if (hipMemcpy(db[q], hb[q], sizeof(double) * n2, hipMemcpyHostToDevice) != hipSuccess)
-            rc = TBL_ERR_HIP;
-    if (rc == TBL_OK) {
-        const tbl_base b{db[0], db[1], db[2], db[3], db[4]};
-        rc = tbl_drv_gv_dev(c, Re, kphys, &b, df[0], df[1], df[2], df[3], df[4], df[5], df[6],
-                            df[7], df[8]);
-    }
-    if (rc == TBL_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = TBL_ERR_HIP;
-    for (int f = 0; f < 3 && rc == TBL_OK; ++f)
-        if (hipMemcpy(hout[f], df[6 + f], sizeof(double) * n3, hipMemcpyDeviceToHost) !=
-            hipSuccess)
-            rc = TBL_ERR_HIP;
-    if (rc == TBL_ERR_HIP) tbl_set_last_error("tbl_drv_gv: copy failed", __FILE__, __LINE__);
-    (void)hipFree(d);
-    return rc;
+    return drv_host(c, Re, kphys, false, hb, 5, hin, hout);
+}
+
+extern "C" int tbl_drv_fi(tbl_ctx *c, double Re, int kphys, const double *uxb,
+                          const double *uyb, const double *wzb, const double *ux,
+                          const double *uy, const double *uz, const double *wx, const double *wy,
+                          const double *wz, double *dvx, double *dvy, double *dvz)
+{
+    const bool with_base = uxb && uyb && wzb;
+    if (!c || (kphys == 0 && !with_base) || !ux || !uy || !uz || !wx || !wy || !wz || !dvx ||
+        !dvy || !dvz || !(Re > 0.0))
+        return TBL_ERR_ARG;
+    TRY(check_kphys(c, kphys));
+    const double *hb[3] = {uxb, uyb, wzb};
+    const double *hin[6] = {ux, uy, uz, wx, wy, wz};
+    double *hout[3] = {dvx, dvy, dvz};
+    return drv_host(c, Re, kphys, true, hb, kphys == 0 ? 3 : 0, hin, hout);
 }
 
 extern "C" int tbl_rk4_step_dev(tbl_ctx *c, double Re, double dt, int kphys, const tbl_base *b,

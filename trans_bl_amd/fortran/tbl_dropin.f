@@ -5,7 +5,8 @@ c
 c     Same names and argument lists as the reference (poi_uy,
 c     poisson_par.f:6; derparx, derparxx, dery, deryy, deryfv,
 c     derivs_par.f:80-177; loop, main_par.f:1762-1785; drv_gv,
-c     main_par.f:1339-1403), same hidden inputs: the PARAMETERs of
+c     main_par.f:1339-1403; drv_fi, main_par.f:1264-1336), same hidden
+c     inputs: the PARAMETERs of
 c     par.for, COMMON /var/, /coefs/, /bd/ (bdfc), /derw/ (d2uydx2),
 c     /blas/ (uxb, uyb, wzb) and /vc/ (varg, dvargdx).  Compile this file
 c     instead
@@ -28,15 +29,6 @@ cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
       integer l
 
       if (c_associated(tbl_ctx_handle)) return
-      ! my_form 0 and 4 (immersed, LES) give the mean mode's top row a
-      ! source from the base flow (duexmdx, uyb; poisson_par.f:145-153)
-      ! that this layer does not pass: refuse them instead of solving a
-      ! different top row
-      if (my_form .eq. 0 .or. my_form .eq. 4) then
-        write(*,*) 'tbl_setup: my_form ', my_form,
-     &             ' not supported (only 1 and 2)'
-        stop
-      end if
       grid%nx = ptsx
       grid%ny = jmax
       grid%nmodes = kfour
@@ -76,6 +68,27 @@ cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
       end
 
 cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
+      subroutine tbl_poisson_form(name)
+
+      ! my_form 0 and 4 (immersed, LES) give poi_uy's mean mode a top-row
+      ! source from the base flow (duexmdx, uyb; poisson_par.f:145-153)
+      ! that this layer does not pass: poi_uy and loop refuse them instead
+      ! of solving a different top row (the derivatives and drv_fi serve
+      ! every form)
+      implicit none
+      include 'par.for'
+      character*(*) name
+
+      if (my_form .eq. 0 .or. my_form .eq. 4) then
+        write(*,*) name, ': my_form ', my_form,
+     &             ' not supported (only 1 and 2)'
+        stop
+      end if
+
+      return
+      end
+
+cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
       subroutine poi_uy(dwzdx, erromax)
 
       ! v-Poisson FAS multigrid for every mode (poisson_par.f:6-64):
@@ -92,6 +105,7 @@ cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
       real(c_double) residual(kfour)
       type(tbl_solve_stats_t) stats
 
+      call tbl_poisson_form('poi_uy')
       call tbl_setup
       rc = tbl_poi_uy(tbl_ctx_handle, wx, dwzdx, uy, erromax, tbl_opts,
      &                cycles, residual, stats)
@@ -209,6 +223,7 @@ cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
       real(c_double) residual(kfour)
       type(tbl_solve_stats_t) stats
 
+      call tbl_poisson_form('loop')
       call tbl_setup
       rc = tbl_loop(tbl_ctx_handle, erro, bdfc, d2uydx2, ux, uy, uz,
      &              wx, wy, wz, tbl_th_ptr(my_form .eq. 2, th),
@@ -243,6 +258,30 @@ cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
       rc = tbl_drv_gv(tbl_ctx_handle, Re, 0, uxb, uyb, wzb, varg,
      &                dvargdx, ux, uy, uz, wx, wy, wz, dvx, dvy, dvz)
       if (rc .ne. 0) call tbl_fail('drv_gv', rc)
+
+      return
+      end
+
+cccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccccc
+      subroutine drv_fi(dvx, dvy, dvz)
+
+      ! the RK right-hand side of the flat-plate form (main_par.f:
+      ! 1264-1336) with nlterms_fi, the reference's active call (kphys
+      ! points in z): dvx, dvy, dvz on rows 2..jmax from COMMON /var/
+      use tbl_iface
+      implicit none
+      include 'par.for'
+      include 'comm.var'
+      real*8 uxb(ptsx,jmax), uyb(ptsx,jmax), wzb(ptsx,jmax)
+      complex*16 dvx(ptsx,jmax,kfour), dvy(ptsx,jmax,kfour),
+     &           dvz(ptsx,jmax,kfour)
+      common/blas/ uxb, uyb, wzb
+      integer(c_int) rc
+
+      call tbl_setup
+      rc = tbl_drv_fi(tbl_ctx_handle, Re, kphys, uxb, uyb, wzb,
+     &                ux, uy, uz, wx, wy, wz, dvx, dvy, dvz)
+      if (rc .ne. 0) call tbl_fail('drv_fi', rc)
 
       return
       end

@@ -138,6 +138,17 @@ module tbl_iface
       complex(c_double_complex), intent(inout) :: dvx(*), dvy(*), dvz(*)
       integer(c_int) :: rc
     end function
+    function tbl_drv_fi(ctx, Re, kphys, uxb, uyb, wzb, ux, uy, uz, wx, wy, wz, dvx, dvy, dvz) &
+        bind(C, name="tbl_drv_fi") result(rc)
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      real(c_double), value :: Re
+      integer(c_int), value :: kphys
+      real(c_double), intent(in) :: uxb(*), uyb(*), wzb(*)
+      complex(c_double_complex), intent(in) :: ux(*), uy(*), uz(*), wx(*), wy(*), wz(*)
+      complex(c_double_complex), intent(inout) :: dvx(*), dvy(*), dvz(*)
+      integer(c_int) :: rc
+    end function
 
   end interface
 
