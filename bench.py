@@ -10,7 +10,8 @@ inputs resident in HBM.
 
 Paths: plain FP64 FAS multigrid (the reference's cycle with the GPU's colour
 line ordering), and FP32 line smoothing under FP64 defect correction started
-from zero, from the learned initial guess (when a trained network is
+from zero (first correction by full multigrid, three pre-sweeps per level),
+from the learned initial guess (when a trained network is
 present, trans_bl_amd/weights/guess_*.bin) and from the previous stage's
 solution (packets 0.4 upstream: the reference's warm start, reported beside).
 `value` is the whole-job mode-solves/s of the mixed path from zero, or from
@@ -39,6 +40,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# the mixed solve from zero: the first correction by full multigrid, three
+# pre-sweeps and one post-sweep per level
+ZERO_START_MIXED = dict(fmg=1, nu1=3)
 # per-kernel HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs
 # (rocprofv3 --pmc ... -- python3 -m trans_bl_amd.prof_kernels, summarized by
 # trans_bl_amd.prof_kernels --summarize)
@@ -319,6 +323,8 @@ def main():
                                                  **kw)
             else:
                 ud.zero_()
+                if prec == 1:
+                    kw.update(ZERO_START_MIXED)
                 cyc, res, st = ctx.solve_rhs_dev(fd.data_ptr(), ud.data_ptr(), a.tol, guess=1, **kw)
             if it >= warm:
                 stats.append((cyc.copy(), res.copy(), st))
@@ -437,6 +443,8 @@ def main():
                    "nx": grid.nx, "ny": grid.ny, "modes": K, "levels": grid.levels,
                    "dx": grid.dx, "dy0": grid.dy0, "stf": grid.stf, "beta": grid.beta,
                    "initial_guess": "learned" if flag_learned else "zero",
+                   "zero_start_cycles": "first correction by full multigrid, then V-cycles; "
+                                        "3 pre-sweeps, 1 post-sweep",
                    "parallelism": "1 GPU, all modes batched"},
         "flagship": sm,
         "plain_fp64": dict(s64, value=round(rate64, 2),

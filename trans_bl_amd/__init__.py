@@ -60,7 +60,7 @@ class CCoefs(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("max_cycles", C.c_int), ("nu1", C.c_int),
                 ("nu2", C.c_int), ("ncoarse", C.c_int), ("use_graph", C.c_int),
-                ("guess", C.c_int), ("reserved", C.c_int * 5)]
+                ("guess", C.c_int), ("fmg", C.c_int), ("reserved", C.c_int * 4)]
 
 
 class SolveStats(C.Structure):
@@ -312,8 +312,8 @@ class Context:
         return a
 
     @staticmethod
-    def opts(precision=0, max_cycles=11, nu1=0, nu2=0, ncoarse=0, use_graph=0, guess=0):
-        return SolveOpts(precision, max_cycles, nu1, nu2, ncoarse, use_graph, guess)
+    def opts(precision=0, max_cycles=11, nu1=0, nu2=0, ncoarse=0, use_graph=0, guess=0, fmg=0):
+        return SolveOpts(precision, max_cycles, nu1, nu2, ncoarse, use_graph, guess, fmg)
 
     # ---- the reference's subroutines on host arrays ----------------------
     def poi_uy(self, wx, dwzdx, uy, erromax, **kw):

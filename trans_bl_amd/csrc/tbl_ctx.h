@@ -50,10 +50,11 @@ struct tbl_ctx {
     // captured V-cycle graphs (precision 0 / 1) and their parameters
     // two cached graphs per precision (the RK loop alternates two
     // tolerances); the least recently used one is replaced
-    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    int graph_key[2][2][5] = {};
-    double graph_tol[2][2] = {{0, 0}, {0, 0}};
-    int graph_last[2] = {0, 0};
+    // (third kind: the mixed cycle with a full-multigrid correction)
+    hipGraphExec_t graph[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    int graph_key[3][2][5] = {};
+    double graph_tol[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    int graph_last[3] = {0, 0, 0};
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 

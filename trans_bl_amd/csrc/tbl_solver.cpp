@@ -462,30 +462,25 @@ int mixed_sweeps(tbl_ctx *c, int l, cx<float> **cur, int n, bool zs, cx<double> 
                                 c->H32[l], c->stream, zs, vup);
 }
 
-// One FP64 defect-correction step with an FP32 correction-scheme V-cycle
-int mixed_cycle(tbl_ctx *c, const Sched &s, double tol)
+// FP32 correction-scheme V-cycle on levels t..L-1 for L e = S32[t]: e[t]
+// holds its start (zero when start_zero -- the staged and one-launch
+// smoothers' first sweep then reads no column it has not relaxed itself,
+// so e[t] need not even be cleared on those shapes); the coarser levels
+// start from zero; vup: level t's last post-sweep adds its e into that FP64
+// field (the update kernel's work)
+int correction_vcycle(tbl_ctx *c, const Sched &s, std::vector<cx<float> *> &e, int t,
+                      bool start_zero, cx<double> *vup)
 {
     hipStream_t st = c->stream;
     const int L = c->L, K = c->K;
-    hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(256), 0, st, c->d_cycle, c->d_resmax, K, c->d_nactive);
-    // FP64 residual of the current v, rounded into the FP32 source
-    defect<double, float>(c->la_d[0], c->V[0], c->S[0], c->S32[0], -1.0, c->d_active, c->d_resmax, st);
-    hipLaunchKernelGGL(decide2_kernel, dim3((K + 255) / 256), dim3(256), 0, st, K, c->d_resmax,
-                       tol, c->d_cycle, c->d_active, c->d_cycles, c->d_lastres, c->d_nactive);
-    // e of each level (the out-of-place sweep moves it between two buffers)
-    std::vector<cx<float> *> e(c->E32);
-    // e starts at zero on every level: the staged and one-launch smoothers'
-    // first sweep reads no column it has not relaxed itself (e's boundary
-    // columns are never written and its wall rows only ever receive zeros,
-    // all zero since allocation); on the other shapes and the coarsest level
-    // e is cleared first (level 1 by a fill, coarser ones by the restriction)
+    // e's boundary columns are never written and its wall rows only ever
+    // receive zeros, all zero since allocation; on the other shapes and the
+    // coarsest level e is cleared first (by the caller or the restriction)
     auto zero_start = [&](int l) {
         return s.nu1 > 0 && l < L - 1 && (c->G32[l] < 0 || shape_is_sweep(c->G32[l]));
     };
-    const bool zs = zero_start(0);
-    if (!zs) convert<float, float>(c->lv[0].plane * K, e[0], e[0], 1, st);
-    for (int l = 0; l < L - 1; ++l) {
-        TRY(mixed_sweeps(c, l, &e[l], s.nu1, zero_start(l), nullptr));
+    for (int l = t; l < L - 1; ++l) {
+        TRY(mixed_sweeps(c, l, &e[l], s.nu1, (l > t || start_zero) && zero_start(l), nullptr));
         defect<float, float>(c->la_f[l], e[l], c->S32[l], c->R32[l], -1.0f, c->d_active,
                              nullptr, st);
         restrict_level<float>(c->lv[l], c->lv[l + 1], K, e[l], c->R32[l], e[l + 1], nullptr,
@@ -494,48 +489,116 @@ int mixed_cycle(tbl_ctx *c, const Sched &s, double tol)
     // coarsest correction in FP32 too, from the zero the restriction left
     TRY(smooth_coarsest<float>(c->la_f[L - 1], e[L - 1], c->S32[L - 1], c->lu_f[L - 1], s.nc,
                                static_cast<float>(rf_of(s.nc)), c->d_active, c->G32[L - 1], st));
-    // v += e: inside the finest level's last post-sweep on the staged and
-    // one-launch shapes (each line adds its final e), else by the update
-    // kernel
-    const bool fused = zs && s.nu2 > 0;
-    for (int l = L - 1; l >= 1; --l) {
+    for (int l = L - 1; l >= t + 1; --l) {
         prolong_level<float>(c->lv[l - 1], c->lv[l], K, e[l - 1], e[l], nullptr, 0, c->d_active,
                              st);
-        TRY(mixed_sweeps(c, l - 1, &e[l - 1], s.nu2, false, fused && l == 1 ? c->V[0] : nullptr));
+        TRY(mixed_sweeps(c, l - 1, &e[l - 1], s.nu2, false, l - 1 == t ? vup : nullptr));
+    }
+    return TBL_OK;
+}
+
+// the finest level's first sweep from e = 0 without clearing e (the
+// staged and one-launch shapes)
+bool finest_zero_start(const tbl_ctx *c, const Sched &s)
+{
+    const int G = c->G32[0];
+    return s.nu1 > 0 && c->L > 1 && (G < 0 || shape_is_sweep(G));
+}
+
+// v += e: inside the finest level's last post-sweep on those shapes (each
+// line adds its final e), else by the update kernel
+bool fused_update(const tbl_ctx *c, const Sched &s)
+{
+    return finest_zero_start(c, s) && s.nu2 > 0;
+}
+
+// One FP64 defect-correction step with an FP32 correction-scheme V-cycle
+int mixed_cycle(tbl_ctx *c, const Sched &s, double tol)
+{
+    hipStream_t st = c->stream;
+    const int K = c->K;
+    hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(256), 0, st, c->d_cycle, c->d_resmax, K, c->d_nactive);
+    // FP64 residual of the current v, rounded into the FP32 source
+    defect<double, float>(c->la_d[0], c->V[0], c->S[0], c->S32[0], -1.0, c->d_active, c->d_resmax, st);
+    hipLaunchKernelGGL(decide2_kernel, dim3((K + 255) / 256), dim3(256), 0, st, K, c->d_resmax,
+                       tol, c->d_cycle, c->d_active, c->d_cycles, c->d_lastres, c->d_nactive);
+    // e of each level (the out-of-place sweep moves it between two buffers)
+    std::vector<cx<float> *> e(c->E32);
+    const bool fused = fused_update(c, s);
+    if (!finest_zero_start(c, s)) convert<float, float>(c->lv[0].plane * K, e[0], e[0], 1, st);
+    TRY(correction_vcycle(c, s, e, 0, true, fused ? c->V[0] : nullptr));
+    if (!fused) update(c->lv[0], K, c->V[0], e[0], c->d_active, st);
+    return TBL_OK;
+}
+
+// The same step with the correction solved by full multigrid (the first
+// cycle of a solve from zero when asked for): the defect restricted to every
+// level, solved on the coarsest, then on each finer level the prolonged
+// coarse solution improved by a correction V-cycle of that level and below,
+// up to the finest
+int mixed_fmg_cycle(tbl_ctx *c, const Sched &s, double tol)
+{
+    hipStream_t st = c->stream;
+    const int L = c->L, K = c->K;
+    hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(256), 0, st, c->d_cycle, c->d_resmax, K, c->d_nactive);
+    defect<double, float>(c->la_d[0], c->V[0], c->S[0], c->S32[0], -1.0, c->d_active, c->d_resmax, st);
+    hipLaunchKernelGGL(decide2_kernel, dim3((K + 255) / 256), dim3(256), 0, st, K, c->d_resmax,
+                       tol, c->d_cycle, c->d_active, c->d_cycles, c->d_lastres, c->d_nactive);
+    std::vector<cx<float> *> e(c->E32);
+    // the source on every level (e of the coarser levels cleared)
+    for (int l = 0; l < L - 1; ++l)
+        restrict_level<float>(c->lv[l], c->lv[l + 1], K, e[l], c->S32[l], e[l + 1], nullptr,
+                              c->S32[l + 1], 0, c->d_active, st);
+    TRY(smooth_coarsest<float>(c->la_f[L - 1], e[L - 1], c->S32[L - 1], c->lu_f[L - 1], s.nc,
+                               static_cast<float>(rf_of(s.nc)), c->d_active, c->G32[L - 1], st));
+    convert<float, float>(c->lv[0].plane * K, e[0], e[0], 1, st);
+    const bool fused = fused_update(c, s);
+    for (int t = L - 2; t >= 0; --t) {
+        prolong_level<float>(c->lv[t], c->lv[t + 1], K, e[t], e[t + 1], nullptr, 0, c->d_active,
+                             st);
+        TRY(correction_vcycle(c, s, e, t, false, t == 0 && fused ? c->V[0] : nullptr));
     }
     if (!fused) update(c->lv[0], K, c->V[0], e[0], c->d_active, st);
     return TBL_OK;
 }
 
-int run_cycle(tbl_ctx *c, int prec, const Sched &s, double tol, bool use_graph)
+// kind 0: FP64 FAS cycle, 1: mixed cycle, 2: mixed cycle with a full-
+// multigrid correction (each kind has its own two cached graphs)
+int cycle_of(tbl_ctx *c, int kind, const Sched &s, double tol)
 {
-    if (!use_graph) return prec ? mixed_cycle(c, s, tol) : fas_cycle(c, s, tol);
+    return kind == 0 ? fas_cycle(c, s, tol) : kind == 1 ? mixed_cycle(c, s, tol)
+                                                        : mixed_fmg_cycle(c, s, tol);
+}
+
+int run_cycle(tbl_ctx *c, int kind, const Sched &s, double tol, bool use_graph)
+{
+    if (!use_graph) return cycle_of(c, kind, s, tol);
     const int key[5] = {s.nu1, s.nu2, s.nc, shapes_key(c->G64), shapes_key(c->G32)};
     int slot = -1;
     for (int q = 0; q < 2; ++q)
-        if (c->graph[prec][q] && std::memcmp(key, c->graph_key[prec][q], sizeof(key)) == 0 &&
-            c->graph_tol[prec][q] == tol)
+        if (c->graph[kind][q] && std::memcmp(key, c->graph_key[kind][q], sizeof(key)) == 0 &&
+            c->graph_tol[kind][q] == tol)
             slot = q;
     if (slot < 0) {
-        slot = 1 - c->graph_last[prec];
-        if (c->graph[prec][slot]) {
-            TBL_HIP(hipGraphExecDestroy(c->graph[prec][slot]));
-            c->graph[prec][slot] = nullptr;
+        slot = 1 - c->graph_last[kind];
+        if (c->graph[kind][slot]) {
+            TBL_HIP(hipGraphExecDestroy(c->graph[kind][slot]));
+            c->graph[kind][slot] = nullptr;
         }
         if (!c->stream) return TBL_ERR_STATE;   // capture needs a non-null stream
         hipGraph_t g;
         TBL_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = prec ? mixed_cycle(c, s, tol) : fas_cycle(c, s, tol);
+        int rc = cycle_of(c, kind, s, tol);
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc != TBL_OK) return rc;
         TBL_HIP(e);
-        TBL_HIP(hipGraphInstantiate(&c->graph[prec][slot], g, nullptr, nullptr, 0));
+        TBL_HIP(hipGraphInstantiate(&c->graph[kind][slot], g, nullptr, nullptr, 0));
         TBL_HIP(hipGraphDestroy(g));
-        std::memcpy(c->graph_key[prec][slot], key, sizeof(key));
-        c->graph_tol[prec][slot] = tol;
+        std::memcpy(c->graph_key[kind][slot], key, sizeof(key));
+        c->graph_tol[kind][slot] = tol;
     }
-    c->graph_last[prec] = slot;
-    TBL_HIP(hipGraphLaunch(c->graph[prec][slot], c->stream));
+    c->graph_last[kind] = slot;
+    TBL_HIP(hipGraphLaunch(c->graph[kind][slot], c->stream));
     return TBL_OK;
 }
 
@@ -552,6 +615,7 @@ int solve_dev(tbl_ctx *c, const double *wx, const double *dz, double *uy, int di
     const int prec = opts ? opts->precision : 0;
     if (prec < 0 || prec > 1) return TBL_ERR_ARG;
     if (opts && (opts->guess < 0 || opts->guess > 3)) return TBL_ERR_ARG;
+    if (opts && (opts->fmg < 0 || opts->fmg > 1)) return TBL_ERR_ARG;
     const bool use_graph = opts && opts->use_graph;
     hipStream_t st = c->stream;
     const int K = c->K;
@@ -571,8 +635,11 @@ int solve_dev(tbl_ctx *c, const double *wx, const double *dz, double *uy, int di
     fill_int(c->d_cycle, 1, 0, st);
     fill(c->d_lastres, K, 0.0, st);
     int done_cycles = 0;
+    // full multigrid needs the zero start (its coarse levels see only the
+    // defect) and the correction scheme of the mixed cycle
+    const bool fmg = opts && opts->fmg && prec == 1 && zero_start && c->L > 1;
     for (int cyc = 1; cyc <= s.maxc; ++cyc) {
-        TRY(run_cycle(c, prec, s, tol, use_graph));
+        TRY(run_cycle(c, cyc == 1 && fmg ? 2 : prec, s, tol, use_graph));
         TBL_HIP(hipGetLastError());
         TBL_HIP(hipMemcpyAsync(c->h_nactive, c->d_nactive, sizeof(int), hipMemcpyDeviceToHost, st));
         TBL_HIP(hipStreamSynchronize(st));

@@ -29,8 +29,8 @@ module tbl_iface
 
   type, bind(C) :: tbl_solve_opts_t
     integer(c_int) :: precision = 0, max_cycles = 0, nu1 = 0, nu2 = 0, ncoarse = 0
-    integer(c_int) :: use_graph = 0, guess = 0
-    integer(c_int) :: reserved(5) = 0
+    integer(c_int) :: use_graph = 0, guess = 0, fmg = 0
+    integer(c_int) :: reserved(4) = 0
   end type
 
   type, bind(C) :: tbl_solve_stats_t

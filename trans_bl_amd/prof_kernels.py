@@ -74,8 +74,11 @@ def main():
     for prec in (1, 0):
         u.zero_()
         torch.cuda.synchronize()
+        # the mixed solve in the benchmark's zero-start form (full-multigrid
+        # first correction, three pre-sweeps)
+        form = dict(fmg=1, nu1=3) if prec == 1 else {}
         cyc, res, st = ctx.solve_rhs_dev(f.data_ptr(), u.data_ptr(), 1e-9, precision=prec,
-                                         max_cycles=40, guess=1, use_graph=0)
+                                         max_cycles=40, guess=1, use_graph=0, **form)
         print(json.dumps({"stage": "solve", "precision": prec, "cycles_max": int(cyc.max()),
                           "residual_max": float(res.max())}), flush=True)
     outs = [torch.empty_like(u) for _ in range(4)]
