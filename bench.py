@@ -10,10 +10,11 @@ inputs resident in HBM.
 
 Paths: plain FP64 FAS multigrid (the reference's cycle with the GPU's colour
 line ordering), and FP32 line smoothing under FP64 defect correction started
-from zero (first correction by full multigrid, three pre-sweeps per level),
-from the learned initial guess (when a trained network is
+from zero, from the learned initial guess (when a trained network is
 present, trans_bl_amd/weights/guess_*.bin) and from the previous stage's
-solution (packets 0.4 upstream: the reference's warm start, reported beside).
+solution (packets 0.4 upstream: the reference's warm start, reported beside);
+the mixed solves correct first by full multigrid and then by V-cycles, with
+three pre-sweeps per level.
 `value` is the whole-job mode-solves/s of the mixed path from zero, or from
 the learned guess when that is faster by more than 1 %; the plain FP64 path
 is reported beside it.
@@ -40,9 +41,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# the mixed solve from zero: the first correction by full multigrid, three
-# pre-sweeps and one post-sweep per level
-ZERO_START_MIXED = dict(fmg=1, nu1=3)
+# the mixed solve (every start): the first correction by full multigrid,
+# three pre-sweeps and one post-sweep per level
+MIXED_FORM = dict(fmg=1, nu1=3)
 # per-kernel HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs
 # (rocprofv3 --pmc ... -- python3 -m trans_bl_amd.prof_kernels, summarized by
 # trans_bl_amd.prof_kernels --summarize)
@@ -226,6 +227,9 @@ def rk4_step_bench(torch, ctx, grid, steps=2):
 
     fp = {k: F[k].data_ptr() for k in names}
     for prec in (0, 1):
+        # the cycle's default form: at the loop's 1e-5 / 1e-6 from the previous
+        # stage the full-multigrid first correction saves a cycle per stage
+        # but costs about as much (38.4 ms per step either way)
         kw = dict(precision=prec, use_graph=1, max_cycles=11)
         timed(lambda: ctx.rk4_step_dev(Re, dt, 0, bp, bd.data_ptr(), d2.data_ptr(), fp, **kw))
         ms, cyc = [], None
@@ -313,6 +317,8 @@ def main():
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
             kw = dict(precision=prec, max_cycles=40, use_graph=1)
+            if prec == 1:
+                kw.update(MIXED_FORM)
             if learned:
                 ud.zero_()
                 cyc, res, st = guess.solve(ctx, fd, ud, a.tol, **kw)
@@ -323,8 +329,6 @@ def main():
                                                  **kw)
             else:
                 ud.zero_()
-                if prec == 1:
-                    kw.update(ZERO_START_MIXED)
                 cyc, res, st = ctx.solve_rhs_dev(fd.data_ptr(), ud.data_ptr(), a.tol, guess=1, **kw)
             if it >= warm:
                 stats.append((cyc.copy(), res.copy(), st))
@@ -443,8 +447,8 @@ def main():
                    "nx": grid.nx, "ny": grid.ny, "modes": K, "levels": grid.levels,
                    "dx": grid.dx, "dy0": grid.dy0, "stf": grid.stf, "beta": grid.beta,
                    "initial_guess": "learned" if flag_learned else "zero",
-                   "zero_start_cycles": "first correction by full multigrid, then V-cycles; "
-                                        "3 pre-sweeps, 1 post-sweep",
+                   "mixed_cycles": "first correction by full multigrid, then V-cycles; "
+                                   "3 pre-sweeps, 1 post-sweep",
                    "parallelism": "1 GPU, all modes batched"},
         "flagship": sm,
         "plain_fp64": dict(s64, value=round(rate64, 2),

@@ -87,12 +87,13 @@ typedef struct tbl_solve_opts {
                             (against the restricted source, after two line
                             sweeps) exceeds 10x uy's or is not finite;
                          3: learned initial guess for every mode (no test) */
-    int fmg;          /* 1 (precision 1 with guess 1 only): the first cycle
-                         solves its correction by full multigrid -- the
+    int fmg;          /* precision 1 only (ignored for 0): 1 -- the first
+                         cycle solves its correction by full multigrid (the
                          defect restricted to every level, solved on the
                          coarsest, then each finer level starts from the
                          prolonged coarser solution and runs one correction
-                         V-cycle; no counterpart in the reference */
+                         V-cycle); 2 -- every cycle does; any start.  No
+                         counterpart in the reference */
     int reserved[4];
 } tbl_solve_opts;
 

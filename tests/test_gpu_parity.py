@@ -174,29 +174,38 @@ def test_mixed_precision_matches_fp64_oracle(torch_mod):
 
 
 @pytest.mark.parametrize("nu1", [0, 3])
-def test_fmg_first_cycle_matches_fp64_oracle(torch_mod, nu1):
-    # the full-multigrid first cycle of a zero-start mixed solve: same
-    # stopping rule and tolerance as the V-cycle path, fewer cycles
+@pytest.mark.parametrize("start", ["zero", "warm"])
+def test_fmg_cycles_match_fp64_oracle(torch_mod, nu1, start):
+    # full-multigrid corrections (first cycle, every cycle) from zero and
+    # from a warm start: same stopping rule and tolerance as the V-cycle
+    # path, fewer cycles, graph replay bit for bit
     g = CFG1
     f = wl.wavepacket_rhs(g, tag="mixed")
     z = np.zeros_like(f)
     ref, _, _ = oracle.poi_uy(oparams(g), z, -f, z, 0.0, max_cycles=25, order=0)
-    kw = dict(precision=1, max_cycles=30, guess=1, nu1=nu1)
     with context(g) as ctx:
-        vc, cv, _, _ = ctx.poi_uy(None, -f, z, 1e-9, **kw)
-        got, cyc, res, st = ctx.poi_uy(None, -f, z, 1e-9, fmg=1, **kw)
-        graph, cg, _, _ = ctx.poi_uy(None, -f, z, 1e-9, fmg=1, use_graph=1, **kw)
-        again, ca, _, _ = ctx.poi_uy(None, -f, z, 1e-9, fmg=1, use_graph=1, **kw)
-        # fmg applies to the zero start only: from uy it changes nothing
-        w0, cw0, _, _ = ctx.poi_uy(None, -f, vc, 1e-9, precision=1, max_cycles=30, nu1=nu1)
-        w1, cw1, _, _ = ctx.poi_uy(None, -f, vc, 1e-9, precision=1, max_cycles=30, nu1=nu1,
-                                   fmg=1)
-    assert st["nconverged"] == g.nmodes and np.all(res < 1e-9)
-    assert rel_l2(got, ref) < 1e-8, rel_l2(got, ref)
-    assert cyc.sum() < cv.sum(), (cyc, cv)
+        u0 = z
+        if start == "warm":
+            # the solution for the packets 0.4 upstream
+            fp = wl.wavepacket_rhs(g, tag="mixed", shift=-0.4)
+            u0, _, _, _ = ctx.poi_uy(None, -fp, z, 1e-9, precision=1, max_cycles=30)
+        kw = dict(precision=1, max_cycles=30, guess=1 if start == "zero" else 0, nu1=nu1)
+        vc, cv, _, _ = ctx.poi_uy(None, -f, u0, 1e-9, **kw)
+        got, cyc, res, st = ctx.poi_uy(None, -f, u0, 1e-9, fmg=1, **kw)
+        graph, cg, _, _ = ctx.poi_uy(None, -f, u0, 1e-9, fmg=1, use_graph=1, **kw)
+        again, ca, _, _ = ctx.poi_uy(None, -f, u0, 1e-9, fmg=1, use_graph=1, **kw)
+        every, ce, re, se = ctx.poi_uy(None, -f, u0, 1e-9, fmg=2, use_graph=1, **kw)
+        # ignored by the FP64 cycle
+        d0, cd0, _, _ = ctx.poi_uy(None, -f, u0, 1e-9, max_cycles=30)
+        d1, cd1, _, _ = ctx.poi_uy(None, -f, u0, 1e-9, max_cycles=30, fmg=2)
+    for out, c, r, s in ((got, cyc, res, st), (every, ce, re, se)):
+        assert s["nconverged"] == g.nmodes and np.all(r < 1e-9)
+        assert rel_l2(out, ref) < 1e-8, rel_l2(out, ref)
+        assert c.sum() < cv.sum(), (c, cv)
+    assert ce.sum() <= cyc.sum(), (ce, cyc)
     assert np.array_equal(got, graph) and np.array_equal(got, again)
     assert np.array_equal(cyc, cg) and np.array_equal(cyc, ca)
-    assert np.array_equal(w0, w1) and np.array_equal(cw0, cw1)
+    assert np.array_equal(d0, d1) and np.array_equal(cd0, cd1)
 
 
 @pytest.mark.parametrize("precision", [0, 1])

@@ -531,11 +531,10 @@ int mixed_cycle(tbl_ctx *c, const Sched &s, double tol)
     return TBL_OK;
 }
 
-// The same step with the correction solved by full multigrid (the first
-// cycle of a solve from zero when asked for): the defect restricted to every
-// level, solved on the coarsest, then on each finer level the prolonged
-// coarse solution improved by a correction V-cycle of that level and below,
-// up to the finest
+// The same step with the correction solved by full multigrid: the defect
+// restricted to every level, solved on the coarsest, then on each finer
+// level the prolonged coarse solution improved by a correction V-cycle of
+// that level and below, up to the finest
 int mixed_fmg_cycle(tbl_ctx *c, const Sched &s, double tol)
 {
     hipStream_t st = c->stream;
@@ -615,7 +614,7 @@ int solve_dev(tbl_ctx *c, const double *wx, const double *dz, double *uy, int di
     const int prec = opts ? opts->precision : 0;
     if (prec < 0 || prec > 1) return TBL_ERR_ARG;
     if (opts && (opts->guess < 0 || opts->guess > 3)) return TBL_ERR_ARG;
-    if (opts && (opts->fmg < 0 || opts->fmg > 1)) return TBL_ERR_ARG;
+    if (opts && (opts->fmg < 0 || opts->fmg > 2)) return TBL_ERR_ARG;
     const bool use_graph = opts && opts->use_graph;
     hipStream_t st = c->stream;
     const int K = c->K;
@@ -635,11 +634,12 @@ int solve_dev(tbl_ctx *c, const double *wx, const double *dz, double *uy, int di
     fill_int(c->d_cycle, 1, 0, st);
     fill(c->d_lastres, K, 0.0, st);
     int done_cycles = 0;
-    // full multigrid needs the zero start (its coarse levels see only the
-    // defect) and the correction scheme of the mixed cycle
-    const bool fmg = opts && opts->fmg && prec == 1 && zero_start && c->L > 1;
+    // full multigrid for the correction of the mixed cycle (a correction
+    // scheme: every cycle solves L e = defect from e = 0), for the first
+    // cycle (fmg 1) or every cycle (fmg 2)
+    const int fmg = opts && prec == 1 && c->L > 1 ? opts->fmg : 0;
     for (int cyc = 1; cyc <= s.maxc; ++cyc) {
-        TRY(run_cycle(c, cyc == 1 && fmg ? 2 : prec, s, tol, use_graph));
+        TRY(run_cycle(c, (cyc == 1 && fmg) || fmg == 2 ? 2 : prec, s, tol, use_graph));
         TBL_HIP(hipGetLastError());
         TBL_HIP(hipMemcpyAsync(c->h_nactive, c->d_nactive, sizeof(int), hipMemcpyDeviceToHost, st));
         TBL_HIP(hipStreamSynchronize(st));
