@@ -328,7 +328,7 @@ def main():
                 cyc, res, st = ctx.solve_rhs_dev(fd.data_ptr(), ud.data_ptr(), a.tol, guess=0,
                                                  **kw)
             else:
-                ud.zero_()
+                # the zero start (guess 1) reads nothing of uy: no clearing
                 cyc, res, st = ctx.solve_rhs_dev(fd.data_ptr(), ud.data_ptr(), a.tol, guess=1, **kw)
             if it >= warm:
                 stats.append((cyc.copy(), res.copy(), st))

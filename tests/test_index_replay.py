@@ -214,3 +214,57 @@ def test_line_rows_cover_rows_1_to_ny_once(G, ny):
     # 193 = 1 + 64 * 3: no idle lane slot at the benchmark's finest level
     if ny == 193 and G == 64:
         assert R == 3
+
+
+# ---- restricting defect (defect_march_kernel<float, float, true>, tbl_mg_kernels.hip)
+K_DEF_ROWS_RS = 58    # kDefRowsRs: rows per wavefront, three halo rows each side
+
+
+def _defect_strip(nx):
+    """defect_strip<float, float>: columns per strip (even)."""
+    return 32 if nx >= 512 else 16
+
+
+@pytest.mark.parametrize("nx,ny", [(1025, 193), (513, 97), (257, 49), (129, 25), (385, 97),
+                                   (193, 49), (129, 33), (2049, 257), (257, 65), (65, 17)])
+def test_restricting_defect_writes_every_coarse_source_point_once(nx, ny):
+    """The restricting defect's wavefronts own rows 1 + 58 c + 0..57 (lanes
+    3..60) and form the defect on lanes 2..61; each strip of columns starts
+    on an odd column and walks one column past its end.  On odd columns i
+    (from the strip's third) a lane of an owned even row writes coarse point
+    ((i-1)/2, j/2) from the defects of rows j-1..j+1 (the top row: j alone)
+    at columns i-2..i.  Replayed: every interior coarse point (columns
+    1..cnx-2, rows 1..cny-1, the points restrict_kernel writes) is written
+    exactly once, from lanes that formed their defect and columns the strip
+    walked, and no load leaves the field."""
+    kH = 3
+    strip = _defect_strip(nx)
+    cnx, cny = (nx - 1) // 2 + 1, (ny - 1) // 2 + 1
+    nstrips = (nx - 2 + strip - 1) // strip
+    jchunks = (ny - 1 + K_DEF_ROWS_RS - 1) // K_DEF_ROWS_RS
+    count = np.zeros((cnx, cny), int)
+    lane = np.arange(64)
+    for sc in range(nstrips):
+        ia = 1 + sc * strip
+        ib0 = min(ia + strip, nx - 1)
+        ib = ib0 + 1 if ib0 < nx - 1 else ib0
+        assert ib <= nx - 1                       # S loads: columns < nx
+        for jc in range(jchunks):
+            j = 1 + jc * K_DEF_ROWS_RS - kH + lane
+            own = (lane >= kH) & (lane < kH + K_DEF_ROWS_RS) & (j >= 1) & (j < ny)
+            calc = (lane >= kH - 1) & (lane <= kH + K_DEF_ROWS_RS) & (j >= 1) & (j < ny)
+            for i in range(ia + 2, ib, 2):
+                assert i & 1
+                ic = (i - 1) >> 1
+                if not 1 <= ic <= cnx - 2:
+                    continue
+                for L in np.nonzero(own & (j % 2 == 0))[0]:
+                    jcr = j[L] >> 1
+                    if not 1 <= jcr <= cny - 1:
+                        continue
+                    used = [L] if j[L] == ny - 1 else [L - 1, L, L + 1]
+                    assert all(0 <= u < 64 and calc[u] for u in used)
+                    count[ic, jcr] += 1
+    expect = np.zeros_like(count)
+    expect[1:cnx - 1, 1:cny] = 1
+    assert np.array_equal(count, expect)

@@ -125,6 +125,11 @@ int smooth_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu
 template <typename T, typename TO>
 void defect(const LevelArgs<T> &L, const cx<T> *V, const cx<T> *S, cx<TO> *out, T si,
             const int *active, double *resmax, hipStream_t st);
+// Sc (the coarser level Cl's source) = the full-weighting restriction of
+// the defect S - L v of level L, in one pass (the defect is not stored)
+template <typename T>
+void defect_restrict(const LevelArgs<T> &L, const Level &Cl, const cx<T> *V, const cx<T> *S,
+                     cx<T> *Sc, const int *active, hipStream_t st);
 template <typename T>
 void restrict_level(const Level &F, const Level &Cl, int K, const cx<T> *Vf, const cx<T> *Rf,
                     cx<T> *Vc, cx<T> *Wc, cx<T> *Sc, int fas, const int *active, hipStream_t st);

@@ -56,16 +56,32 @@ __host__ __device__ inline int defect_strip(int nx)
     return nx >= 512 ? 32 : 16;
 }
 
-template <typename T, typename TO>
+// The restricting form (RS): the defect's full-weighting restriction onto
+// the next coarser level's source instead of the defect itself.  Its
+// wavefronts own kDefRowsRs rows with three halo rows each side, so the
+// defect is formed on one halo row each side as well and every even fine
+// row has its two neighbours' defects in the wavefront; each strip walks
+// one column past its end for the same reason in x.
+constexpr int kDefRowsRs = 58;
+template <typename T>
+struct CoarseOut {
+    cx<T> *S;              // the coarser level's source (nullptr: plain defect)
+    int nx, ny, ld;
+    size_t plane;
+};
+
+template <typename T, typename TO, bool RS = false>
 __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
                                                            const cx<T> *__restrict__ V,
                                                            const cx<T> *S, cx<TO> *out, T si,
                                                            const int *active, double *resmax,
-                                                           int strip, int nstrips, int jchunks)
+                                                           int strip, int nstrips, int jchunks,
+                                                           CoarseOut<TO> co)
 {
     // fused multiply-adds in FP32 and in the mixed defect (FP64 in, FP32
     // out: no reference counterpart); the FP64 defect rounds as written
     constexpr bool kFuse = sizeof(T) == 4 || sizeof(TO) == 4;
+    constexpr int kRows = RS ? kDefRowsRs : kDefRows, kHalo = RS ? 3 : 2;
     const int lane = threadIdx.x & 63;
     const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int k = blockIdx.y;
@@ -73,10 +89,14 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
     if (active && !active[k]) return;
     const int nx = L.nx, ny = L.ny, ld = L.ld;
     const int jc = item % jchunks, sc = item / jchunks;
-    const int j = 1 + jc * kDefRows - 2 + lane;
+    const int j = 1 + jc * kRows - kHalo + lane;
     const bool jin = j >= 0 && j < ny;
-    const bool own = lane >= 2 && lane < 2 + kDefRows && j >= 1 && j < ny;
-    const int ia = 1 + sc * strip, ib = min(ia + strip, nx - 1);
+    const bool own = lane >= kHalo && lane < kHalo + kRows && j >= 1 && j < ny;
+    // rows whose defect is formed: the owned ones, and with RS one more each side
+    const bool calc = RS ? lane >= kHalo - 1 && lane <= kHalo + kRows && j >= 1 && j < ny : own;
+    const int ia = 1 + sc * strip, ib0 = min(ia + strip, nx - 1);
+    // RS: the strip's last even column needs the defect of the next column
+    const int ib = RS && ib0 < nx - 1 ? ib0 + 1 : ib0;
     const size_t pbytes = L.plane * sizeof(cx<T>);
     const __amdgpu_buffer_rsrc_t vr = buf_rsrc(V + static_cast<size_t>(k) * L.plane, pbytes);
     const __amdgpu_buffer_rsrc_t sr = buf_rsrc(S + static_cast<size_t>(k) * L.plane, pbytes);
@@ -85,7 +105,7 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
         return buf_load(vr, ok ? static_cast<unsigned>((i * ld + j) * sizeof(cx<T>)) : kBufOff, T(0));
     };
     auto sload = [&](int i) -> cx<T> {
-        const bool ok = own && i < nx;
+        const bool ok = calc && i < nx;
         return buf_load(sr, ok ? static_cast<unsigned>((i * ld + j) * sizeof(cx<T>)) : kBufOff, T(0));
     };
     const T kk = static_cast<T>(k);
@@ -96,7 +116,7 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
     // per-point divisions were most of the kernel's VALU work
     const T iscx = T(1) / scx, ifac = T(1) / L.fac_y;
     T q = T(0);
-    if (own) q = (j == 1) ? L.qdy2[0] : (j == ny - 2) ? L.qdy2[j - 1] : L.qdy2[j - 2];
+    if (calc) q = (j == 1) ? L.qdy2[0] : (j == ny - 2) ? L.qdy2[j - 1] : L.qdy2[j - 2];
     // interior rows: the row's coefficients folded once --
     // lv = (cp0 xm + cp1 xs + cp2 xp) / scx + (cp3..cp7 . v) q
     //      + k2b2 (cp0 vm1 + cp1 w2 + cp2 vp1) / fac_y
@@ -109,6 +129,8 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
     // win[t] = column i0 - 2 + t of the current block (i0 .. i0+B-1 computed),
     // sq[t] = source column i0 + t; nb / ns: the next block's new columns
     cx<T> win[B + 4], sq[B], nb[B], ns[B];
+    // RS: the (1,2,1) row sums of the defect at the two previous columns
+    cx<T> rc2 = {T(0), T(0)}, rc1 = {T(0), T(0)};
 #pragma unroll
     for (int t = 0; t < B + 4; ++t) win[t] = vload(ia - 2 + t);
 #pragma unroll
@@ -158,7 +180,8 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
             const cx<T> xm = dpp<0x138>(z, xs), xp = dpp<0x130>(z, xs);
             const cx<T> vm1 = dpp<0x138>(z, w2), vm2 = dpp<0x138>(z, vm1);
             const cx<T> vp1 = dpp<0x130>(z, w2), vp2 = dpp<0x130>(z, vp1);
-            if (own) {
+            cx<T> rv = {T(0), T(0)};   // the defect of this point (RS)
+            if (calc) {
                 cx<T> lv;
                 if (j == 1) {
                     const cx<T> b = L.sp[2] * xp + L.sp[1] * xs;
@@ -185,13 +208,34 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
                 }
                 const size_t o = static_cast<size_t>(k) * L.plane + static_cast<size_t>(i) * ld + j;
                 const cx<T> r = mac_f<kFuse>(si, lv, s_cur);   // s + si lv
-                out[o] = cx<TO>{static_cast<TO>(r.re), static_cast<TO>(r.im)};
-                if (resmax) {
+                rv = r;
+                if (!RS) out[o] = cx<TO>{static_cast<TO>(r.re), static_cast<TO>(r.im)};
+                if (!RS && resmax) {
                     // max of |r|^2; one square root per wavefront below
                     // (sqrt is monotone and correctly rounded: same maximum)
                     const double rr = static_cast<double>(r.re), ri = static_cast<double>(r.im);
                     amax = fmax(amax, rr * rr + ri * ri);
                 }
+            }
+            if constexpr (RS) {
+                // restrict_kernel's full weighting, separated: the row sum
+                // (1,2,1) of this column (the top row alone), then on odd i
+                // the column sum (1,2,1) of the three previous row sums
+                // onto coarse column (i-1)/2 (rows of coarse index
+                // 1..cny-2 weighted 1/16, the top row 1/4)
+                const cx<T> rdn = dpp<0x138>(z, rv), rup = dpp<0x130>(z, rv);
+                const bool top = j == ny - 1;
+                const cx<T> rc = top ? rv : rdn + T(2) * rv + rup;
+                const int ic = (i - 1) >> 1, jcr = j >> 1;
+                if ((i & 1) && i >= ia + 2 && own && !(j & 1) && ic >= 1 && ic <= co.nx - 2 &&
+                    jcr >= 1 && jcr <= co.ny - 1) {
+                    const cx<T> t = rc2 + T(2) * rc1 + rc;
+                    const cx<T> sv = (top ? T(0.25) : T(6.25e-2)) * t;
+                    co.S[static_cast<size_t>(k) * co.plane + static_cast<size_t>(ic) * co.ld + jcr] =
+                        cx<TO>{static_cast<TO>(sv.re), static_cast<TO>(sv.im)};
+                }
+                rc2 = rc1;
+                rc1 = rc;
             }
         }
         // slide: the block's last four columns lead the next block's window
@@ -522,7 +566,21 @@ void defect(const LevelArgs<T> &L, const cx<T> *V, const cx<T> *S, cx<TO> *out, 
     const int jchunks = (L.ny - 1 + kDefRows - 1) / kDefRows;
     const int blocks = (nstrips * jchunks + 3) / 4;
     hipLaunchKernelGGL((defect_march_kernel<T, TO>), dim3(blocks, L.K), dim3(256), 0, st, L, V, S,
-                       out, si, active, resmax, strip, nstrips, jchunks);
+                       out, si, active, resmax, strip, nstrips, jchunks, CoarseOut<TO>{});
+}
+
+template <typename T>
+void defect_restrict(const LevelArgs<T> &L, const Level &Cl, const cx<T> *V, const cx<T> *S,
+                     cx<T> *Sc, const int *active, hipStream_t st)
+{
+    const int strip = defect_strip<T, T>(L.nx);   // even: strips start on odd columns
+    const int nstrips = (L.nx - 2 + strip - 1) / strip;
+    const int jchunks = (L.ny - 1 + kDefRowsRs - 1) / kDefRowsRs;
+    const int blocks = (nstrips * jchunks + 3) / 4;
+    const CoarseOut<T> co{Sc, Cl.nx, Cl.ny, Cl.ld, Cl.plane};
+    hipLaunchKernelGGL((defect_march_kernel<T, T, true>), dim3(blocks, L.K), dim3(256), 0, st, L,
+                       V, S, static_cast<cx<T> *>(nullptr), T(-1), active,
+                       static_cast<double *>(nullptr), strip, nstrips, jchunks, co);
 }
 
 template <typename T>
@@ -608,6 +666,8 @@ template void defect<double, float>(const LevelArgs<double> &, const cx<double> 
                                     hipStream_t);
 template void defect<float, float>(const LevelArgs<float> &, const cx<float> *, const cx<float> *,
                                    cx<float> *, float, const int *, double *, hipStream_t);
+template void defect_restrict<float>(const LevelArgs<float> &, const Level &, const cx<float> *,
+                                     const cx<float> *, cx<float> *, const int *, hipStream_t);
 template void restrict_level<double>(const Level &, const Level &, int, const cx<double> *,
                                      const cx<double> *, cx<double> *, cx<double> *, cx<double> *,
                                      int, const int *, hipStream_t);

@@ -481,10 +481,13 @@ int correction_vcycle(tbl_ctx *c, const Sched &s, std::vector<cx<float> *> &e, i
     };
     for (int l = t; l < L - 1; ++l) {
         TRY(mixed_sweeps(c, l, &e[l], s.nu1, (l > t || start_zero) && zero_start(l), nullptr));
-        defect<float, float>(c->la_f[l], e[l], c->S32[l], c->R32[l], -1.0f, c->d_active,
-                             nullptr, st);
-        restrict_level<float>(c->lv[l], c->lv[l + 1], K, e[l], c->R32[l], e[l + 1], nullptr,
-                              c->S32[l + 1], zero_start(l + 1) ? -1 : 0, c->d_active, st);
+        // the coarser source is the restricted defect (one pass, no stored
+        // defect); the coarser e starts from zero -- cleared here unless its
+        // first sweep starts from zero by itself
+        defect_restrict<float>(c->la_f[l], c->lv[l + 1], e[l], c->S32[l], c->S32[l + 1],
+                               c->d_active, st);
+        if (!zero_start(l + 1))
+            convert<float, float>(c->lv[l + 1].plane * K, e[l + 1], e[l + 1], 1, st);
     }
     // coarsest correction in FP32 too, from the zero the restriction left
     TRY(smooth_coarsest<float>(c->la_f[L - 1], e[L - 1], c->S32[L - 1], c->lu_f[L - 1], s.nc,
