@@ -208,6 +208,30 @@ def test_fmg_cycles_match_fp64_oracle(torch_mod, nu1, start):
     assert np.array_equal(d0, d1) and np.array_equal(cd0, cd1)
 
 
+@pytest.mark.parametrize("g", [
+    wl.Grid(129, 33, 4, 2, 0.03, 1e-3, 1.03, 0.2, 0.6),      # two levels
+    SMALL,                                                    # three
+    wl.Grid(257, 65, 6, 4, 0.02, 5e-4, 1.02, 0.0, 0.3),      # four
+    wl.Grid(193, 49, 3, 3, 0.025, 8e-4, 1.025, 0.0, 0.4)],   # odd mode count
+    ids=["129x33x4_L2", "129x33x4_L3", "257x65x6_L4", "193x49x3_L3"])
+def test_fmg_on_other_level_counts(torch_mod, g):
+    # full-multigrid corrections (first cycle, every cycle) on level counts
+    # 2-4: the FP64 stopping rule met, the answer within the mixed path's
+    # 1e-8 of the FP64 cycle's, at most the V-cycle form's largest cycle count
+    f = wl.wavepacket_rhs(g, tag="fmg")
+    z = np.zeros_like(f)
+    with context(g) as ctx:
+        ref, _, _, st64 = ctx.poi_uy(None, -f, z, 1e-10, max_cycles=30)
+        vc, cv, _, _ = ctx.poi_uy(None, -f, z, 1e-9, precision=1, max_cycles=30, guess=1)
+        for fmg in (1, 2):
+            got, cyc, res, st = ctx.poi_uy(None, -f, z, 1e-9, precision=1, max_cycles=30,
+                                           guess=1, fmg=fmg, use_graph=1)
+            assert st["nconverged"] == g.nmodes and np.all(res < 1e-9), (fmg, res)
+            assert rel_l2(got, ref) < 1e-8, (fmg, rel_l2(got, ref))
+            assert cyc.max() <= cv.max(), (fmg, cyc, cv)
+    assert st64["nconverged"] == g.nmodes
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_graph_replay_equals_stream_launches(torch_mod, precision):
     g = SMALL
