@@ -14,7 +14,7 @@ from zero, from the learned initial guess (when a trained network is
 present, trans_bl_amd/weights/guess_*.bin) and from the previous stage's
 solution (packets 0.4 upstream: the reference's warm start, reported beside);
 the mixed solves correct first by full multigrid and then by V-cycles, with
-three pre-sweeps per level.
+three pre-sweeps per level and three coarsest-level sweeps.
 `value` is the whole-job mode-solves/s of the mixed path from zero, or from
 the learned guess when that is faster by more than 1 %; the plain FP64 path
 is reported beside it.
@@ -42,8 +42,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # the mixed solve (every start): the first correction by full multigrid,
-# three pre-sweeps and one post-sweep per level
-MIXED_FORM = dict(fmg=1, nu1=3)
+# three pre-sweeps and one post-sweep per level, three coarsest-level sweeps
+MIXED_FORM = dict(fmg=1, nu1=3, ncoarse=3)
 # per-kernel HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs
 # (rocprofv3 --pmc ... -- python3 -m trans_bl_amd.prof_kernels, summarized by
 # trans_bl_amd.prof_kernels --summarize)
@@ -448,7 +448,7 @@ def main():
                    "dx": grid.dx, "dy0": grid.dy0, "stf": grid.stf, "beta": grid.beta,
                    "initial_guess": "learned" if flag_learned else "zero",
                    "mixed_cycles": "first correction by full multigrid, then V-cycles; "
-                                   "3 pre-sweeps, 1 post-sweep",
+                                   "3 pre-sweeps, 1 post-sweep, 3 coarsest sweeps",
                    "parallelism": "1 GPU, all modes batched"},
         "flagship": sm,
         "plain_fp64": dict(s64, value=round(rate64, 2),

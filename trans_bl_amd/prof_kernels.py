@@ -75,8 +75,8 @@ def main():
         u.zero_()
         torch.cuda.synchronize()
         # the mixed solve in the benchmark's zero-start form (full-multigrid
-        # first correction, three pre-sweeps)
-        form = dict(fmg=1, nu1=3) if prec == 1 else {}
+        # first correction, three pre-sweeps, three coarsest sweeps)
+        form = dict(fmg=1, nu1=3, ncoarse=3) if prec == 1 else {}
         cyc, res, st = ctx.solve_rhs_dev(f.data_ptr(), u.data_ptr(), 1e-9, precision=prec,
                                          max_cycles=40, guess=1, use_graph=0, **form)
         print(json.dumps({"stage": "solve", "precision": prec, "cycles_max": int(cyc.max()),
