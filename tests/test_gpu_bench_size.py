@@ -92,12 +92,13 @@ def test_mixed_precision_against_fp64_oracle(setup, fixed_point):
     ev = max(rel_l2(got[k], ref[k]) for k in MODES)
     print(f"1025x193x64 mixed precision v vs FP64 oracle (modes {MODES}): rel l2 {ev:.2e}")
     assert ev < 1e-8
-    # the benchmark's form: full-multigrid first cycle, three pre-sweeps
+    # the benchmark's form: full-multigrid first cycle, three pre-sweeps,
+    # three coarsest sweeps
     gf, cf, rf, _ = ctx.poi_uy(None, -f, z, 1e-9, precision=1, guess=1, fmg=1, nu1=3,
-                               use_graph=1)
+                               ncoarse=3, use_graph=1)
     assert np.all(rf < 1e-9)
     ef = max(rel_l2(gf[k], ref[k]) for k in MODES)
-    print(f"1025x193x64 mixed precision, fmg, nu1 3: rel l2 {ef:.2e}, cycles "
+    print(f"1025x193x64 mixed precision, fmg, nu1 3, ncoarse 3: rel l2 {ef:.2e}, cycles "
           f"{cf.mean():.2f} (V-cycle start {cyc.mean():.2f})")
     assert ef < 1e-8
     for kind in KINDS:
