@@ -125,7 +125,10 @@ __global__ void __launch_bounds__(256) defect_march_kernel(LevelArgs<T> L,
     const T ax0 = L.cp[0] * iscx, ax1 = L.cp[1] * iscx, ax2 = L.cp[2] * iscx;
     const T cv0 = L.cp[3] * q, cv1 = L.cp[4] * q + L.cp[0] * kf, cv2 = L.cp[5] * q + L.cp[1] * kf;
     const T cv3 = L.cp[6] * q + L.cp[2] * kf, cv4 = L.cp[7] * q;
-    constexpr int B = kDefBlock<T>;
+    // the restricting form holds two more row sums: 4 columns per block keep
+    // it at 96 registers, five wavefronts per SIMD (three at 8; measured
+    // 7.26 -> 7.18 ms per 1025x193x64 mixed solve)
+    constexpr int B = RS ? 4 : kDefBlock<T>;
     // win[t] = column i0 - 2 + t of the current block (i0 .. i0+B-1 computed),
     // sq[t] = source column i0 + t; nb / ns: the next block's new columns
     cx<T> win[B + 4], sq[B], nb[B], ns[B];
