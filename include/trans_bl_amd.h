@@ -371,6 +371,22 @@ tbl_ctx *tbl_batch_context(tbl_batch *batch, int p);
 int tbl_batch_stage(tbl_batch *batch, const double *const *f, double *const *v,
                     double *const *d, double tol, int cycles, int precision, int use_graph,
                     double *ms);
+/* The same stage solved to the tolerance instead of for a fixed cycle count
+   (no counterpart in the reference): the derivatives and the solve's first
+   V-cycles are replayed as one captured hipGraph, then further cycles run
+   only for the problems that still have unconverged modes (a host check of
+   the problems' active-mode counts per cycle), up to opts->max_cycles.  The
+   graph holds as many cycles as the previous call needed (the first call:
+   opts->max_cycles), so a sequence of similar stages launches no masked
+   cycles.  opts: precision, nu1, nu2, ncoarse, fmg as for tbl_poi_uy_dev,
+   max_cycles >= 1 the cap (use_graph and guess are not used: the solve
+   starts from v).  Per mode the result equals tbl_batch_stage's with the
+   same options and max_cycles cycles.  *cycles_run: cycles of the slowest
+   problem; *ms: time of the stage on the device from the graph's launch,
+   host checks included (a re-capture is not). */
+int tbl_batch_stage_converge(tbl_batch *batch, const double *const *f, double *const *v,
+                             double *const *d, double tol, const tbl_solve_opts *opts,
+                             double *ms, int *cycles_run);
 /* Per-mode completed V-cycles and last residual of problem p's last stage. */
 int tbl_batch_results(tbl_batch *batch, int p, int *cycles, double *residual);
 

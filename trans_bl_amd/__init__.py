@@ -82,7 +82,7 @@ EXPORTS = [
     "tbl_net_set_params", "tbl_net_forward", "tbl_net_gradients", "tbl_net_train_step",
     "tbl_net_synchronize", "tbl_mfma_gemm_bf16", "tbl_guess_attach", "tbl_guess_shape",
     "tbl_guess_pairs", "tbl_batch_create", "tbl_batch_destroy", "tbl_batch_context",
-    "tbl_batch_stage", "tbl_batch_results", "tbl_checkpoint_write", "tbl_checkpoint_read",
+    "tbl_batch_stage", "tbl_batch_stage_converge", "tbl_batch_results", "tbl_checkpoint_write", "tbl_checkpoint_read",
     "tbl_baseflow_write", "tbl_baseflow_read", "tbl_poi_ux_dev", "tbl_poi_uz_dev",
     "tbl_wx_wall_dev", "tbl_wz_wall_dev", "tbl_bzone_dev", "tbl_filter_trid_dev",
     "tbl_outuy_dev", "tbl_loop_dev", "tbl_loop", "tbl_bdfc_generate", "tbl_terms_dev",

@@ -18,6 +18,8 @@ def _setup():
         "tbl_batch_context": ([vp, ip], vp),
         "tbl_batch_stage": ([vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), dp, ip, ip, ip,
                              C.POINTER(dp)], C.c_int),
+        "tbl_batch_stage_converge": ([vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), dp, vp,
+                                      C.POINTER(dp), C.POINTER(ip)], C.c_int),
         "tbl_batch_results": ([vp, ip, vp, vp], C.c_int),
     }
     for name, (args, res) in types.items():
@@ -72,6 +74,19 @@ class Batch:
         _check(self.L.tbl_batch_stage(self.h, fa, va, da, tol, cycles, precision, use_graph,
                                       C.byref(ms)), "tbl_batch_stage")
         return ms.value
+
+    def stage_converge(self, f_ptrs, v_ptrs, d_ptrs, tol, max_cycles=12, **form):
+        """The stage solved to tol (tbl_batch_stage_converge); form: precision,
+        nu1, nu2, ncoarse, fmg.  Returns (ms, cycles of the slowest problem)."""
+        n = len(self.grids)
+        fa = (C.c_void_p * n)(*f_ptrs)
+        va = (C.c_void_p * n)(*v_ptrs)
+        da = (C.c_void_p * (4 * n))(*d_ptrs)
+        o = Context.opts(max_cycles=max_cycles, **form)
+        ms, run = C.c_double(), C.c_int()
+        _check(self.L.tbl_batch_stage_converge(self.h, fa, va, da, tol, C.byref(o), C.byref(ms),
+                                               C.byref(run)), "tbl_batch_stage_converge")
+        return ms.value, run.value
 
     def results(self, p):
         K = self.grids[p].nmodes

@@ -61,13 +61,24 @@ def run_uneven(a, torch):
     fp = [t.data_ptr() for t in fs]
     vp = [t.data_ptr() for t in vs]
     dp = [t.data_ptr() for t in ds]
-    times = []
+    # --converge: solved to the tolerance (tbl_batch_stage_converge), with
+    # --form fmg the full-multigrid first correction and three pre-sweeps in
+    # FP32 (the coarsest sweeps by the default rule: the uneven batch's
+    # ny = 257 problems need more than three); else a.cycles fixed V-cycles
+    # (tbl_batch_stage)
+    form = dict(fmg=1, nu1=3) if a.precision == 1 and a.form == "fmg" else {}
+    times, runs = [], []
     for it in range(a.warmup + a.steps):
         for v in vs:
             v.zero_()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        ms = bt.stage(fp, vp, dp, a.tol, a.cycles, a.precision, 1)
+        if a.converge:
+            ms, run = bt.stage_converge(fp, vp, dp, a.tol, max_cycles=a.cycles,
+                                        precision=a.precision, **form)
+            runs.append(run)
+        else:
+            ms = bt.stage(fp, vp, dp, a.tol, a.cycles, a.precision, 1)
         wall = time.perf_counter() - t0
         if it >= a.warmup:
             times.append((ms, wall))
@@ -81,7 +92,9 @@ def run_uneven(a, torch):
            "mode_solves_per_s": round(modes / (ms / 1e3), 1),
            "converged_modes": int(sum(int(np.sum(r < a.tol)) for r in res)),
            "cycles_max": int(max(c.max() for c in cyc)),
-           "precision": a.precision, "fixed_cycles": a.cycles,
+           "precision": a.precision,
+           "cycles": ({"mode": "to the tolerance", "cap": a.cycles, "run_per_call": runs,
+                       "form": form} if a.converge else {"mode": "fixed", "count": a.cycles}),
            "line_shapes": {k: {"fp64": v.get("fp64"), "fp32": v.get("fp32")}
                            for k, v in sorted(tuned.items())},
            "grids": [f"{g.nx}x{g.ny}x{g.nmodes}" for g in grids]}
@@ -135,13 +148,21 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", choices=["uneven32", "large"], default="uneven32")
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed stages (default 1; 2 with --converge, whose second "
+                         "call re-captures its graph for the cycles the first needed)")
     ap.add_argument("--cycles", type=int, default=12)
     ap.add_argument("--stages", type=int, default=200)
     ap.add_argument("--precision", type=int, default=1)
     ap.add_argument("--tol", type=float, default=1e-9)
     ap.add_argument("--learned", action="store_true")
+    ap.add_argument("--converge", action="store_true",
+                    help="uneven32: solve to the tolerance (tbl_batch_stage_converge)")
+    ap.add_argument("--form", choices=["default", "fmg"], default="default",
+                    help="uneven32 --converge: the solver form")
     a = ap.parse_args()
+    if a.warmup is None:
+        a.warmup = 2 if a.converge else 1
     out = run_uneven(a, torch) if a.config == "uneven32" else run_large(a, torch)
     print(json.dumps(out))
 

@@ -22,6 +22,10 @@ using namespace tbl;
 namespace tbl {
 int stage_issue(tbl_ctx *c, const double *f, double *v, double *const *d, double tol,
                 const tbl_solve_opts *opts);
+int stage_begin(tbl_ctx *c, const double *f, double *v, double *const *d, double tol,
+                const tbl_solve_opts *opts, int cycles);
+int stage_cycle(tbl_ctx *c, double tol, const tbl_solve_opts *opts, int cyc);
+int stage_end(tbl_ctx *c, double *v);
 }
 
 struct tbl_batch {
@@ -33,7 +37,44 @@ struct tbl_batch {
     double key_tol = 0.0;
     int key_cycles = 0, key_prec = 0;
     hipEvent_t t0 = nullptr, t1 = nullptr;
+    // the converging stage: its graph (derivatives and the first cycles),
+    // keyed like the fixed one plus the solver form; the cycles it holds
+    // (the previous call's need); per-problem active counts on the host
+    hipGraphExec_t cgraph = nullptr;
+    std::vector<const void *> ckey;
+    double ckey_tol = 0.0;
+    int ckey_form[6] = {};
+    int ckey_base = 0;
+    int cbase = 0;
+    // each problem's active-mode count and the largest per-mode cycle count
+    // (n + 1 values: batch_state_kernel into d_state, read into h_state)
+    struct Probe *d_probe = nullptr;
+    int *d_state = nullptr;
+    int *h_state = nullptr;
 };
+
+// where batch_state_kernel reads a problem's counters
+struct Probe {
+    const int *nactive, *cycles;
+    int K;
+};
+
+// state[p] = problem p's count of active modes at its last convergence
+// check; state[n] = the largest completed-cycle count of any mode
+__global__ void __launch_bounds__(256) batch_state_kernel(const Probe *pr, int n, int *state)
+{
+    __shared__ int need;
+    if (threadIdx.x == 0) need = 0;
+    __syncthreads();
+    int m = 0;
+    for (int p = 0; p < n; ++p) {
+        if (threadIdx.x == 0) state[p] = *pr[p].nactive;
+        for (int k = threadIdx.x; k < pr[p].K; k += blockDim.x) m = max(m, pr[p].cycles[k]);
+    }
+    atomicMax(&need, m);
+    __syncthreads();
+    if (threadIdx.x == 0) state[n] = need;
+}
 
 #define TRYB(x)                        \
     do {                               \
@@ -48,6 +89,10 @@ extern "C" int tbl_batch_destroy(tbl_batch *b)
         if (c) tbl_destroy(c);
     for (auto e : b->ev) (void)hipEventDestroy(e);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->cgraph) (void)hipGraphExecDestroy(b->cgraph);
+    if (b->d_probe) (void)hipFree(b->d_probe);
+    if (b->d_state) (void)hipFree(b->d_state);
+    if (b->h_state) (void)hipHostFree(b->h_state);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     if (b->t0) (void)hipEventDestroy(b->t0);
     if (b->t1) (void)hipEventDestroy(b->t1);
@@ -76,6 +121,16 @@ extern "C" int tbl_batch_create(int n, const tbl_grid *grids, tbl_batch **out)
     }
     if (rc == TBL_OK && (hipEventCreate(&b->t0) != hipSuccess || hipEventCreate(&b->t1) != hipSuccess))
         rc = TBL_ERR_HIP;
+    if (rc == TBL_OK) {
+        std::vector<Probe> pr;
+        for (auto *c : b->ctx) pr.push_back({c->d_nactive, c->d_cycles, c->K});
+        if (hipMalloc(reinterpret_cast<void **>(&b->d_probe), sizeof(Probe) * n) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void **>(&b->d_state), sizeof(int) * (n + 1)) != hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void **>(&b->h_state), sizeof(int) * (n + 1),
+                          hipHostMallocDefault) != hipSuccess ||
+            hipMemcpy(b->d_probe, pr.data(), sizeof(Probe) * n, hipMemcpyHostToDevice) != hipSuccess)
+            rc = TBL_ERR_HIP;
+    }
     if (rc != TBL_OK) {
         tbl_batch_destroy(b);
         return rc;
@@ -161,6 +216,124 @@ extern "C" int tbl_batch_stage(tbl_batch *b, const double *const *f, double *con
         TBL_HIP(hipEventElapsedTime(&t, b->t0, b->t1));
         *ms = t;
     }
+    return TBL_OK;
+}
+
+// The converging stage's first part on the batch stream (fork to every
+// problem's stream and join back): derivatives, source, `cycles` V-cycles.
+static int issue_begin(tbl_batch *b, const double *const *f, double *const *v,
+                       double *const *d, double tol, const tbl_solve_opts *o, int cycles)
+{
+    const int n = static_cast<int>(b->ctx.size());
+    TBL_HIP(hipEventRecord(b->ev[0], b->stream));
+    for (int p = 0; p < n; ++p) {
+        tbl_ctx *c = b->ctx[p];
+        TBL_HIP(hipStreamWaitEvent(c->stream, b->ev[0], 0));
+        TRYB(stage_begin(c, f[p], v[p], d + 4 * p, tol, o, std::min(cycles, o->max_cycles)));
+        TBL_HIP(hipEventRecord(b->ev[1 + p], c->stream));
+        TBL_HIP(hipStreamWaitEvent(b->stream, b->ev[1 + p], 0));
+    }
+    return TBL_OK;
+}
+
+// One RK4 stage of every problem, solved to the tolerance: the derivatives
+// and the solve's first cycles replayed as one captured graph, then further
+// cycles only for the problems that still have unconverged modes (one host
+// check of the problems' active-mode counts per cycle), up to
+// opts->max_cycles; the copy-out last.  The graph holds as many cycles as
+// the previous call needed (the largest per-mode count, plus the check
+// that finds it converged), the first call opts->max_cycles; it is
+// re-captured when that number, the pointers, the tolerance or the solver
+// form change.  Per mode the result is that of tbl_batch_stage with the
+// same options and max_cycles fixed cycles.  *cycles_run: the cycles the
+// slowest problem ran.
+extern "C" int tbl_batch_stage_converge(tbl_batch *b, const double *const *f, double *const *v,
+                                        double *const *d, double tol,
+                                        const tbl_solve_opts *opts, double *ms,
+                                        int *cycles_run)
+{
+    if (!b || !f || !v || !d || !opts) return TBL_ERR_ARG;
+    if (opts->precision < 0 || opts->precision > 1 || opts->fmg < 0 || opts->fmg > 2 ||
+        opts->max_cycles < 1 || opts->nu1 < 0 || opts->nu2 < 0 || opts->ncoarse < 0)
+        return TBL_ERR_ARG;
+    const int n = static_cast<int>(b->ctx.size());
+    const int maxc = opts->max_cycles;
+    std::vector<const void *> key;
+    for (int p = 0; p < n; ++p) {
+        key.push_back(f[p]);
+        key.push_back(v[p]);
+        for (int q = 0; q < 4; ++q) key.push_back(d[4 * p + q]);
+    }
+    const int form[6] = {opts->precision, opts->nu1, opts->nu2, opts->ncoarse, opts->fmg, maxc};
+    if (b->cbase < 1 || std::memcmp(form, b->ckey_form, sizeof(form)) != 0) b->cbase = maxc;
+    const int base = std::min(b->cbase, maxc);
+    if (!b->cgraph || key != b->ckey || tol != b->ckey_tol ||
+        std::memcmp(form, b->ckey_form, sizeof(form)) != 0 || base != b->ckey_base) {
+        if (b->cgraph) {
+            TBL_HIP(hipGraphExecDestroy(b->cgraph));
+            b->cgraph = nullptr;
+        }
+        hipGraph_t g;
+        TBL_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int rc = issue_begin(b, f, v, d, tol, opts, base);
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        TRYB(rc);
+        TBL_HIP(e);
+        TBL_HIP(hipGraphInstantiate(&b->cgraph, g, nullptr, nullptr, 0));
+        TBL_HIP(hipGraphDestroy(g));
+        b->ckey = key;
+        b->ckey_tol = tol;
+        std::memcpy(b->ckey_form, form, sizeof(form));
+        b->ckey_base = base;
+    }
+    // (a capture is host work: *ms starts after it)
+    TBL_HIP(hipEventRecord(b->t0, b->stream));
+    TBL_HIP(hipGraphLaunch(b->cgraph, b->stream));
+    // after the graph and after each further cycle: one read of every
+    // problem's active count (and the cycles needed so far); a further cycle
+    // for each problem whose last check found active modes
+    int run = base;
+    for (;;) {
+        hipLaunchKernelGGL(batch_state_kernel, dim3(1), dim3(256), 0, b->stream, b->d_probe, n,
+                           b->d_state);
+        TBL_HIP(hipMemcpyAsync(b->h_state, b->d_state, sizeof(int) * (n + 1),
+                               hipMemcpyDeviceToHost, b->stream));
+        TBL_HIP(hipStreamSynchronize(b->stream));
+        if (run >= maxc) break;
+        bool any = false;
+        TBL_HIP(hipEventRecord(b->ev[0], b->stream));
+        for (int p = 0; p < n; ++p) {
+            if (b->h_state[p] <= 0) continue;
+            tbl_ctx *c = b->ctx[p];
+            TBL_HIP(hipStreamWaitEvent(c->stream, b->ev[0], 0));
+            TRYB(stage_cycle(c, tol, opts, run + 1));
+            TBL_HIP(hipEventRecord(b->ev[1 + p], c->stream));
+            TBL_HIP(hipStreamWaitEvent(b->stream, b->ev[1 + p], 0));
+            any = true;
+        }
+        if (!any) break;
+        ++run;
+    }
+    // the cycles this call needed (and the check that found them converged):
+    // the graph of the next call holds them
+    b->cbase = std::min(b->h_state[n] + 1, maxc);
+    TBL_HIP(hipEventRecord(b->ev[0], b->stream));
+    for (int p = 0; p < n; ++p) {
+        tbl_ctx *c = b->ctx[p];
+        TBL_HIP(hipStreamWaitEvent(c->stream, b->ev[0], 0));
+        TRYB(stage_end(c, v[p]));
+        TBL_HIP(hipEventRecord(b->ev[1 + p], c->stream));
+        TBL_HIP(hipStreamWaitEvent(b->stream, b->ev[1 + p], 0));
+    }
+    TBL_HIP(hipEventRecord(b->t1, b->stream));
+    TBL_HIP(hipEventSynchronize(b->t1));
+    TBL_HIP(hipGetLastError());
+    if (ms) {
+        float t = 0.f;
+        TBL_HIP(hipEventElapsedTime(&t, b->t0, b->t1));
+        *ms = t;
+    }
+    if (cycles_run) *cycles_run = run;
     return TBL_OK;
 }
 

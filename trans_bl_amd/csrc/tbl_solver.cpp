@@ -857,12 +857,22 @@ namespace tbl {
 // synchronisation, so it can be captured): the four derivatives of v, then
 // the v-Poisson solve with a fixed number of V-cycles, warm-started from v
 // and written back into v.
-int stage_issue(tbl_ctx *c, const double *f, double *v, double *const *d, double tol,
-                const tbl_solve_opts *opts)
+// the cycle kind of cycle cyc (1-based) of a solve with these options
+static int kind_of(const tbl_ctx *c, const tbl_solve_opts *opts, int cyc)
+{
+    const int prec = opts ? opts->precision : 0;
+    const int fmg = opts && prec == 1 && c->L > 1 ? opts->fmg : 0;
+    return (cyc == 1 && fmg) || fmg == 2 ? 2 : prec;
+}
+
+// A stage's first part: the four derivatives of v, the source, the solve's
+// flags and its first `cycles` V-cycles from v (no copy-out), launched on
+// the context's stream (capturable).
+int stage_begin(tbl_ctx *c, const double *f, double *v, double *const *d, double tol,
+                const tbl_solve_opts *opts, int cycles)
 {
     hipStream_t st = c->stream;
     const Sched s = sched_of(opts, c->lv[c->L - 1].ny);
-    const int prec = opts ? opts->precision : 0;
     const int K = c->K;
     TRY(derivs4(c->dargs, v, d[0], d[1], d[2], d[3], st));
     gather_source(c->la_d[0], nullptr, f, v, c->g.beta, c->S[0], c->V[0], 1, st);
@@ -870,9 +880,30 @@ int stage_issue(tbl_ctx *c, const double *f, double *v, double *const *d, double
     fill_int(c->d_cycles, K, 0, st);
     fill_int(c->d_cycle, 1, 0, st);
     fill(c->d_lastres, K, 0.0, st);
-    for (int cyc = 1; cyc <= s.maxc; ++cyc) TRY(prec ? mixed_cycle(c, s, tol) : fas_cycle(c, s, tol));
-    scatter(c->g.nx, c->g.ny, K, c->lv[0], c->V[0], v, st);
+    for (int cyc = 1; cyc <= cycles; ++cyc) TRY(cycle_of(c, kind_of(c, opts, cyc), s, tol));
     return TBL_OK;
+}
+
+// One more V-cycle of a stage's solve (cycle number cyc), replayed from the
+// context's cached graphs.
+int stage_cycle(tbl_ctx *c, double tol, const tbl_solve_opts *opts, int cyc)
+{
+    const Sched s = sched_of(opts, c->lv[c->L - 1].ny);
+    return run_cycle(c, kind_of(c, opts, cyc), s, tol, true);
+}
+
+// A stage's copy-out: v = the solution.
+int stage_end(tbl_ctx *c, double *v)
+{
+    scatter(c->g.nx, c->g.ny, c->K, c->lv[0], c->V[0], v, c->stream);
+    return TBL_OK;
+}
+
+int stage_issue(tbl_ctx *c, const double *f, double *v, double *const *d, double tol,
+                const tbl_solve_opts *opts)
+{
+    TRY(stage_begin(c, f, v, d, tol, opts, sched_of(opts, c->lv[c->L - 1].ny).maxc));
+    return stage_end(c, v);
 }
 
 }  // namespace tbl
