@@ -1,5 +1,7 @@
 // Coarsest-level smoother: every sweep of the level in one launch, one
 // workgroup per mode with the level resident in LDS.
+#include <type_traits>
+
 #include "tbl_relax.h"
 
 namespace tbl {
@@ -184,16 +186,16 @@ __global__ void __launch_bounds__(256) smooth_coarsest_reg_kernel(LevelArgs<T> L
 // in LDS; the factors, the same for every line of a phase, are read from
 // global memory at wavefront-uniform addresses (scalar loads, no LDS
 // traffic in the chains).
-template <typename T>
+template <typename T, bool PART = false>
 __host__ __device__ inline size_t coarse_2s_lds_bytes(int nx, int ny)
 {
     const size_t ls = static_cast<size_t>(coarse_stride(ny));
-    return (2 * ls * nx + ls * phase_lines(1, nx)) * sizeof(cx<T>);
+    return (2 * ls * nx + ls * phase_lines(1, nx)) * (PART ? sizeof(T) : sizeof(cx<T>));
 }
 
-// one part (real or imaginary; element stride 2) of one line; f: the
-// factors' fields, fs apart
-template <typename T, int NY>
+// one part (real or imaginary; element stride ST: 2 in a complex line, 1
+// in a line of one part) of one line; f: the factors' fields, fs apart
+template <typename T, int NY, int ST = 2>
 __device__ __forceinline__ void coarse_chain(const T *rb, T *col, const T *__restrict__ f, int fs,
                                              T rf)
 {
@@ -203,7 +205,7 @@ __device__ __forceinline__ void coarse_chain(const T *rb, T *col, const T *__res
     // y_j = (r_j - l2_j y_{j-2}) - l1_j y_{j-1}
 #pragma unroll
     for (int j = 0; j < NY; ++j) {
-        const T yy = fma(-l1[j], y1, fma(-l2[j], y2, rb[2 * j]));
+        const T yy = fma(-l1[j], y1, fma(-l2[j], y2, rb[ST * j]));
         Y[j] = yy;
         y2 = y1; y1 = yy;
     }
@@ -216,7 +218,7 @@ __device__ __forceinline__ void coarse_chain(const T *rb, T *col, const T *__res
         b = a; a = xx;
     }
 #pragma unroll
-    for (int j = 1; j < NY; ++j) col[2 * j] = fma(rf, Y[j] - col[2 * j], col[2 * j]);
+    for (int j = 1; j < NY; ++j) col[ST * j] = fma(rf, Y[j] - col[ST * j], col[ST * j]);
 }
 
 // threads per workgroup: 512 for NY <= 16, 768 up to 32 rows, 1024 when a
@@ -228,28 +230,41 @@ constexpr int kCoarse2sThreads = 512;
 template <int NY>
 constexpr int kCoarse2sThreadsFor = NY > 32 ? 1024 : NY > 16 ? 768 : kCoarse2sThreads;
 
-template <typename T, int NY>
+// PART: the operator's coefficients are real, so the real and imaginary
+// parts of a mode are two independent solves; with PART a workgroup takes
+// one part (blockIdx.y) of one mode -- twice the workgroups, each with half
+// the right-hand-side work, half the LDS and one thread per line in the
+// substitutions -- with the same operations on each part as the complex
+// form, so the same results bit for bit.
+template <typename T, int NY, bool PART = false>
 __global__ void __launch_bounds__(kCoarse2sThreadsFor<NY>) smooth_coarsest_2s_kernel(LevelArgs<T> L, cx<T> *__restrict__ V,
                                                                   const cx<T> *__restrict__ S,
                                                                   const T *__restrict__ lu,
                                                                   int sweeps, T rf,
                                                                   const int *active)
 {
+    using E = std::conditional_t<PART, T, cx<T>>;   // an LDS element
     constexpr int LS = NY | 1;
     extern __shared__ unsigned char smem_raw[];
     const int k = blockIdx.x;
     if (active && !active[k]) return;
+    const int part = PART ? static_cast<int>(blockIdx.y) : 0;
     const int nx = L.nx, ld = L.ld;
-    cx<T> *sv = reinterpret_cast<cx<T> *>(smem_raw);
-    cx<T> *ss = sv + static_cast<size_t>(LS) * nx;
-    cx<T> *rbuf = ss + static_cast<size_t>(LS) * nx;
+    E *sv = reinterpret_cast<E *>(smem_raw);
+    E *ss = sv + static_cast<size_t>(LS) * nx;
+    E *rbuf = ss + static_cast<size_t>(LS) * nx;
     cx<T> *gv = V + static_cast<size_t>(k) * L.plane;
     const cx<T> *gs = S + static_cast<size_t>(k) * L.plane;
     const T *glu = lu + (static_cast<size_t>(k) * 2) * kLuFields * L.lus;
+    // the workgroup's part of a complex value, and the store back
+    auto get = [&](const cx<T> &z) -> E {
+        if constexpr (PART) return part ? z.im : z.re;
+        else return z;
+    };
     for (int e = threadIdx.x; e < nx * NY; e += blockDim.x) {
         const int i = e / NY, j = e % NY;
-        sv[i * LS + j] = gv[static_cast<size_t>(i) * ld + j];
-        ss[i * LS + j] = gs[static_cast<size_t>(i) * ld + j];
+        sv[i * LS + j] = get(gv[static_cast<size_t>(i) * ld + j]);
+        ss[i * LS + j] = get(gs[static_cast<size_t>(i) * ld + j]);
     }
     __syncthreads();
     const T qx = L.qx;
@@ -271,6 +286,7 @@ __global__ void __launch_bounds__(kCoarse2sThreadsFor<NY>) smooth_coarsest_2s_ke
         cc = L.cp[2];
     }
     const bool grouped = jr >= NY - 2;
+    const E zero = E{};
     for (int it = 0; it < sweeps; ++it) {
         for (int phase = 0; phase < 4; ++phase) {
             const int nl = phase_lines(phase, nx);
@@ -279,13 +295,20 @@ __global__ void __launch_bounds__(kCoarse2sThreadsFor<NY>) smooth_coarsest_2s_ke
                     const int c = e / NY, j = e % NY;
                     const int i = phase_column(0, c, nx);
                     const int side = c == 0 ? -1 : 1;
-                    const cx<T> *col = sv + static_cast<size_t>(i) * LS;
-                    auto X = [&](int jj) { return xoff<T>(col, LS, jj, side, false); };
-                    cx<T> rhs;
+                    const E *col = sv + static_cast<size_t>(i) * LS;
+                    // xoff's border form: 10 a - 4 b + 14 c - 6 d + e
+                    auto X = [&](int jj) -> E {
+                        const int s = side < 0 ? LS : -LS;
+                        return mac(T(-6), col[jj + 3 * s],
+                                   mac(T(14), col[jj + 2 * s],
+                                       mac(T(-4), col[jj + s], T(10) * col[jj - s]))) +
+                               col[jj + 4 * s];
+                    };
+                    E rhs;
                     if (j == 0) {
                         rhs = col[0];
                     } else {
-                        const cx<T> sj = ss[static_cast<size_t>(i) * LS + j];
+                        const E sj = ss[static_cast<size_t>(i) * LS + j];
                         if (j == 1) rhs = sj - qx * (L.sp[2] * X(2) + L.sp[1] * X(1));
                         else if (j <= NY - 3)
                             rhs = sj - qx * (L.cp[0] * X(j - 1) + L.cp[1] * X(j) + L.cp[2] * X(j + 1));
@@ -300,17 +323,24 @@ __global__ void __launch_bounds__(kCoarse2sThreadsFor<NY>) smooth_coarsest_2s_ke
                 for (int q = 0; q < ncg; ++q) {
                     const int c = cs + q * NG;
                     const int i = 2 + (phase - 1) + 3 * min(c, nl - 1);
-                    const cx<T> *col = sv + static_cast<size_t>(i) * LS;
-                    cx<T> x = {T(0), T(0)};
+                    const E *col = sv + static_cast<size_t>(i) * LS;
+                    E x = zero;
                     if (jr < NY) {
-                        const cx<T> m2 = col[jr - 2 * LS], m1 = col[jr - LS];
-                        const cx<T> p1 = col[jr + LS], p2 = col[jr + 2 * LS];
+                        const E m2 = col[jr - 2 * LS], m1 = col[jr - LS];
+                        const E p1 = col[jr + LS], p2 = col[jr + 2 * LS];
                         x = grouped ? -(p2 + m2) + T(16) * (p1 + m1)
                                     : T(-1) * m2 + T(16) * m1 + T(16) * p1 - T(1) * p2;
                     }
-                    const cx<T> xm = shfl_up(x, 1, RW), xp = shfl_down(x, 1, RW);
+                    E xm, xp;
+                    if constexpr (PART) {
+                        xm = __shfl_up(x, 1, RW);
+                        xp = __shfl_down(x, 1, RW);
+                    } else {
+                        xm = shfl_up(x, 1, RW);
+                        xp = shfl_down(x, 1, RW);
+                    }
                     if (c < nl && jr < NY) {
-                        const cx<T> rhs =
+                        const E rhs =
                             jr == 0 ? col[0]
                                     : ss[static_cast<size_t>(i) * LS + jr] -
                                           qx * ((ca * xm + cb * x) + cc * xp);
@@ -319,20 +349,31 @@ __global__ void __launch_bounds__(kCoarse2sThreadsFor<NY>) smooth_coarsest_2s_ke
                 }
             }
             __syncthreads();
-            // stage 2: two threads (real, imaginary part) per line
-            const int c = threadIdx.x >> 1, part = threadIdx.x & 1;
+            // stage 2: the substitutions, one thread per line and part
+            const int c = PART ? static_cast<int>(threadIdx.x) : static_cast<int>(threadIdx.x >> 1);
+            const int pp = PART ? 0 : static_cast<int>(threadIdx.x & 1);
             if (c < nl) {
                 const int i = phase_column(phase, c, nx);
-                coarse_chain<T, NY>(reinterpret_cast<const T *>(rbuf + c * LS) + part,
-                                    reinterpret_cast<T *>(sv + static_cast<size_t>(i) * LS) + part,
-                                    glu + (phase == 0 ? kLuFields * L.lus : 0), L.lus, rf);
+                const T *fb = glu + (phase == 0 ? kLuFields * L.lus : 0);
+                if constexpr (PART)
+                    coarse_chain<T, NY, 1>(rbuf + c * LS, sv + static_cast<size_t>(i) * LS, fb,
+                                           L.lus, rf);
+                else
+                    coarse_chain<T, NY>(reinterpret_cast<const T *>(rbuf + c * LS) + pp,
+                                        reinterpret_cast<T *>(sv + static_cast<size_t>(i) * LS) + pp,
+                                        fb, L.lus, rf);
             }
             __syncthreads();
         }
     }
     for (int e = threadIdx.x; e < nx * NY; e += blockDim.x) {
         const int i = e / NY, j = e % NY;
-        gv[static_cast<size_t>(i) * ld + j] = sv[i * LS + j];
+        if constexpr (PART) {
+            T *g = reinterpret_cast<T *>(gv + static_cast<size_t>(i) * ld + j) + part;
+            *g = sv[i * LS + j];
+        } else {
+            gv[static_cast<size_t>(i) * ld + j] = sv[i * LS + j];
+        }
     }
 }
 
@@ -363,11 +404,12 @@ int smooth_coarsest(const LevelArgs<T> &L, cx<T> *V, const cx<T> *S, const T *lu
     G = sweep_shape(G);
     if (reg_ny && coarse_2s_lds_bytes<T>(L.nx, L.ny) <= 160 * 1024 &&
         2 * phase_lines(1, L.nx) <= kCoarse2sThreads) {
-        const size_t shm = coarse_2s_lds_bytes<T>(L.nx, L.ny);
+        // one workgroup per (mode, real or imaginary part)
+        const size_t shm = coarse_2s_lds_bytes<T, true>(L.nx, L.ny);
         bool done = false;
 #define X(NY)                                                                                 \
     if (!done && L.ny == NY) {                                                                \
-        hipLaunchKernelGGL((smooth_coarsest_2s_kernel<T, NY>), dim3(L.K),                  \
+        hipLaunchKernelGGL((smooth_coarsest_2s_kernel<T, NY, true>), dim3(L.K, 2),            \
                            dim3(kCoarse2sThreadsFor<NY>),                                     \
                            shm, st,                                                           \
                            L, V, S, lu, sweeps, rf, active);                                  \
@@ -438,7 +480,9 @@ int coarse_kernel_attributes()
 #undef X
 #define X(NY)                                                                                 \
     for (const void *f : {reinterpret_cast<const void *>(&smooth_coarsest_2s_kernel<double, NY>), \
-                          reinterpret_cast<const void *>(&smooth_coarsest_2s_kernel<float, NY>)}) { \
+                          reinterpret_cast<const void *>(&smooth_coarsest_2s_kernel<float, NY>), \
+                          reinterpret_cast<const void *>(&smooth_coarsest_2s_kernel<double, NY, true>), \
+                          reinterpret_cast<const void *>(&smooth_coarsest_2s_kernel<float, NY, true>)}) { \
         hipError_t r = allow_full_lds(f);                                                     \
         if (r != hipSuccess) e = r;                                                           \
     }
