@@ -38,8 +38,8 @@ struct tbl_batch {
     int key_cycles = 0, key_prec = 0;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     // the converging stage: its graph (derivatives and the first cycles),
-    // keyed like the fixed one plus the solver form; the cycles it holds
-    // (the previous call's need); per-problem active counts on the host
+    // keyed like the fixed one plus the solver form and the cycles it holds
+    // (ckey_base); cbase: the cycles the next call's graph is to hold
     hipGraphExec_t cgraph = nullptr;
     std::vector<const void *> ckey;
     double ckey_tol = 0.0;
