@@ -641,14 +641,22 @@ int solve_dev(tbl_ctx *c, const double *wx, const double *dz, double *uy, int di
     // scheme: every cycle solves L e = defect from e = 0), for the first
     // cycle (fmg 1) or every cycle (fmg 2)
     const int fmg = opts && prec == 1 && c->L > 1 ? opts->fmg : 0;
+    // The host reads the active-mode count after each cycle and stops at
+    // zero; each read idles the device for a round trip.  The first cycles
+    // run back to back without it: as many as the previous solve of this
+    // context ran before its last two (modes stop on the device, so a solve
+    // that converges earlier only launches some cycles that do nothing).
+    const int ahead = std::min(std::max(c->last_done - 2, 0), s.maxc);
     for (int cyc = 1; cyc <= s.maxc; ++cyc) {
         TRY(run_cycle(c, (cyc == 1 && fmg) || fmg == 2 ? 2 : prec, s, tol, use_graph));
         TBL_HIP(hipGetLastError());
+        done_cycles = cyc;
+        if (cyc <= ahead && cyc < s.maxc) continue;
         TBL_HIP(hipMemcpyAsync(c->h_nactive, c->d_nactive, sizeof(int), hipMemcpyDeviceToHost, st));
         TBL_HIP(hipStreamSynchronize(st));
-        done_cycles = cyc;
         if (*c->h_nactive == 0) break;
     }
+    c->last_done = done_cycles;
     (void)done_cycles;
     scatter(c->g.nx, c->g.ny, K, F, c->V[0], uy, st);
     TBL_HIP(hipGetLastError());
