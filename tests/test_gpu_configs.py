@@ -104,6 +104,18 @@ def test_large_stage_mixed_against_fp64(torch_mod):
               f"rel l2 {e:.2e}")
         assert res32.max() < 1e-9 and res64.max() < 1e-9
         assert e < 1e-8
+        # the full-multigrid first correction with three pre-sweeps (the
+        # coarsest sweeps by the default rule: coarsest ny 33 here)
+        vf = torch.zeros_like(f)
+        cycf, resf, _ = ctx.solve_rhs_dev(f.data_ptr(), vf.data_ptr(), 1e-9, precision=1,
+                                          max_cycles=40, use_graph=1, guess=1, fmg=1, nu1=3)
+        torch.cuda.synchronize()
+        ef = rel_l2(vf.cpu().numpy(), v64.cpu().numpy())
+        print(f"  mixed, fmg, nu1 3: cycles {cycf.mean():.2f} (max {cycf.max()}), residual "
+              f"{resf.max():.2e}; rel l2 {ef:.2e}")
+        assert resf.max() < 1e-9 and ef < 1e-8
+        assert cycf.max() <= cyc32.max()
+        del vf
         # derivatives of the solution: the long x-lines (2049 points) and y
         dx = torch.zeros_like(f)
         dy = torch.zeros_like(f)
