@@ -122,6 +122,8 @@ def run_large(a, torch):
             ctx.deriv_dev(q, v.data_ptr(), ds[q].data_ptr())
         ctx.synchronize()
         kw = dict(precision=a.precision, max_cycles=40, use_graph=1)
+        if a.precision == 1 and a.form == "fmg":
+            kw.update(fmg=1, nu1=3)
         if guess is not None:
             cyc, res, st = ctx.solve_rhs_dev(f.data_ptr(), v.data_ptr(), a.tol, guess=2, **kw)
         else:
@@ -138,7 +140,7 @@ def run_large(a, torch):
             "residual_max": float(max(s[1] for s in stats)),
             "fallback_total": int(sum(s[2] for s in stats)),
             "initial_guess": "learned" if guess is not None else "previous stage",
-            "precision": a.precision,
+            "precision": a.precision, "form": a.form,
             "line_shapes": {k: {"fp64": v.get("fp64"), "fp32": v.get("fp32")}
                             for k, v in sorted(tuned.items())}}
 
@@ -159,7 +161,8 @@ def main():
     ap.add_argument("--converge", action="store_true",
                     help="uneven32: solve to the tolerance (tbl_batch_stage_converge)")
     ap.add_argument("--form", choices=["default", "fmg"], default="default",
-                    help="uneven32 --converge: the solver form")
+                    help="the solver form (uneven32: with --converge; fmg: full-multigrid "
+                         "first correction, three pre-sweeps)")
     a = ap.parse_args()
     if a.warmup is None:
         a.warmup = 2 if a.converge else 1
