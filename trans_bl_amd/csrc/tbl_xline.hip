@@ -342,9 +342,12 @@ __global__ void outflow_row_kernel(cxd *uy, int nx, int ny, int K)
 
 size_t xline_lds(int R) { return sizeof(cxd) * 64 * static_cast<size_t>(R); }
 
-// rows per lane compiled for (up to 576-point lines); longer lines run the
-// run-time R form
-#define TBL_XLR(X) X(1) X(2) X(3) X(5) X(9)
+// rows per lane compiled for (up to 576-point lines, and the 1025-point
+// lines of the pentadiagonal solve and the integration: 2-4 % faster there
+// on 1025x193x64, while the filter's run-time form stayed faster); longer
+// lines run the run-time R form
+#define TBL_XLR_F(X) X(1) X(2) X(3) X(5) X(9)
+#define TBL_XLR(X) TBL_XLR_F(X) X(17)
 
 int xpenta(const XPenta &a, hipStream_t st)
 {
@@ -375,7 +378,7 @@ int xfilter(const XFilter &a, hipStream_t st)
     const int lines = 3 * a.K * (a.ny - 1);
 #define X(RR) \
     if (a.R == RR) { hipLaunchKernelGGL(xfilter_kernel<RR>, dim3(lines), dim3(64), xline_lds(a.R), st, a); return TBL_OK; }
-    TBL_XLR(X)
+    TBL_XLR_F(X)
 #undef X
     hipLaunchKernelGGL(xfilter_kernel<0>, dim3(lines), dim3(64), xline_lds(a.R), st, a);
     return TBL_OK;
