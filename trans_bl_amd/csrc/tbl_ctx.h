@@ -39,7 +39,7 @@ struct tbl_ctx {
     int *d_active = nullptr, *d_cycles = nullptr, *d_nactive = nullptr, *d_cycle = nullptr;
     double *d_resmax = nullptr, *d_lastres = nullptr;
     int *h_nactive = nullptr;     // pinned
-    int last_done = 0;            // cycles the last solve launched
+    int last_need = 0;            // cycles the last solve needed (0: it did not converge)
 
     // derivatives
     double *d_qdy = nullptr, *d_thomas = nullptr, *d_tseg = nullptr;

@@ -644,9 +644,11 @@ int solve_dev(tbl_ctx *c, const double *wx, const double *dz, double *uy, int di
     // The host reads the active-mode count after each cycle and stops at
     // zero; each read idles the device for a round trip.  The first cycles
     // run back to back without it: as many as the previous solve of this
-    // context ran before its last two (modes stop on the device, so a solve
-    // that converges earlier only launches some cycles that do nothing).
-    const int ahead = std::min(std::max(c->last_done - 2, 0), s.maxc);
+    // context needed before its last two (modes stop on the device, so a
+    // solve that converges earlier only launches some cycles that do
+    // nothing; after a solve that did not converge, none).
+    const int ahead = std::min(std::max(c->last_need - 2, 0), s.maxc);
+    bool all = false;
     for (int cyc = 1; cyc <= s.maxc; ++cyc) {
         TRY(run_cycle(c, (cyc == 1 && fmg) || fmg == 2 ? 2 : prec, s, tol, use_graph));
         TBL_HIP(hipGetLastError());
@@ -654,9 +656,11 @@ int solve_dev(tbl_ctx *c, const double *wx, const double *dz, double *uy, int di
         if (cyc <= ahead && cyc < s.maxc) continue;
         TBL_HIP(hipMemcpyAsync(c->h_nactive, c->d_nactive, sizeof(int), hipMemcpyDeviceToHost, st));
         TBL_HIP(hipStreamSynchronize(st));
-        if (*c->h_nactive == 0) break;
+        if (*c->h_nactive == 0) {
+            all = true;
+            break;
+        }
     }
-    c->last_done = done_cycles;
     (void)done_cycles;
     scatter(c->g.nx, c->g.ny, K, F, c->V[0], uy, st);
     TBL_HIP(hipGetLastError());
@@ -666,6 +670,11 @@ int solve_dev(tbl_ctx *c, const double *wx, const double *dz, double *uy, int di
     TBL_HIP(hipMemcpyAsync(cyc.data(), c->d_cycles, sizeof(int) * K, hipMemcpyDeviceToHost, st));
     TBL_HIP(hipMemcpyAsync(res.data(), c->d_lastres, sizeof(double) * K, hipMemcpyDeviceToHost, st));
     TBL_HIP(hipStreamSynchronize(st));
+    // the cycles this solve needed: the last mode's count plus the check
+    // that found it converged
+    int need = 0;
+    for (int k = 0; k < K; ++k) need = std::max(need, cyc[k]);
+    c->last_need = all ? need + 1 : 0;
     if (cycles) std::memcpy(cycles, cyc.data(), sizeof(int) * K);
     if (residual) std::memcpy(residual, res.data(), sizeof(double) * K);
     if (stats) {
